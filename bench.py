@@ -1,0 +1,150 @@
+"""Benchmark: homogenization solves per second on one MI355X.
+
+Workload (one "step"): a 256^3 periodic design (seeded randc density, seed 0, rotate3 symmetry,
+volume 0.2), E = 10, nu = 0.3, SIMP x = rho^3, Emin = 1e-9.  A step uploads nothing: the density is
+resident in HBM.  It runs Homogenization::update (Galerkin coarse operators + coarsest FP64
+factorisation), the six macro-strain load cases by MG-PCG from a zero initial guess to
+relres = ||f - K u|| / ||f|| <= 1e-5 (FP64 norms), and C^H.
+
+Prints one JSON line.  The dominant kernel's average launch time is measured with HIP events on the
+library's stream during the timed steps; the CPU baseline is the FP64 oracle (test
+infrastructure) timed on the host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; stencil GB/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, n, R=6, s_u=4, s_f=4, s_w=4):
+    """Algorithmic HBM bytes per launch of the finest-level kernels (FP32 storage)."""
+    N = n ** 3
+    if name == "fine_apply":      # y = K u: read u, write y, read w
+        return N * (3 * R * (s_u + s_u) + s_w)
+    if name == "fine_residual":   # r = f - K u: read u, f, write r, read w
+        return N * (3 * R * (s_u + s_f + s_u) + s_w)
+    if name == "fine_gs":         # one colour = 1/8 of an in-place sweep: u read+write, f read, w read
+        return N * (3 * R * (2 * s_u + s_f) + s_w) / 8.0
+    raise KeyError(name)
+
+
+def cpu_baseline(n_sample, n_target, tol):
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    rho = oracle.randc_density(n_sample, 0, 0.2, "rotate3").astype(np.float64)
+    t0 = time.perf_counter()
+    mg = oracle.MG(n_sample)
+    mg.set_density(rho ** 3)
+    C, chi, code, its = mg.homogenize(tol=tol, maxit=400, mode="pcg")
+    dt = time.perf_counter() - t0
+    scale = (n_sample / n_target) ** 3
+    return {
+        "value": (1.0 / dt) * scale,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": ("FP64 oracle, full solve at %d^3 (Galerkin set-up + 6 load cases, MG-PCG to %.0e, iters %s) "
+                   "in %.1f s on %d threads; scaled by (%d/%d)^3 to %d^3" %
+                   (n_sample, tol, its, dt, threads, n_sample, n_target, n_target)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import homo3d_amd
+
+    n = args.n
+    log("bench: n=%d tol=%.1e steps=%d warmup=%d" % (n, args.tol, args.steps, args.warmup))
+    rho = homo3d_amd.randc_density(n, 0, 0.2, "rotate3")
+    x = torch.from_numpy((rho.astype(np.float64) ** 3).astype(np.float32)).to("cuda:0")
+    torch.cuda.synchronize()
+    hom = homo3d_amd.Homogenization(homo3d_amd.HomoConfig(reso=n, solver="pcg", max_iter=400))
+
+    def step():
+        hom.update(x.data_ptr(), n, on_device=True)
+        st = hom.solve(args.tol, warm=False)
+        C = hom.elasticMatrix(solve=False)
+        return st, C
+
+    for i in range(args.warmup):
+        st, C = step()
+        log("warmup %d: iters %s relres %.2e %.2fs" % (i, st["iters"], max(st["relres"]), st["seconds"]))
+    hom.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for i in range(args.steps):
+        st, C = step()
+        stats.append(st)
+        log("step %d: iters %s relres %.2e" % (i, st["iters"], max(st["relres"])))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = {k: hom.profile_get(k) for k in ("fine_gs", "fine_apply", "fine_residual")}
+    hom.profile_enable(False)
+    for k, (cnt, ms) in prof.items():
+        if cnt:
+            log("kernel %-14s launches %6d total %9.2f ms avg %8.4f ms  %.1f GB/s algorithmic" %
+                (k, cnt, ms, ms / cnt, kernel_bytes(k, n) / (ms / cnt * 1e-3) / 1e9))
+    dom = max(prof, key=lambda k: prof[k][1])
+    cnt, ms = prof[dom]
+    avg_s = ms / cnt * 1e-3
+    achieved = kernel_bytes(dom, n) / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "avg_launch_ms": round(ms / cnt, 5), "algorithmic_bytes_per_launch": kernel_bytes(dom, n)}
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps / dt, 5),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "%d^3 periodic homogenization: 6 macro-strain load cases, plain MG-PCG "
+                               "(V(1,1) 8-colour block GS, FP64 coarsest) to relres %.0e, seeded randc density "
+                               "(seed 0, rotate3, vol 0.2), E=10 nu=0.3 p=3" % (n, args.tol),
+                   "n": n, "tol": args.tol, "solver": "mg-pcg", "warm_start": "none",
+                   "iters": stats[-1]["iters"], "relres": [float("%.3e" % r) for r in stats[-1]["relres"]]},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("cpu baseline: oracle at %d^3" % args.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, n, args.tol)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,306 @@
+"""ctypes binding of include/homo3d.h and the reference-named Python façade."""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+HOMO3D_OK, HOMO3D_NOT_CONVERGED, HOMO3D_DIVERGED, HOMO3D_BAD_ARG, HOMO3D_HIP_ERROR, HOMO3D_NO_SUPPORT = range(6)
+SOLVER_MG, SOLVER_PCG = 0, 1
+FIELD_U, FIELD_F, FIELD_R = 0, 1, 2
+SYM = {"reflect3": 0, "reflect6": 1, "rotate3": 2, "none": 3}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libhomo3d.so")
+_lib = None
+
+
+class Homo3dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("homo3d status %d: %s" % (code, msg))
+        self.code = code
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("reso", ctypes.c_int * 3), ("youngs_modulus", ctypes.c_double), ("poisson_ratio", ctypes.c_double),
+                ("emin", ctypes.c_double), ("fem_rel_thres", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("solver", ctypes.c_int), ("device", ctypes.c_int), ("sor_omega", ctypes.c_double),
+                ("diag_clamp", ctypes.c_double)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int * 6), ("relres", ctypes.c_double * 6),
+                ("seconds", ctypes.c_double)]
+
+
+EXPORTS = {
+    "homo3d_default_config": (None, [ctypes.POINTER(_Config)]),
+    "homo3d_create": (ctypes.c_int, [ctypes.POINTER(_Config), ctypes.POINTER(ctypes.c_void_p)]),
+    "homo3d_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "homo3d_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+    "homo3d_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "homo3d_update_density": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "homo3d_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_Stats)]),
+    "homo3d_elastic_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "homo3d_sensitivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                          ctypes.c_int, ctypes.c_int]),
+    "homo3d_field_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_int]),
+    "homo3d_macro_rhs": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_apply_K": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_smooth": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "homo3d_restrict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_prolong": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_coarse_solve": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_vcycle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_get_stencil": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "homo3d_symmetrize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int]),
+    "homo3d_oc_filter": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "homo3d_oc_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int]),
+    "homo3d_randc_density": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_void_p]),
+    "homo3d_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_profile_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long),
+                                          ctypes.POINTER(ctypes.c_double)]),
+}
+
+
+def lib():
+    """Load libhomo3d.so (raises when it has not been built: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(lib_path):
+            raise ImportError("libhomo3d.so not built: run `python -m homo3d_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(lib_path)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def randc_density(n, seed, vol, sym="reflect6"):
+    out = np.zeros(int(n) ** 3, dtype=np.float32)
+    rc = lib().homo3d_randc_density(int(n), int(seed), float(vol), SYM[sym] if isinstance(sym, str) else int(sym),
+                                    _ptr(out))
+    if rc:
+        raise Homo3dError(rc, "randc_density")
+    return out
+
+
+class HomoConfig:
+    """cfg::HomoConfig (cmdline.h:39-60) fields used by the solve path, with the reference defaults
+    (cmdline.cpp:4-27: E = 10, nu = 0.3, relthres = 1e-2, reso 64)."""
+
+    def __init__(self, reso=64, youngsModulu=10.0, poissonRatio=0.3, femRelThres=1e-2, max_iter=200,
+                 solver="mg", emin=1e-9, device=0, sor_omega=1.0, diag_clamp=1e-4, **kw):
+        self.reso = [reso] * 3 if np.isscalar(reso) else list(reso)
+        self.youngsModulu = youngsModulu
+        self.poissonRatio = poissonRatio
+        self.femRelThres = femRelThres
+        self.max_iter = max_iter
+        self.solver = solver
+        self.emin = emin
+        self.device = device
+        self.sor_omega = sor_omega
+        self.diag_clamp = diag_clamp
+        # optimisation-side fields of cfg::HomoConfig (kept for drivers)
+        self.volRatio = kw.get("volRatio", 0.3)
+        self.filterRadius = kw.get("filterRadius", 2.0)
+        self.designStep = kw.get("designStep", 0.05)
+        self.dampRatio = kw.get("dampRatio", 0.5)
+        self.finthres = kw.get("finthres", 5e-4)
+        self.sym = kw.get("sym", "reflect6")
+
+    def _c(self):
+        c = _Config()
+        lib().homo3d_default_config(ctypes.byref(c))
+        for i in range(3):
+            c.reso[i] = int(self.reso[i])
+        c.youngs_modulus = self.youngsModulu
+        c.poisson_ratio = self.poissonRatio
+        c.emin = self.emin
+        c.fem_rel_thres = self.femRelThres
+        c.max_iter = self.max_iter
+        c.solver = SOLVER_PCG if self.solver == "pcg" else SOLVER_MG
+        c.device = self.device
+        c.sor_omega = self.sor_omega
+        c.diag_clamp = self.diag_clamp
+        return c
+
+
+class Homogenization:
+    """homo::Homogenization (homogenization/homogenization.h:17-61) over the C ABI."""
+
+    def __init__(self, config):
+        self.config = config
+        self._h = ctypes.c_void_p()
+        c = config._c()
+        rc = lib().homo3d_create(ctypes.byref(c), ctypes.byref(self._h))
+        if rc:
+            raise Homo3dError(rc, "homo3d_create(reso=%r)" % (config.reso,))
+        buf = (ctypes.c_int * 16)()
+        nl = lib().homo3d_levels(self._h, buf, 16)
+        self.sizes = [buf[i] for i in range(nl)]
+        self.n = self.sizes[0]
+        self.N = self.n ** 3
+        self.last_stats = None
+
+    def close(self):
+        if self._h:
+            lib().homo3d_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what, allow=()):
+        if rc and rc not in allow:
+            raise Homo3dError(rc, "%s: %s" % (what, lib().homo3d_last_error(self._h).decode()))
+        return rc
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return lib().homo3d_stream(self._h)
+
+    # --- Homogenization API ---
+    def update(self, rho, pitchT=-1, on_device=False):
+        """Homogenization::update(float* rho, int pitchT): rho holds x = rho^p per cell."""
+        if on_device:
+            ptr, pitch = rho, pitchT
+        else:
+            a = np.ascontiguousarray(rho, dtype=np.float32).ravel()
+            pitch = self.n if pitchT < 0 else pitchT
+            if a.size < pitch * self.n * self.n:
+                raise ValueError("density array too small for pitch %d" % pitch)
+            self._rho = a
+            ptr = _ptr(a)
+        self._chk(lib().homo3d_update_density(self._h, ptr, int(pitch), int(bool(on_device))), "update")
+
+    def solve(self, tol=None, warm=True):
+        st = _Stats()
+        rc = lib().homo3d_solve(self._h, float(tol if tol is not None else self.config.femRelThres), int(warm),
+                                ctypes.byref(st))
+        self._chk(rc, "solve", allow=(HOMO3D_NOT_CONVERGED,))
+        self.last_stats = {"status": st.status, "iters": list(st.iters), "relres": list(st.relres),
+                           "seconds": st.seconds}
+        return self.last_stats
+
+    def elasticMatrix(self, solve=True, tol=None, warm=True):
+        """Homogenization::elasticMatrix(double C[6][6]): solves the 6 load cases, returns C^H."""
+        if solve:
+            self.solve(tol, warm)
+        C = (ctypes.c_double * 36)()
+        self._chk(lib().homo3d_elastic_matrix(self._h, C), "elasticMatrix")
+        return np.array(C[:]).reshape(6, 6)
+
+    def Sensitivity(self, dC, pitchT=-1):
+        """Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder=true): returns dJ/dx per cell."""
+        d = np.ascontiguousarray(dC, dtype=np.float32).ravel()
+        pitch = self.n if pitchT < 0 else pitchT
+        out = np.zeros(pitch * self.n * self.n, dtype=np.float32)
+        self._chk(lib().homo3d_sensitivity(self._h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ptr(out),
+                                           int(pitch), 0), "Sensitivity")
+        return out
+
+    # --- Grid / MG level operations (all 6 load cases) ---
+    def level_shape(self, level):
+        return (6, 3, self.sizes[level] ** 3)
+
+    def get_field(self, level, field):
+        out = np.zeros(self.level_shape(level), dtype=np.float32)
+        self._chk(lib().homo3d_field_copy(self._h, level, field, _ptr(out), 0), "field_copy")
+        return out
+
+    def set_field(self, level, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(self.level_shape(level))
+        self._chk(lib().homo3d_field_copy(self._h, level, field, _ptr(a), 1), "field_copy")
+
+    def macro_rhs(self):
+        self._chk(lib().homo3d_macro_rhs(self._h), "macro_rhs")
+
+    def residual(self, level=0):
+        self._chk(lib().homo3d_residual(self._h, level), "residual")
+
+    def apply_K(self):
+        self._chk(lib().homo3d_apply_K(self._h), "apply_K")
+
+    def smooth(self, level=0, omega=1.0, reverse=False):
+        self._chk(lib().homo3d_smooth(self._h, level, float(omega), int(reverse)), "smooth")
+
+    def restrict(self, level):
+        self._chk(lib().homo3d_restrict(self._h, level), "restrict")
+
+    def prolong(self, level):
+        self._chk(lib().homo3d_prolong(self._h, level), "prolong")
+
+    def coarse_solve(self):
+        self._chk(lib().homo3d_coarse_solve(self._h), "coarse_solve")
+
+    def vcycle(self, symmetric=False):
+        self._chk(lib().homo3d_vcycle(self._h, int(symmetric)), "vcycle")
+
+    def stencil(self, level):
+        out = np.zeros((self.sizes[level] ** 3, 27, 3, 3), dtype=np.float32)
+        self._chk(lib().homo3d_get_stencil(self._h, level, _ptr(out)), "get_stencil")
+        return out
+
+    # --- density utilities on this grid ---
+    def symmetrize(self, field, sym):
+        a = np.ascontiguousarray(field, dtype=np.float32).ravel().copy()
+        self._chk(lib().homo3d_symmetrize(self._h, _ptr(a), self.n, SYM[sym] if isinstance(sym, str) else int(sym),
+                                          0), "symmetrize")
+        return a
+
+    def profile_enable(self, on=True):
+        self._chk(lib().homo3d_profile_enable(self._h, int(on)), "profile_enable")
+
+    def profile_get(self, kernel):
+        cnt = ctypes.c_long()
+        ms = ctypes.c_double()
+        self._chk(lib().homo3d_profile_get(self._h, kernel.encode(), ctypes.byref(cnt), ctypes.byref(ms)),
+                  "profile_get")
+        return cnt.value, ms.value
+
+
+class OCOptimizer:
+    """homo::OCOptimizer (optimization/OCOptimizer.h:7-20): filterSens and update on the GPU."""
+
+    def __init__(self, hom, min_density=0.001, stepLimit=0.05, dampExponent=0.5):
+        self.hom = hom
+        self.minRho = min_density
+        self.step_limit = stepLimit
+        self.damp = dampExponent
+
+    def filterSens(self, sens, rho, radius=2.0, periodic=False):
+        s = np.ascontiguousarray(sens, dtype=np.float32).ravel().copy()
+        r = np.ascontiguousarray(rho, dtype=np.float32).ravel()
+        self.hom._chk(lib().homo3d_oc_filter(self.hom.handle, _ptr(s), _ptr(r), self.hom.n, float(radius),
+                                             int(periodic), 0), "oc_filter")
+        return s
+
+    def update(self, sens, rho, volratio):
+        s = np.ascontiguousarray(sens, dtype=np.float32).ravel()
+        r = np.ascontiguousarray(rho, dtype=np.float32).ravel().copy()
+        self.hom._chk(lib().homo3d_oc_update(self.hom.handle, _ptr(s), _ptr(r), self.hom.n, float(volratio),
+                                             float(self.minRho), float(self.step_limit), float(self.damp), 0),
+                      "oc_update")
+        return r

@@ -1,0 +1,948 @@
+// Host side of the homo3d C ABI: hierarchy set-up, V-cycle, stand-alone MG and MG-PCG drivers,
+// C^H / sensitivity and density updates.  Every device operation is a kernel of kernels.hip
+// launched on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/homo3d.h"
+#include "kernels.h"
+
+using namespace homo3d;
+
+namespace {
+
+constexpr int kMaxLevels = 16;
+constexpr int kRhs = 6;
+constexpr int kDotBlocks = 1024;
+
+struct Level {
+    int n = 0;
+    size_t N = 0;
+    float* u = nullptr;
+    float* f = nullptr;
+    float* r = nullptr;
+    float* st = nullptr;  // levels >= 1: [243][N]
+};
+
+struct ProfEvent {
+    int name = -1;
+    int a = -1, b = -1;  // indices into the handle's event pool
+};
+
+const char* const kProfNames[] = {"fine_gs", "fine_apply", "fine_residual"};
+constexpr int kProfKinds = 3;
+constexpr size_t kEventPool = 8192;
+
+}  // namespace
+
+struct homo3d_handle {
+    homo3d_config cfg{};
+    int nl = 0;
+    Level lv[kMaxLevels];
+    hipStream_t stream = nullptr;
+    float* d_ke = nullptr;
+    float* d_kchi = nullptr;
+    float* d_G = nullptr;
+    float* d_w = nullptr;
+    float* d_stage = nullptr;   // pitched staging buffer for host densities / sensitivities
+    size_t stage_elems = 0;
+    double* d_Ainv = nullptr;
+    int mc = 0;
+    // solution fields (warm start) and PCG work vectors, level-0 sized, 6 load cases
+    float* X = nullptr;
+    float* F = nullptr;
+    float* P = nullptr;
+    float* Q = nullptr;
+    float* Z = nullptr;
+    float* Rr = nullptr;
+    double* d_partial = nullptr;
+    double* d_sc = nullptr;
+    double* d_tmp = nullptr;
+    float* d_ftmp = nullptr;
+    int* d_gperm = nullptr;
+    int* d_gflip = nullptr;
+    double ke[576];
+    bool density_set = false;
+    std::string err;
+    bool prof = false;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    std::vector<ProfEvent> pending;
+    long prof_count[kProfKinds] = {0, 0, 0};
+    double prof_ms[kProfKinds] = {0, 0, 0};
+};
+
+namespace {
+
+int set_err(homo3d_handle* h, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                               \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) return set_err(h, HOMO3D_HIP_ERROR, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// Element matrix (templateMatrix.cpp:63-166): 2x2x2 Gauss quadrature of B^T C B on the unit hex.
+// ---------------------------------------------------------------------------------------------
+void element_matrix(double E, double nu, double* ke) {
+    const double lam = nu * E / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+    double C[6][6] = {};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) C[i][j] = lam;
+    for (int i = 0; i < 3; i++) C[i][i] += 2 * mu;
+    for (int i = 3; i < 6; i++) C[i][i] = mu;
+    std::vector<double> K(576, 0.0);
+    const double g = std::sqrt(1.0 / 3.0);
+    for (int q = 0; q < 8; q++) {
+        const double pt[3] = {(1 + (2 * (q & 1) - 1) * g) / 2, (1 + (2 * ((q >> 1) & 1) - 1) * g) / 2,
+                              (1 + (2 * (q >> 2) - 1) * g) / 2};
+        double B[6][24] = {};
+        for (int b = 0; b < 8; b++) {
+            const int id[3] = {b & 1, (b >> 1) & 1, b >> 2};
+            double lin[3], dn[3];
+            for (int k = 0; k < 3; k++) lin[k] = id[k] ? pt[k] : 1 - pt[k];
+            for (int k = 0; k < 3; k++) dn[k] = (id[k] ? 1.0 : -1.0) * lin[(k + 1) % 3] * lin[(k + 2) % 3];
+            for (int k = 0; k < 3; k++) B[k][3 * b + k] = dn[k];
+            B[3][3 * b + 1] = dn[2];
+            B[3][3 * b + 2] = dn[1];
+            B[4][3 * b + 0] = dn[2];
+            B[4][3 * b + 2] = dn[0];
+            B[5][3 * b + 0] = dn[1];
+            B[5][3 * b + 1] = dn[0];
+        }
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) {
+                double s = 0;
+                for (int a = 0; a < 6; a++)
+                    for (int c = 0; c < 6; c++) s += B[a][i] * C[a][c] * B[c][j];
+                K[i * 24 + j] += s * 0.125;
+            }
+    }
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 24; j++) ke[i * 24 + j] = 0.5 * (K[i * 24 + j] + K[j * 24 + i]);
+}
+
+double macro_disp(int node, int s, int c) {
+    const double px = node & 1, py = (node >> 1) & 1, pz = node >> 2;
+    switch (s) {
+        case 0: return c == 0 ? px : 0;
+        case 1: return c == 1 ? py : 0;
+        case 2: return c == 2 ? pz : 0;
+        case 3: return c == 1 ? pz / 2 : (c == 2 ? py / 2 : 0);
+        case 4: return c == 0 ? pz / 2 : (c == 2 ? px / 2 : 0);
+        default: return c == 0 ? py / 2 : (c == 1 ? px / 2 : 0);
+    }
+}
+
+double hat(int o) { return o == 0 ? 1.0 : ((o == 1 || o == -1) ? 0.5 : 0.0); }
+
+// Level sizes (Grid::buildRoot / coarse2, grid.cpp:21-151) in integer arithmetic.
+int level_sizes(int n, int* sizes) {
+    if (n < 8 || n > 512) return -1;
+    int fl = 0;
+    while ((1 << (fl + 1)) <= n) fl++;
+    const int k = std::max(fl - 2, 0);
+    if (n % (1 << k) != 0) return -1;  // only sizes c * 2^k are solved as requested
+    const int c = n >> k;
+    for (int l = 0; l <= k; l++) sizes[l] = c << (k - l);
+    return k + 1;
+}
+
+// Host FP64 Cholesky (lower) and explicit inverse of the pinned coarsest system.
+bool cholesky_inverse(int m, std::vector<double>& A) {
+    for (int j = 0; j < m; j++) {
+        double s = A[(size_t)j * m + j];
+        for (int k = 0; k < j; k++) s -= A[(size_t)j * m + k] * A[(size_t)j * m + k];
+        if (!(s > 0)) return false;
+        const double l = std::sqrt(s);
+        A[(size_t)j * m + j] = l;
+        for (int i = j + 1; i < m; i++) {
+            double t = A[(size_t)i * m + j];
+            for (int k = 0; k < j; k++) t -= A[(size_t)i * m + k] * A[(size_t)j * m + k];
+            A[(size_t)i * m + j] = t / l;
+        }
+    }
+    // Linv (lower) in place of L
+    std::vector<double> Li((size_t)m * m, 0.0);
+    for (int j = 0; j < m; j++) {
+        Li[(size_t)j * m + j] = 1.0 / A[(size_t)j * m + j];
+        for (int i = j + 1; i < m; i++) {
+            double s = 0;
+            for (int k = j; k < i; k++) s -= A[(size_t)i * m + k] * Li[(size_t)k * m + j];
+            Li[(size_t)i * m + j] = s / A[(size_t)i * m + i];
+        }
+    }
+    // A^{-1} = Linv^T Linv
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j <= i; j++) {
+            double s = 0;
+            for (int k = i; k < m; k++) s += Li[(size_t)k * m + i] * Li[(size_t)k * m + j];
+            A[(size_t)i * m + j] = s;
+            A[(size_t)j * m + i] = s;
+        }
+    return true;
+}
+
+// Symmetry groups (TensorExpression.cuh:72-262) in the canonical order shared with the checker:
+// element = (perm, flips), out[k] = flip_k ? n-1-p[perm[k]] : p[perm[k]].
+const int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+const int kPermSign[6] = {1, -1, -1, 1, 1, -1};
+
+void symmetry_group(int sym, std::vector<int>& perm, std::vector<int>& flip) {
+    perm.clear();
+    flip.clear();
+    auto add = [&](int p, int f) {
+        perm.push_back(kPerm[p][0] | (kPerm[p][1] << 2) | (kPerm[p][2] << 4));
+        flip.push_back(f);
+    };
+    if (sym == HOMO3D_SYM_REFLECT3) {
+        for (int f = 0; f < 8; f++) add(0, f);
+    } else if (sym == HOMO3D_SYM_REFLECT6) {
+        for (int p = 0; p < 6; p++)
+            for (int f = 0; f < 8; f++) add(p, f);
+    } else if (sym == HOMO3D_SYM_ROTATE3) {
+        for (int p = 0; p < 6; p++)
+            for (int f = 0; f < 8; f++) {
+                const int nflip = (f & 1) + ((f >> 1) & 1) + (f >> 2);
+                if (kPermSign[p] * ((nflip & 1) ? -1 : 1) == 1) add(p, f);
+            }
+    }
+}
+
+void prof_flush(homo3d_handle* h) {
+    if (h->pending.empty()) return;
+    hipStreamSynchronize(h->stream);
+    for (auto& ev : h->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, h->pool[ev.a], h->pool[ev.b]);
+        h->prof_count[ev.name] += 1;
+        h->prof_ms[ev.name] += ms;
+    }
+    h->pending.clear();
+    h->pool_used = 0;
+}
+
+// Event pair around one launch of a profiled kernel kind (0 fine_gs, 1 fine_apply, 2 fine_residual).
+void prof_begin(homo3d_handle* h, int kind, ProfEvent& ev) {
+    if (!h->prof) return;
+    if (h->pool_used + 2 > h->pool.size()) prof_flush(h);
+    while (h->pool.size() < h->pool_used + 2 && h->pool.size() < kEventPool) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        h->pool.push_back(e);
+    }
+    if (h->pool_used + 2 > h->pool.size()) return;
+    ev.name = kind;
+    ev.a = (int)h->pool_used++;
+    ev.b = (int)h->pool_used++;
+    hipEventRecord(h->pool[ev.a], h->stream);
+}
+
+void prof_end(homo3d_handle* h, ProfEvent& ev) {
+    if (!h->prof || ev.name < 0) return;
+    hipEventRecord(h->pool[ev.b], h->stream);
+    h->pending.push_back(ev);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Level operations
+// ---------------------------------------------------------------------------------------------
+void op_smooth(homo3d_handle* h, int l, float* u, const float* f, float omega, int reverse) {
+    const Level& L = h->lv[l];
+    for (int cc = 0; cc < 8; cc++) {
+        const int c = reverse ? 7 - cc : cc;
+        if (l == 0) {
+            ProfEvent ev;
+            prof_begin(h, 0, ev);
+            launch_fine_gs(h->stream, L.n, c, omega, reverse, h->d_ke, h->d_w, u, f);
+            prof_end(h, ev);
+        } else {
+            launch_stencil_gs(h->stream, L.n, c, omega, reverse, L.st, u, f);
+        }
+    }
+}
+
+void op_residual(homo3d_handle* h, int l, const float* u, const float* f, float* r) {
+    const Level& L = h->lv[l];
+    if (l == 0) {
+        ProfEvent ev;
+        prof_begin(h, 2, ev);
+        launch_fine_apply(h->stream, L.n, h->d_ke, h->d_w, u, f, r);
+        prof_end(h, ev);
+    } else {
+        launch_stencil_apply(h->stream, L.n, L.st, u, f, r);
+    }
+}
+
+// One V(1,1) cycle on the level-0 pair (u0, f0) (MG::v_cycle, MG.cpp:46-70).  When rn2 is given it
+// receives, per load case, the squared norm of the level-0 residual after pre-smoothing.
+void vcycle(homo3d_handle* h, float* u0, const float* f0, int symmetric, double* rn2) {
+    const float omega = (float)h->cfg.sor_omega;
+    for (int l = 0; l < h->nl - 1; l++) {
+        Level& L = h->lv[l];
+        float* u = l == 0 ? u0 : L.u;
+        const float* f = l == 0 ? f0 : L.f;
+        if (l > 0) hipMemsetAsync(u, 0, sizeof(float) * 3 * kRhs * L.N, h->stream);
+        op_smooth(h, l, u, f, omega, 0);
+        op_residual(h, l, u, f, L.r);
+        if (l == 0 && rn2) launch_dot6(h->stream, 3 * L.N, L.r, L.r, h->d_partial, kDotBlocks, rn2);
+        launch_restrict(h->stream, L.n, L.r, h->lv[l + 1].f);
+    }
+    const Level& Lc = h->lv[h->nl - 1];
+    launch_coarse_solve(h->stream, Lc.n, h->d_Ainv, Lc.f, Lc.u);
+    for (int l = h->nl - 2; l >= 0; l--) {
+        Level& L = h->lv[l];
+        float* u = l == 0 ? u0 : L.u;
+        const float* f = l == 0 ? f0 : L.f;
+        launch_prolong(h->stream, L.n, h->lv[l + 1].u, u, l == 0 ? 1 : 0);
+        op_smooth(h, l, u, f, omega, symmetric ? 1 : 0);
+    }
+}
+
+int ensure_stage(homo3d_handle* h, size_t elems) {
+    if (h->stage_elems >= elems) return HOMO3D_OK;
+    if (h->d_stage) hipFree(h->d_stage);
+    h->d_stage = nullptr;
+    HIPCHK(h, hipMalloc(&h->d_stage, sizeof(float) * elems));
+    h->stage_elems = elems;
+    return HOMO3D_OK;
+}
+
+// Bring a pitched lexicographic array to the device: returns a device pointer (the caller's when
+// on_device, else the staging buffer).
+int stage_in(homo3d_handle* h, const float* a, int pitch, int on_device, const float** dptr) {
+    const int n = h->lv[0].n;
+    if (on_device) {
+        *dptr = a;
+        return HOMO3D_OK;
+    }
+    const size_t elems = (size_t)pitch * n * n;
+    int rc = ensure_stage(h, elems);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->d_stage, a, sizeof(float) * elems, hipMemcpyHostToDevice, h->stream));
+    *dptr = h->d_stage;
+    return HOMO3D_OK;
+}
+
+bool check_handle(homo3d_handle* h) { return h != nullptr; }
+
+int require_density(homo3d_handle* h) {
+    if (!h->density_set) return set_err(h, HOMO3D_BAD_ARG, "density not set: call homo3d_update_density first");
+    return HOMO3D_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+void homo3d_default_config(homo3d_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->reso[0] = c->reso[1] = c->reso[2] = 64;
+    c->youngs_modulus = 10.0;
+    c->poisson_ratio = 0.3;
+    c->emin = 1e-9;
+    c->fem_rel_thres = 1e-2;
+    c->max_iter = 200;
+    c->solver = HOMO3D_SOLVER_MG;
+    c->device = 0;
+    c->sor_omega = 1.0;
+    c->diag_clamp = 1e-4;
+}
+
+int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
+    if (!cfg || !out) return HOMO3D_BAD_ARG;
+    *out = nullptr;
+    const int n = cfg->reso[0];
+    if (cfg->reso[1] != n || cfg->reso[2] != n) return HOMO3D_NO_SUPPORT;
+    int sizes[kMaxLevels];
+    const int nl = level_sizes(n, sizes);
+    if (nl < 2) return HOMO3D_NO_SUPPORT;
+    if (!(cfg->poisson_ratio > -1 && cfg->poisson_ratio < 0.5) || !(cfg->youngs_modulus > 0)) return HOMO3D_BAD_ARG;
+    homo3d_handle* h = new homo3d_handle();
+    h->cfg = *cfg;
+    h->nl = nl;
+    auto fail = [&](int code) {
+        homo3d_destroy(h);
+        return code;
+    };
+    if (hipSetDevice(cfg->device) != hipSuccess) return fail(HOMO3D_HIP_ERROR);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(HOMO3D_HIP_ERROR);
+    for (int l = 0; l < nl; l++) {
+        Level& L = h->lv[l];
+        L.n = sizes[l];
+        L.N = (size_t)L.n * L.n * L.n;
+        const size_t bytes = sizeof(float) * 3 * kRhs * L.N;
+        if (hipMalloc(&L.u, bytes) || hipMalloc(&L.f, bytes) || hipMalloc(&L.r, bytes)) return fail(HOMO3D_HIP_ERROR);
+        hipMemset(L.u, 0, bytes);
+        hipMemset(L.f, 0, bytes);
+        hipMemset(L.r, 0, bytes);
+        if (l > 0 && hipMalloc(&L.st, sizeof(float) * 243 * L.N)) return fail(HOMO3D_HIP_ERROR);
+    }
+    const size_t N0 = h->lv[0].N, vb = sizeof(float) * 3 * kRhs * N0;
+    if (hipMalloc(&h->X, vb) || hipMalloc(&h->F, vb) || hipMalloc(&h->P, vb) || hipMalloc(&h->Q, vb) ||
+        hipMalloc(&h->Z, vb) || hipMalloc(&h->Rr, vb) || hipMalloc(&h->d_w, sizeof(float) * N0))
+        return fail(HOMO3D_HIP_ERROR);
+    hipMemset(h->X, 0, vb);
+    hipMemset(h->F, 0, vb);
+    if (hipMalloc(&h->d_partial, sizeof(double) * 21 * kDotBlocks) || hipMalloc(&h->d_sc, sizeof(double) * 64) ||
+        hipMalloc(&h->d_tmp, sizeof(double) * 64) || hipMalloc(&h->d_ftmp, sizeof(float) * 4096))
+        return fail(HOMO3D_HIP_ERROR);
+    // element matrix, macro loads and the level-1 Galerkin table
+    element_matrix(cfg->youngs_modulus, cfg->poisson_ratio, h->ke);
+    std::vector<float> kef(576), kchi(6 * 24), G(64 * 27 * 9, 0.f);
+    for (int i = 0; i < 576; i++) kef[i] = (float)h->ke[i];
+    for (int s = 0; s < 6; s++)
+        for (int i = 0; i < 24; i++) {
+            double t = 0;
+            for (int b = 0; b < 8; b++)
+                for (int c = 0; c < 3; c++) t += h->ke[i * 24 + 3 * b + c] * macro_disp(b, s, c);
+            kchi[s * 24 + i] = (float)t;
+        }
+    for (int c = 0; c < 64; c++) {
+        const int ox = (c & 3) - 2, oy = ((c >> 2) & 3) - 2, oz = (c >> 4) - 2;
+        for (int D = 0; D < 27; D++) {
+            const int Dx = D % 3 - 1, Dy = D / 3 % 3 - 1, Dz = D / 9 - 1;
+            double S[9] = {};
+            for (int a = 0; a < 8; a++) {
+                const double wa = hat(ox + (a & 1)) * hat(oy + ((a >> 1) & 1)) * hat(oz + (a >> 2));
+                if (wa == 0) continue;
+                for (int b = 0; b < 8; b++) {
+                    const double wb = hat(ox + (b & 1) - 2 * Dx) * hat(oy + ((b >> 1) & 1) - 2 * Dy) *
+                                      hat(oz + (b >> 2) - 2 * Dz);
+                    if (wb == 0) continue;
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) S[i * 3 + j] += wa * wb * h->ke[(3 * a + i) * 24 + 3 * b + j];
+                }
+            }
+            for (int k = 0; k < 9; k++) G[(c * 27 + D) * 9 + k] = (float)S[k];
+        }
+    }
+    if (hipMalloc(&h->d_ke, sizeof(float) * 576) || hipMalloc(&h->d_kchi, sizeof(float) * 144) ||
+        hipMalloc(&h->d_G, sizeof(float) * G.size()) || hipMalloc(&h->d_gperm, sizeof(int) * 48) ||
+        hipMalloc(&h->d_gflip, sizeof(int) * 48))
+        return fail(HOMO3D_HIP_ERROR);
+    hipMemcpy(h->d_ke, kef.data(), sizeof(float) * 576, hipMemcpyHostToDevice);
+    hipMemcpy(h->d_kchi, kchi.data(), sizeof(float) * 144, hipMemcpyHostToDevice);
+    hipMemcpy(h->d_G, G.data(), sizeof(float) * G.size(), hipMemcpyHostToDevice);
+    const int nc = sizes[nl - 1];
+    h->mc = 3 * nc * nc * nc;
+    if (hipMalloc(&h->d_Ainv, sizeof(double) * h->mc * h->mc)) return fail(HOMO3D_HIP_ERROR);
+    if (hipDeviceSynchronize() != hipSuccess) return fail(HOMO3D_HIP_ERROR);
+    *out = h;
+    return HOMO3D_OK;
+}
+
+int homo3d_destroy(homo3d_handle* h) {
+    if (!h) return HOMO3D_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (hipEvent_t e : h->pool) hipEventDestroy(e);
+    for (int l = 0; l < kMaxLevels; l++) {
+        hipFree(h->lv[l].u);
+        hipFree(h->lv[l].f);
+        hipFree(h->lv[l].r);
+        hipFree(h->lv[l].st);
+    }
+    void* ptrs[] = {h->d_ke, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->X, h->F, h->P, h->Q, h->Z,
+                    h->Rr, h->d_partial, h->d_sc, h->d_tmp, h->d_ftmp, h->d_gperm, h->d_gflip};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return HOMO3D_OK;
+}
+
+const char* homo3d_last_error(const homo3d_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int homo3d_levels(const homo3d_handle* h, int* sizes, int max_levels) {
+    if (!h) return -1;
+    for (int l = 0; l < h->nl && l < max_levels; l++) sizes[l] = h->lv[l].n;
+    return h->nl;
+}
+
+void* homo3d_stream(homo3d_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+// Homogenization::update (homogenization.cpp:79-85): cell weights w = Emin + x, Galerkin stencils of
+// every coarse level (MG::updateStencils, MG.cpp:341-354) and the coarsest FP64 factorisation.
+int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_device) {
+    if (!h || !x) return HOMO3D_BAD_ARG;
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const float* dx = nullptr;
+    int rc = stage_in(h, x, pitch, on_device, &dx);
+    if (rc) return rc;
+    launch_set_weights(h->stream, n, pitch, (float)h->cfg.emin, dx, h->d_w);
+    const float clamp = (float)h->cfg.diag_clamp;
+    launch_galerkin_fine(h->stream, n, h->d_G, h->d_ke, h->d_w, h->lv[1].st, clamp);
+    for (int l = 2; l < h->nl; l++) launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l - 1].st, h->lv[l].st, clamp);
+    // coarsest dense system
+    const Level& Lc = h->lv[h->nl - 1];
+    std::vector<float> st(243 * Lc.N);
+    HIPCHK(h, hipMemcpyAsync(st.data(), Lc.st, sizeof(float) * st.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int m = h->mc, nc = Lc.n;
+    std::vector<double> A((size_t)m * m, 0.0);
+    for (size_t v = 0; v < Lc.N; v++) {
+        const int x0 = (int)(v % nc), y0 = (int)((v / nc) % nc), z0 = (int)(v / ((size_t)nc * nc));
+        for (int d = 0; d < 27; d++) {
+            const int xq = (x0 + d % 3 - 1 + nc) % nc, yq = (y0 + d / 3 % 3 - 1 + nc) % nc, zq = (z0 + d / 9 - 1 + nc) % nc;
+            const size_t q = xq + (size_t)nc * (yq + (size_t)nc * zq);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) A[(3 * v + i) * m + 3 * q + j] += st[(size_t)(d * 9 + i * 3 + j) * Lc.N + v];
+        }
+    }
+    for (int i = 0; i < m; i++)
+        for (int j = i + 1; j < m; j++) {
+            const double s = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
+            A[(size_t)i * m + j] = A[(size_t)j * m + i] = s;
+        }
+    if (!cholesky_inverse(m, A)) return set_err(h, HOMO3D_DIVERGED, "coarsest system is not positive definite");
+    HIPCHK(h, hipMemcpyAsync(h->d_Ainv, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->density_set = true;
+    return HOMO3D_OK;
+}
+
+// The six macro-strain solves of Homogenization::elasticMatrix (homogenization.cu:471-477), run in
+// lockstep.  HOMO3D_SOLVER_MG: stand-alone V(1,1) cycles until the level-0 residual computed after
+// pre-smoothing satisfies ||r||/(||f||+1e-10) <= tol for every load case (MG::solveEquation,
+// MG.cpp:113-203, incl. one reset of the guess on explosion).  HOMO3D_SOLVER_PCG: MG-preconditioned
+// CG with a symmetric V-cycle and FP64 inner products.  Solutions stay in the handle (warm start).
+int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* stats) {
+    if (!h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    const Level& L0 = h->lv[0];
+    const size_t M = 3 * L0.N;  // floats per load case
+    const size_t vb = sizeof(float) * kRhs * M;
+    if (tol <= 0) tol = h->cfg.fem_rel_thres;
+    const int maxit = h->cfg.max_iter;
+    launch_macro_rhs(h->stream, L0.n, h->d_kchi, h->d_w, h->F);
+    if (!warm) HIPCHK(h, hipMemsetAsync(h->X, 0, vb, h->stream));
+    double fn2[kRhs];
+    launch_dot6(h->stream, M, h->F, h->F, h->d_partial, kDotBlocks, h->d_tmp);
+    HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int iters[kRhs] = {0, 0, 0, 0, 0, 0};
+    bool done[kRhs] = {false, false, false, false, false, false};
+    int status = HOMO3D_OK;
+    if (h->cfg.solver == HOMO3D_SOLVER_MG) {
+        int overflow[kRhs] = {2, 2, 2, 2, 2, 2};
+        int it = 0;
+        bool all = false;
+        while (!all && it++ < maxit) {
+            vcycle(h, h->X, h->F, 0, h->d_sc);
+            double rn2[kRhs];
+            HIPCHK(h, hipMemcpyAsync(rn2, h->d_sc, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            all = true;
+            for (int r = 0; r < kRhs; r++) {
+                if (done[r]) continue;
+                const double rel = std::sqrt(rn2[r]) / (std::sqrt(fn2[r]) + 1e-10);
+                iters[r] = it;
+                if (rel <= tol) {
+                    done[r] = true;
+                    continue;
+                }
+                if (rel > 10 || it >= maxit - 1) {
+                    if (--overflow[r] <= 0) {
+                        status = HOMO3D_DIVERGED;
+                        done[r] = true;
+                        continue;
+                    }
+                    HIPCHK(h, hipMemsetAsync(h->X + r * M, 0, sizeof(float) * M, h->stream));
+                }
+                all = false;
+            }
+        }
+        for (int r = 0; r < kRhs; r++)
+            if (!done[r]) status = std::max(status, (int)HOMO3D_NOT_CONVERGED);
+    } else {
+        // r = F - K X ; z = M r ; p = z
+        op_residual(h, 0, h->X, h->F, h->Rr);
+        double sc[6 * kRhs];
+        for (int r = 0; r < kRhs; r++) sc[5 * kRhs + r] = (fn2[r] > 0) ? 1.0 : 0.0;
+        double rr[kRhs];
+        launch_dot6(h->stream, M, h->Rr, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
+        HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int r = 0; r < kRhs; r++) {
+            if (fn2[r] <= 0 || std::sqrt(rr[r] / fn2[r]) <= tol) {
+                done[r] = true;
+                sc[5 * kRhs + r] = 0.0;
+            }
+        }
+        HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
+                                 h->stream));
+        HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
+        vcycle(h, h->Z, h->Rr, 1, nullptr);
+        HIPCHK(h, hipMemcpyAsync(h->P, h->Z, vb, hipMemcpyDeviceToDevice, h->stream));
+        launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc);  // rz
+        int it = 0;
+        bool all = std::all_of(done, done + kRhs, [](bool b) { return b; });
+        while (!all && it < maxit) {
+            it++;
+            {
+                ProfEvent ev;
+                prof_begin(h, 1, ev);
+                launch_fine_apply(h->stream, L0.n, h->d_ke, h->d_w, h->P, nullptr, h->Q);
+                prof_end(h, ev);
+            }
+            launch_dot6(h->stream, M, h->P, h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
+            launch_pcg_alpha(h->stream, h->d_sc);
+            launch_pcg_update_xr(h->stream, M, h->d_sc, h->P, h->Q, h->X, h->Rr);
+            launch_dot6(h->stream, M, h->Rr, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
+            HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            bool changed = false;
+            all = true;
+            for (int r = 0; r < kRhs; r++) {
+                if (done[r]) continue;
+                iters[r] = it;
+                if (std::sqrt(rr[r] / fn2[r]) <= tol) {
+                    done[r] = true;
+                    sc[5 * kRhs + r] = 0.0;
+                    changed = true;
+                } else {
+                    all = false;
+                }
+            }
+            if (changed)
+                HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs,
+                                         hipMemcpyHostToDevice, h->stream));
+            if (all) break;
+            HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
+            vcycle(h, h->Z, h->Rr, 1, nullptr);
+            launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
+            launch_pcg_beta(h->stream, h->d_sc);
+            launch_pcg_update_p(h->stream, M, h->d_sc, h->Z, h->P);
+        }
+        for (int r = 0; r < kRhs; r++)
+            if (!done[r]) status = std::max(status, (int)HOMO3D_NOT_CONVERGED);
+    }
+    // true final residual per load case
+    op_residual(h, 0, h->X, h->F, h->Rr);
+    double rr2[kRhs];
+    launch_dot6(h->stream, M, h->Rr, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
+    HIPCHK(h, hipMemcpyAsync(rr2, h->d_tmp, sizeof(rr2), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    if (stats) {
+        stats->status = status;
+        for (int r = 0; r < kRhs; r++) {
+            stats->iters[r] = iters[r];
+            stats->relres[r] = fn2[r] > 0 ? std::sqrt(rr2[r] / fn2[r]) : 0.0;
+        }
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (status == HOMO3D_DIVERGED) set_err(h, status, "MG numerical explode");
+    return status;
+}
+
+// C^H from the current solutions (Homogenization::elasticMatrix, homogenization.cu:478-575).
+int homo3d_elastic_matrix(homo3d_handle* h, double C[36]) {
+    if (!h || !C) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    const Level& L0 = h->lv[0];
+    launch_elastic_matrix(h->stream, L0.n, h->d_ke, h->d_w, h->X, h->d_partial, kDotBlocks, h->d_tmp);
+    double acc[21];
+    HIPCHK(h, hipMemcpyAsync(acc, h->d_tmp, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int k = 0;
+    for (int i = 0; i < 6; i++)
+        for (int j = i; j < 6; j++) {
+            C[i * 6 + j] = C[j * 6 + i] = acc[k] / (double)L0.N;
+            k++;
+        }
+    return HOMO3D_OK;
+}
+
+// dJ/dx_e for dJ/dC = dC (Homogenization::Sensitivity, homogenization.cu:860-948), lexicographic pitched.
+int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int on_device) {
+    if (!h || !dC || !sens) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    float wdc[21];
+    int k = 0;
+    for (int i = 0; i < 6; i++)
+        for (int j = i; j < 6; j++) wdc[k++] = dC[i * 6 + j] + (i != j ? dC[j * 6 + i] : 0.f);
+    HIPCHK(h, hipMemcpyAsync(h->d_ftmp, wdc, sizeof(wdc), hipMemcpyHostToDevice, h->stream));
+    float* dst = sens;
+    const size_t elems = (size_t)pitch * n * n;
+    if (!on_device) {
+        rc = ensure_stage(h, elems);
+        if (rc) return rc;
+        dst = h->d_stage;
+    }
+    launch_sensitivity(h->stream, n, pitch, h->d_ke, h->X, h->d_ftmp, dst);
+    if (!on_device) HIPCHK(h, hipMemcpyAsync(sens, dst, sizeof(float) * elems, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+static float* field_ptr(homo3d_handle* h, int level, int field) {
+    Level& L = h->lv[level];
+    if (level == 0) {
+        if (field == HOMO3D_FIELD_U) return h->X;
+        if (field == HOMO3D_FIELD_F) return h->F;
+        return L.r;
+    }
+    return field == HOMO3D_FIELD_U ? L.u : (field == HOMO3D_FIELD_F ? L.f : L.r);
+}
+
+int homo3d_field_copy(homo3d_handle* h, int level, int field, float* host, int to_device) {
+    if (!h || !host || level < 0 || level >= h->nl || field < 0 || field > 2) return HOMO3D_BAD_ARG;
+    float* d = field_ptr(h, level, field);
+    const size_t bytes = sizeof(float) * 3 * kRhs * h->lv[level].N;
+    if (to_device)
+        HIPCHK(h, hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, h->stream));
+    else
+        HIPCHK(h, hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_macro_rhs(homo3d_handle* h) {
+    if (!h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    launch_macro_rhs(h->stream, h->lv[0].n, h->d_kchi, h->d_w, h->F);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_residual(homo3d_handle* h, int level) {
+    if (!h || level < 0 || level >= h->nl) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    op_residual(h, level, field_ptr(h, level, 0), field_ptr(h, level, 1), h->lv[level].r);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_apply_K(homo3d_handle* h) {
+    if (!h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    ProfEvent ev;
+    prof_begin(h, 1, ev);
+    launch_fine_apply(h->stream, h->lv[0].n, h->d_ke, h->d_w, h->X, nullptr, h->lv[0].r);
+    prof_end(h, ev);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_smooth(homo3d_handle* h, int level, double omega, int reverse) {
+    if (!h || level < 0 || level >= h->nl - 1) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    op_smooth(h, level, field_ptr(h, level, 0), field_ptr(h, level, 1), (float)omega, reverse);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_restrict(homo3d_handle* h, int level) {
+    if (!h || level < 1 || level >= h->nl) return HOMO3D_BAD_ARG;
+    launch_restrict(h->stream, h->lv[level - 1].n, h->lv[level - 1].r, field_ptr(h, level, 1));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_prolong(homo3d_handle* h, int level) {
+    if (!h || level < 0 || level >= h->nl - 1) return HOMO3D_BAD_ARG;
+    launch_prolong(h->stream, h->lv[level].n, field_ptr(h, level + 1, 0), field_ptr(h, level, 0), level == 0 ? 1 : 0);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_coarse_solve(homo3d_handle* h) {
+    if (!h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    const Level& Lc = h->lv[h->nl - 1];
+    launch_coarse_solve(h->stream, Lc.n, h->d_Ainv, Lc.f, Lc.u);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_vcycle(homo3d_handle* h, int symmetric) {
+    if (!h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    vcycle(h, h->X, h->F, symmetric, nullptr);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_get_stencil(homo3d_handle* h, int level, float* host) {
+    if (!h || !host || level < 1 || level >= h->nl) return HOMO3D_BAD_ARG;
+    const Level& L = h->lv[level];
+    std::vector<float> soa(243 * L.N);
+    HIPCHK(h, hipMemcpyAsync(soa.data(), L.st, sizeof(float) * soa.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (size_t v = 0; v < L.N; v++)
+        for (int k = 0; k < 243; k++) host[v * 243 + k] = soa[(size_t)k * L.N + v];
+    return HOMO3D_OK;
+}
+
+int homo3d_symmetrize(homo3d_handle* h, float* field, int pitch, int sym, int on_device) {
+    if (!h || !field) return HOMO3D_BAD_ARG;
+    if (sym == HOMO3D_SYM_NONE) return HOMO3D_OK;
+    if (sym < 0 || sym > 3) return set_err(h, HOMO3D_BAD_ARG, "unknown symmetry %d", sym);
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    std::vector<int> perm, flip;
+    symmetry_group(sym, perm, flip);
+    HIPCHK(h, hipMemcpyAsync(h->d_gperm, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_gflip, flip.data(), sizeof(int) * flip.size(), hipMemcpyHostToDevice, h->stream));
+    const size_t elems = (size_t)pitch * n * n;
+    int rc = ensure_stage(h, 2 * elems);
+    if (rc) return rc;
+    float* in = h->d_stage;
+    float* out = h->d_stage + elems;
+    if (on_device)
+        HIPCHK(h, hipMemcpyAsync(in, field, sizeof(float) * elems, hipMemcpyDeviceToDevice, h->stream));
+    else
+        HIPCHK(h, hipMemcpyAsync(in, field, sizeof(float) * elems, hipMemcpyHostToDevice, h->stream));
+    launch_symmetrize(h->stream, n, pitch, (int)perm.size(), h->d_gperm, h->d_gflip, in, out);
+    launch_copy_pitched(h->stream, n, pitch, pitch, out, in);
+    HIPCHK(h, hipMemcpyAsync(field, in, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+// Sigmund filter in place on sens (OCOptimizer::filterSens, OCOptimizer.cu:185-195).
+int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch, double radius, int periodic,
+                     int on_device) {
+    if (!h || !sens || !rho) return HOMO3D_BAD_ARG;
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const int hw = (int)(radius + 0.5);
+    const int side = 2 * hw + 1;
+    if (side * side * side > 4096) return set_err(h, HOMO3D_NO_SUPPORT, "filter radius too large");
+    std::vector<float> wk(side * side * side);
+    double W = 0;
+    std::vector<double> wd(wk.size());
+    for (int i = 0; i < side * side * side; i++) {
+        const int o[3] = {i % side - hw, (i / side) % side - hw, i / (side * side) - hw};
+        const double r2 = (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) / (radius * radius);
+        wd[i] = r2 < 1 ? 1 - std::sqrt(r2) : 0.0;
+        W += wd[i];
+    }
+    for (size_t i = 0; i < wk.size(); i++) wk[i] = (float)(wd[i] / W);
+    HIPCHK(h, hipMemcpyAsync(h->d_ftmp, wk.data(), sizeof(float) * wk.size(), hipMemcpyHostToDevice, h->stream));
+    const size_t elems = (size_t)pitch * n * n;
+    int rc = ensure_stage(h, 3 * elems);
+    if (rc) return rc;
+    float* ds = h->d_stage;
+    float* dr = h->d_stage + elems;
+    float* dout = h->d_stage + 2 * elems;
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPCHK(h, hipMemcpyAsync(ds, sens, sizeof(float) * elems, kin, h->stream));
+    HIPCHK(h, hipMemcpyAsync(dr, rho, sizeof(float) * elems, kin, h->stream));
+    launch_oc_filter(h->stream, n, pitch, hw, periodic, h->d_ftmp, ds, dr, dout);
+    launch_copy_pitched(h->stream, n, pitch, pitch, dout, ds);
+    HIPCHK(h, hipMemcpyAsync(sens, ds, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+// OC update of rho in place (OCOptimizer::update, OCOptimizer.cu:218-247).
+int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch, double vol, double min_rho,
+                     double step, double damp, int on_device) {
+    if (!h || !sens || !rho) return HOMO3D_BAD_ARG;
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const size_t elems = (size_t)pitch * n * n;
+    int rc = ensure_stage(h, 3 * elems);
+    if (rc) return rc;
+    float* ds = h->d_stage;
+    float* dr = h->d_stage + elems;
+    float* dn = h->d_stage + 2 * elems;
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPCHK(h, hipMemcpyAsync(ds, sens, sizeof(float) * elems, kin, h->stream));
+    HIPCHK(h, hipMemcpyAsync(dr, rho, sizeof(float) * elems, kin, h->stream));
+    const int nb = 1024;
+    launch_absmax(h->stream, n, pitch, ds, h->d_ftmp, nb);
+    std::vector<float> pm(nb);
+    HIPCHK(h, hipMemcpyAsync(pm.data(), h->d_ftmp, sizeof(float) * nb, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double hi = 0, lo = 0;
+    for (float v : pm) hi = std::max(hi, (double)v);
+    const double Ne = (double)n * n * n;
+    for (int it = 0; it < 20; it++) {
+        const double g = (hi + lo) / 2;
+        double sum = 0;
+        launch_oc_trial(h->stream, n, pitch, g, min_rho, step, damp, ds, dr, dn, h->d_partial, nb, h->d_tmp);
+        HIPCHK(h, hipMemcpyAsync(&sum, h->d_tmp, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        const double cur = sum / Ne;
+        if (cur < vol - 1e-4)
+            hi = g;
+        else if (cur > vol + 1e-4)
+            lo = g;
+        else
+            break;
+    }
+    HIPCHK(h, hipMemcpyAsync(rho, dn, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_profile_enable(homo3d_handle* h, int enable) {
+    if (!h) return HOMO3D_BAD_ARG;
+    prof_flush(h);
+    h->prof = enable != 0;
+    if (enable)
+        for (int k = 0; k < kProfKinds; k++) {
+            h->prof_count[k] = 0;
+            h->prof_ms[k] = 0;
+        }
+    return HOMO3D_OK;
+}
+
+int homo3d_profile_get(homo3d_handle* h, const char* kernel, long* launches, double* total_ms) {
+    if (!h || !kernel) return HOMO3D_BAD_ARG;
+    prof_flush(h);
+    for (int k = 0; k < kProfKinds; k++)
+        if (std::strcmp(kernel, kProfNames[k]) == 0) {
+            if (launches) *launches = h->prof_count[k];
+            if (total_ms) *total_ms = h->prof_ms[k];
+            return HOMO3D_OK;
+        }
+    return set_err(h, HOMO3D_BAD_ARG, "unknown profiled kernel %s", kernel);
+}
+
+}  // extern "C"

@@ -1,0 +1,980 @@
+// gfx950 kernels of the periodic homogenization solve path.
+//
+// Grid convention: a cubic periodic level of n^3 vertices; vertex/cell id v = x + n*(y + n*z); cell
+// (x,y,z) has its minimum corner at vertex (x,y,z); node b of a cell sits at corner +
+// (b&1, b>>1&1, b>>2) (homogenization/templateMatrix.cpp:79-93).  Fields hold R = 6 load cases:
+// component c of load case r of vertex v lives at (r*3 + c)*N + v.  Coarse stencils are stored
+// structure-of-arrays: entry k = d*9 + i*3 + j (d = (dx+1) + 3(dy+1) + 9(dz+1)) of vertex v at k*N + v.
+//
+// The finest level is matrix-free: the 27 3x3 blocks of a vertex row are rebuilt from the 8 cell
+// weights w_e = Emin + x_e and the element matrix KE, which is read with wave-uniform indices so
+// the compiler keeps it in scalar registers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace homo3d {
+
+namespace {
+
+__device__ __forceinline__ int wrap_dec(int a, int n) { return a == 0 ? n - 1 : a - 1; }
+__device__ __forceinline__ int wrap_inc(int a, int n) { return a == n - 1 ? 0 : a + 1; }
+
+// 3x3 block of the assembled fine operator between vertex v and its neighbour v+d:
+// S = sum over the cells containing both of w_e * KE[node(v)][node(v+d)].
+// Cell e (0..7) sits at offset (e&1, e>>1&1, e>>2) - 1 from v; v is its node 7-e.
+template <int D>
+__device__ __forceinline__ void fine_block(const float wc[8], const float* __restrict__ ke, float S[9]) {
+    constexpr int dx = D % 3 - 1, dy = D / 3 % 3 - 1, dz = D / 9 - 1;
+#pragma unroll
+    for (int k = 0; k < 9; k++) S[k] = 0.f;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+        const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
+        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+        const int a = 7 - e, b = bx + 2 * by + 4 * bz;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) S[i * 3 + j] = fmaf(wc[e], ke[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
+    }
+}
+
+struct FineCtx {
+    int xs[3], ys[3], zs[3];
+    float wc[8];
+    __device__ __forceinline__ int nb(int d, int n) const {
+        return xs[d % 3] + n * (ys[d / 3 % 3] + n * zs[d / 9]);
+    }
+};
+
+__device__ __forceinline__ void fine_ctx(FineCtx& c, int x, int y, int z, int n, const float* __restrict__ w) {
+    c.xs[0] = wrap_dec(x, n); c.xs[1] = x; c.xs[2] = wrap_inc(x, n);
+    c.ys[0] = wrap_dec(y, n); c.ys[1] = y; c.ys[2] = wrap_inc(y, n);
+    c.zs[0] = wrap_dec(z, n); c.zs[1] = z; c.zs[2] = wrap_inc(z, n);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        // cell at offset (ex-1, ey-1, ez-1): index xs[ex], ys[ey], zs[ez]
+        c.wc[e] = w[c.xs[e & 1] + n * (c.ys[(e >> 1) & 1] + n * c.zs[e >> 2])];
+    }
+}
+
+// acc[r][i] += sum_{d in mask} S_d u_{v+d}; pinned origin read as zero.
+template <int R, bool SKIP_SELF>
+__device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, const float* __restrict__ ke,
+                                            const float* __restrict__ u, float acc[R][3]) {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
+#define HOMO3D_GATHER_D(DD)                                                                        \
+    if (!(SKIP_SELF && DD == 13)) {                                                                \
+        float S[9];                                                                                \
+        fine_block<DD>(c.wc, ke, S);                                                               \
+        const int q = c.nb(DD, n);                                                                 \
+        const float pm = (q == 0) ? 0.f : 1.f;                                                     \
+        _Pragma("unroll") for (int r = 0; r < R; r++) {                                            \
+            const float u0 = pm * u[(size_t)(r * 3 + 0) * N + q];                                  \
+            const float u1 = pm * u[(size_t)(r * 3 + 1) * N + q];                                  \
+            const float u2 = pm * u[(size_t)(r * 3 + 2) * N + q];                                  \
+            acc[r][0] = fmaf(S[0], u0, fmaf(S[1], u1, fmaf(S[2], u2, acc[r][0])));                 \
+            acc[r][1] = fmaf(S[3], u0, fmaf(S[4], u1, fmaf(S[5], u2, acc[r][1])));                 \
+            acc[r][2] = fmaf(S[6], u0, fmaf(S[7], u1, fmaf(S[8], u2, acc[r][2])));                 \
+        }                                                                                          \
+    }
+    HOMO3D_GATHER_D(0) HOMO3D_GATHER_D(1) HOMO3D_GATHER_D(2) HOMO3D_GATHER_D(3) HOMO3D_GATHER_D(4)
+    HOMO3D_GATHER_D(5) HOMO3D_GATHER_D(6) HOMO3D_GATHER_D(7) HOMO3D_GATHER_D(8) HOMO3D_GATHER_D(9)
+    HOMO3D_GATHER_D(10) HOMO3D_GATHER_D(11) HOMO3D_GATHER_D(12) HOMO3D_GATHER_D(13) HOMO3D_GATHER_D(14)
+    HOMO3D_GATHER_D(15) HOMO3D_GATHER_D(16) HOMO3D_GATHER_D(17) HOMO3D_GATHER_D(18) HOMO3D_GATHER_D(19)
+    HOMO3D_GATHER_D(20) HOMO3D_GATHER_D(21) HOMO3D_GATHER_D(22) HOMO3D_GATHER_D(23) HOMO3D_GATHER_D(24)
+    HOMO3D_GATHER_D(25) HOMO3D_GATHER_D(26)
+#undef HOMO3D_GATHER_D
+}
+
+// Block Gauss-Seidel / SOR update of one vertex: x, y, z in turn (z, y, x when REVERSE)
+// (gs_relaxation_otf_kernel update formula, grid.cu:530-541).
+template <bool REVERSE>
+__device__ __forceinline__ void block_update(float u[3], const float f[3], const float KeU[3], const float Ks[9],
+                                             float omega) {
+    if (!REVERSE) {
+        u[0] = omega * (f[0] - KeU[0] - Ks[1] * u[1] - Ks[2] * u[2]) / Ks[0] + (1.f - omega) * u[0];
+        u[1] = omega * (f[1] - KeU[1] - Ks[3] * u[0] - Ks[5] * u[2]) / Ks[4] + (1.f - omega) * u[1];
+        u[2] = omega * (f[2] - KeU[2] - Ks[6] * u[0] - Ks[7] * u[1]) / Ks[8] + (1.f - omega) * u[2];
+    } else {
+        u[2] = omega * (f[2] - KeU[2] - Ks[6] * u[0] - Ks[7] * u[1]) / Ks[8] + (1.f - omega) * u[2];
+        u[1] = omega * (f[1] - KeU[1] - Ks[3] * u[0] - Ks[5] * u[2]) / Ks[4] + (1.f - omega) * u[1];
+        u[0] = omega * (f[0] - KeU[0] - Ks[1] * u[1] - Ks[2] * u[2]) / Ks[0] + (1.f - omega) * u[0];
+    }
+}
+
+// GS colour c covers vertices with coordinate parity (1-c&1, 1-c>>1&1, 1-c>>2): the reference colours
+// by the parity of the ghost-padded position (grid.cpp:155-198) and sweeps colours 0..7.
+__device__ __forceinline__ void colour_vertex(long t, int n, int colour, int& x, int& y, int& z) {
+    const int h = n >> 1;
+    const int px = 1 - (colour & 1), py = 1 - ((colour >> 1) & 1), pz = 1 - (colour >> 2);
+    x = 2 * (int)(t % h) + px;
+    y = 2 * (int)((t / h) % h) + py;
+    z = 2 * (int)(t / ((long)h * h)) + pz;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Finest level (matrix-free)
+// ---------------------------------------------------------------------------------------------
+
+// MODE 0: out = K u; MODE 1: out = f - K u.  Pinned origin row is zero.
+// (update_residual_otf_kernel_1 grid.cu:1277-1461; v3_stencilOnLeft_kernel grid.cu:3686-3857)
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_fine_apply(int n, size_t N, const float* __restrict__ ke,
+                                                    const float* __restrict__ w, const float* __restrict__ u,
+                                                    const float* __restrict__ f, float* __restrict__ out) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)N) return;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    FineCtx c;
+    fine_ctx(c, x, y, z, n, w);
+    float acc[R][3];
+    fine_gather<R, false>(c, n, N, ke, u, acc);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const size_t o = (size_t)(r * 3 + i) * N + v;
+            float val = (MODE == 1) ? f[o] - acc[r][i] : acc[r][i];
+            out[o] = (v == 0) ? 0.f : val;
+        }
+}
+
+// One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
+template <int R, bool REVERSE>
+__global__ __launch_bounds__(256) void k_fine_gs(int n, size_t N, int colour, float omega,
+                                                 const float* __restrict__ ke, const float* __restrict__ w,
+                                                 float* __restrict__ u, const float* __restrict__ f) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long h = n >> 1;
+    if (t >= h * h * h) return;
+    int x, y, z;
+    colour_vertex(t, n, colour, x, y, z);
+    const long v = x + (long)n * (y + (long)n * z);
+    if (v == 0) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) u[(size_t)(r * 3 + i) * N] = 0.f;
+        return;
+    }
+    FineCtx c;
+    fine_ctx(c, x, y, z, n, w);
+    float acc[R][3];
+    fine_gather<R, true>(c, n, N, ke, u, acc);
+    float Ks[9];
+    fine_block<13>(c.wc, ke, Ks);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        float uu[3], ff[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            uu[i] = u[(size_t)(r * 3 + i) * N + v];
+            ff[i] = f[(size_t)(r * 3 + i) * N + v];
+        }
+        block_update<REVERSE>(uu, ff, acc[r], Ks, omega);
+#pragma unroll
+        for (int i = 0; i < 3; i++) u[(size_t)(r * 3 + i) * N + v] = uu[i];
+    }
+}
+
+// Macro-strain right-hand sides f_s = sum_e w_e KE chi0_s for the 6 strains (grid.cu:1818-1919).
+// kchi: 6 x 24 table of KE * chi0_s.  Pinned row zero.
+__global__ __launch_bounds__(256) void k_macro_rhs(int n, size_t N, const float* __restrict__ kchi,
+                                                   const float* __restrict__ w, float* __restrict__ f) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)N) return;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    FineCtx c;
+    fine_ctx(c, x, y, z, n, w);
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            float a = 0.f;
+#pragma unroll
+            for (int e = 0; e < 8; e++) a = fmaf(c.wc[e], kchi[s * 24 + 3 * (7 - e) + i], a);
+            f[(size_t)(s * 3 + i) * N + v] = (v == 0) ? 0.f : a;
+        }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coarse levels (stored 27-point stencils)
+// ---------------------------------------------------------------------------------------------
+
+template <int R, bool SKIP_SELF>
+__device__ __forceinline__ void stencil_gather(int x, int y, int z, int n, size_t N, long v,
+                                               const float* __restrict__ st, const float* __restrict__ u,
+                                               float acc[R][3]) {
+    const int xs[3] = {wrap_dec(x, n), x, wrap_inc(x, n)};
+    const int ys[3] = {wrap_dec(y, n), y, wrap_inc(y, n)};
+    const int zs[3] = {wrap_dec(z, n), z, wrap_inc(z, n)};
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
+#pragma unroll
+    for (int d = 0; d < 27; d++) {
+        if (SKIP_SELF && d == 13) continue;
+        float S[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) S[k] = st[(size_t)(d * 9 + k) * N + v];
+        const int q = xs[d % 3] + n * (ys[d / 3 % 3] + n * zs[d / 9]);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float u0 = u[(size_t)(r * 3 + 0) * N + q];
+            const float u1 = u[(size_t)(r * 3 + 1) * N + q];
+            const float u2 = u[(size_t)(r * 3 + 2) * N + q];
+            acc[r][0] = fmaf(S[0], u0, fmaf(S[1], u1, fmaf(S[2], u2, acc[r][0])));
+            acc[r][1] = fmaf(S[3], u0, fmaf(S[4], u1, fmaf(S[5], u2, acc[r][1])));
+            acc[r][2] = fmaf(S[6], u0, fmaf(S[7], u1, fmaf(S[8], u2, acc[r][2])));
+        }
+    }
+}
+
+// MODE 0: out = A u; MODE 1: out = f - A u (update_residual_kernel_1, grid.cu:1465-1564)
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_stencil_apply(int n, size_t N, const float* __restrict__ st,
+                                                       const float* __restrict__ u, const float* __restrict__ f,
+                                                       float* __restrict__ out) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)N) return;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    float acc[R][3];
+    stencil_gather<R, false>(x, y, z, n, N, v, st, u, acc);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const size_t o = (size_t)(r * 3 + i) * N + v;
+            out[o] = (MODE == 1) ? f[o] - acc[r][i] : acc[r][i];
+        }
+}
+
+// One colour of the coarse block GS sweep (gs_relaxation_kernel, grid.cu:556-689); no pin on coarse levels.
+template <int R, bool REVERSE>
+__global__ __launch_bounds__(256) void k_stencil_gs(int n, size_t N, int colour, float omega,
+                                                    const float* __restrict__ st, float* __restrict__ u,
+                                                    const float* __restrict__ f) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long h = n >> 1;
+    if (t >= h * h * h) return;
+    int x, y, z;
+    colour_vertex(t, n, colour, x, y, z);
+    const long v = x + (long)n * (y + (long)n * z);
+    float acc[R][3];
+    stencil_gather<R, true>(x, y, z, n, N, v, st, u, acc);
+    float Ks[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Ks[k] = st[(size_t)(13 * 9 + k) * N + v];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        float uu[3], ff[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            uu[i] = u[(size_t)(r * 3 + i) * N + v];
+            ff[i] = f[(size_t)(r * 3 + i) * N + v];
+        }
+        block_update<REVERSE>(uu, ff, acc[r], Ks, omega);
+#pragma unroll
+        for (int i = 0; i < 3; i++) u[(size_t)(r * 3 + i) * N + v] = uu[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Grid transfers (restrict_residual_kernel_1 grid.cu:917-994; prolongate_correction_kernel_1
+// grid.cu:1120-1190).  Weights prod_k (2-|o_k|)/2; R = P^T.
+// ---------------------------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void k_restrict(int nf, size_t Nf, const float* __restrict__ rf,
+                                                  float* __restrict__ fc) {
+    const int nc = nf >> 1;
+    const size_t Nc = (size_t)nc * nc * nc;
+    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= (long)Nc) return;
+    const int X = (int)(V % nc), Y = (int)((V / nc) % nc), Z = (int)(V / ((long)nc * nc));
+    const int xs[3] = {wrap_dec(2 * X, nf), 2 * X, 2 * X + 1};
+    const int ys[3] = {wrap_dec(2 * Y, nf), 2 * Y, 2 * Y + 1};
+    const int zs[3] = {wrap_dec(2 * Z, nf), 2 * Z, 2 * Z + 1};
+    float acc[R][3];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+        const int ox = o % 3, oy = o / 3 % 3, oz = o / 9;
+        const float wgt = (float)((ox == 1 ? 2 : 1) * (oy == 1 ? 2 : 1) * (oz == 1 ? 2 : 1)) * 0.125f;
+        const int q = xs[ox] + nf * (ys[oy] + nf * zs[oz]);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) acc[r][i] = fmaf(wgt, rf[(size_t)(r * 3 + i) * Nf + q], acc[r][i]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) fc[(size_t)(r * 3 + i) * Nc + V] = acc[r][i];
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_prolong(int nf, size_t Nf, const float* __restrict__ uc,
+                                                 float* __restrict__ uf, int pin) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)Nf) return;
+    if (pin && v == 0) return;
+    const int nc = nf >> 1;
+    const size_t Nc = (size_t)nc * nc * nc;
+    const int x = (int)(v % nf), y = (int)((v / nf) % nf), z = (int)(v / ((long)nf * nf));
+    // per axis: even -> one parent weight 1, odd -> two parents weight 1/2
+    int ix[2], iy[2], iz[2];
+    float wx[2], wy[2], wz[2];
+    ix[0] = x >> 1; ix[1] = (x & 1) ? ((x >> 1) + 1 == nc ? 0 : (x >> 1) + 1) : ix[0];
+    wx[0] = (x & 1) ? 0.5f : 1.f; wx[1] = (x & 1) ? 0.5f : 0.f;
+    iy[0] = y >> 1; iy[1] = (y & 1) ? ((y >> 1) + 1 == nc ? 0 : (y >> 1) + 1) : iy[0];
+    wy[0] = (y & 1) ? 0.5f : 1.f; wy[1] = (y & 1) ? 0.5f : 0.f;
+    iz[0] = z >> 1; iz[1] = (z & 1) ? ((z >> 1) + 1 == nc ? 0 : (z >> 1) + 1) : iz[0];
+    wz[0] = (z & 1) ? 0.5f : 1.f; wz[1] = (z & 1) ? 0.5f : 0.f;
+    float acc[R][3];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+        const float wgt = wx[p & 1] * wy[(p >> 1) & 1] * wz[p >> 2];
+        if (wgt == 0.f) continue;
+        const int q = ix[p & 1] + nc * (iy[(p >> 1) & 1] + nc * iz[p >> 2]);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) acc[r][i] = fmaf(wgt, uc[(size_t)(r * 3 + i) * Nc + q], acc[r][i]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) uf[(size_t)(r * 3 + i) * Nf + v] += acc[r][i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Galerkin coarse operators (restrict_stencil_otf_aos_kernel_1 grid_ex.cu:936-1061;
+// restrict_stencil_aos_kernel_1 grid_ex.cu:1065-1160; pinned-origin rows grid.cpp:1184-1325)
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ float hat(int o) { return o == 0 ? 1.f : ((o == 1 || o == -1) ? 0.5f : 0.f); }
+
+// Level-1 stencil away from the pinned origin: st[V][D] = sum_{c} w_{2V+c} G[c][D], c in {-2..1}^3.
+// G (64 x 27 x 9) is the density-independent table of P-weighted KE blocks.
+__global__ __launch_bounds__(256) void k_galerkin_fine(int nf, const float* __restrict__ G,
+                                                       const float* __restrict__ w, float* __restrict__ stc,
+                                                       float clamp) {
+    const int nc = nf >> 1;
+    const size_t Nc = (size_t)nc * nc * nc;
+    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= (long)Nc) return;
+    const int X = (int)(V % nc), Y = (int)((V / nc) % nc), Z = (int)(V / ((long)nc * nc));
+    float wc[64];
+#pragma unroll
+    for (int c = 0; c < 64; c++) {
+        int cx = 2 * X + (c & 3) - 2, cy = 2 * Y + ((c >> 2) & 3) - 2, cz = 2 * Z + (c >> 4) - 2;
+        cx = cx < 0 ? cx + nf : (cx >= nf ? cx - nf : cx);
+        cy = cy < 0 ? cy + nf : (cy >= nf ? cy - nf : cy);
+        cz = cz < 0 ? cz + nf : (cz >= nf ? cz - nf : cz);
+        wc[c] = w[cx + nf * (cy + nf * cz)];
+    }
+    for (int D = 0; D < 27; D++) {
+        float S[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) S[k] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 64; c++) {
+            const float* g = G + (c * 27 + D) * 9;
+#pragma unroll
+            for (int k = 0; k < 9; k++) S[k] = fmaf(wc[c], g[k], S[k]);
+        }
+        if (D == 13) {
+            if (fabsf(S[0]) < clamp) S[0] = clamp;
+            if (fabsf(S[4]) < clamp) S[4] = clamp;
+            if (fabsf(S[8]) < clamp) S[8] = clamp;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) stc[(size_t)(D * 9 + k) * Nc + V] = S[k];
+    }
+}
+
+// Level-1 stencil of the 27 coarse vertices around the pinned origin: node pairs touching the
+// origin contribute identity when they are the same node and nothing otherwise.
+__global__ void k_galerkin_fine_pin(int nf, const float* __restrict__ ke, const float* __restrict__ w,
+                                    float* __restrict__ stc, float clamp) {
+    const int nc = nf >> 1;
+    const size_t Nc = (size_t)nc * nc * nc;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (t%3-1, t/3%3-1, t/9-1) around the origin
+    if (t >= 27 * 27) return;
+    const int vi = t / 27, D = t % 27;
+    const int X = (vi % 3) - 1, Y = (vi / 3 % 3) - 1, Z = (vi / 9) - 1;
+    const int Dx = D % 3 - 1, Dy = D / 3 % 3 - 1, Dz = D / 9 - 1;
+    float S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 64; c++) {
+        const int ox = (c & 3) - 2, oy = ((c >> 2) & 3) - 2, oz = (c >> 4) - 2;  // cell offset from 2V
+        int cx = 2 * X + ox, cy = 2 * Y + oy, cz = 2 * Z + oz;
+        const int cxw = (cx % nf + nf) % nf, cyw = (cy % nf + nf) % nf, czw = (cz % nf + nf) % nf;
+        const float we = w[cxw + nf * (cyw + nf * czw)];
+        for (int a = 0; a < 8; a++) {
+            const int ax = ox + (a & 1), ay = oy + ((a >> 1) & 1), az = oz + (a >> 2);
+            const float wa = hat(ax) * hat(ay) * hat(az);
+            if (wa == 0.f) continue;
+            const bool pa = ((cx + (a & 1)) % nf == 0) && ((cy + ((a >> 1) & 1)) % nf == 0) &&
+                            ((cz + (a >> 2)) % nf == 0);
+            for (int b = 0; b < 8; b++) {
+                const int bx = ox + (b & 1) - 2 * Dx, by = oy + ((b >> 1) & 1) - 2 * Dy, bz = oz + (b >> 2) - 2 * Dz;
+                const float wb = hat(bx) * hat(by) * hat(bz);
+                if (wb == 0.f) continue;
+                const bool pb = ((cx + (b & 1)) % nf == 0) && ((cy + ((b >> 1) & 1)) % nf == 0) &&
+                                ((cz + (b >> 2)) % nf == 0);
+                if (pa || pb) {
+                    if (a == b) {
+                        S[0] += wa * wb;
+                        S[4] += wa * wb;
+                        S[8] += wa * wb;
+                    }
+                } else {
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) S[i * 3 + j] += wa * wb * we * ke[(3 * a + i) * 24 + 3 * b + j];
+                }
+            }
+        }
+    }
+    if (D == 13) {
+        if (fabsf(S[0]) < clamp) S[0] = clamp;
+        if (fabsf(S[4]) < clamp) S[4] = clamp;
+        if (fabsf(S[8]) < clamp) S[8] = clamp;
+    }
+    const int Xw = (X + nc) % nc, Yw = (Y + nc) % nc, Zw = (Z + nc) % nc;
+    const size_t V = Xw + (size_t)nc * (Yw + (size_t)nc * Zw);
+    for (int k = 0; k < 9; k++) stc[(size_t)(D * 9 + k) * Nc + V] = S[k];
+}
+
+// Coarser stencil from a stored one: st_c[V][D] = sum_{o,d} hat(o) A_f[2V+o][d] hat3(o+d-2D).
+__global__ __launch_bounds__(256) void k_galerkin_coarse(int nf, const float* __restrict__ stf,
+                                                         float* __restrict__ stc, float clamp) {
+    const int nc = nf >> 1;
+    const size_t Nc = (size_t)nc * nc * nc, Nf = (size_t)nf * nf * nf;
+    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= (long)Nc) return;
+    const int X = (int)(V % nc), Y = (int)((V / nc) % nc), Z = (int)(V / ((long)nc * nc));
+    const int xs[3] = {wrap_dec(2 * X, nf), 2 * X, 2 * X + 1};
+    const int ys[3] = {wrap_dec(2 * Y, nf), 2 * Y, 2 * Y + 1};
+    const int zs[3] = {wrap_dec(2 * Z, nf), 2 * Z, 2 * Z + 1};
+    for (int D = 0; D < 27; D++) {
+        const int Dx = D % 3 - 1, Dy = D / 3 % 3 - 1, Dz = D / 9 - 1;
+        float S[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) S[k] = 0.f;
+        for (int o = 0; o < 27; o++) {
+            const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
+            const float wo = hat(ox) * hat(oy) * hat(oz);
+            const size_t vf = xs[ox + 1] + (size_t)nf * (ys[oy + 1] + (size_t)nf * zs[oz + 1]);
+            for (int d = 0; d < 27; d++) {
+                const int dx = d % 3 - 1, dy = d / 3 % 3 - 1, dz = d / 9 - 1;
+                const float wd = hat(ox + dx - 2 * Dx) * hat(oy + dy - 2 * Dy) * hat(oz + dz - 2 * Dz);
+                if (wd == 0.f) continue;
+                const float f = wo * wd;
+#pragma unroll
+                for (int k = 0; k < 9; k++) S[k] = fmaf(f, stf[(size_t)(d * 9 + k) * Nf + vf], S[k]);
+            }
+        }
+        if (D == 13) {
+            if (fabsf(S[0]) < clamp) S[0] = clamp;
+            if (fabsf(S[4]) < clamp) S[4] = clamp;
+            if (fabsf(S[8]) < clamp) S[8] = clamp;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) stc[(size_t)(D * 9 + k) * Nc + V] = S[k];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coarsest level: FP64 dense solve u = A^{-1} f with the inverse formed from a Cholesky
+// factorisation of the pinned Galerkin system (replaces the host BiCGSTAB of grid.cpp:349-374).
+// One block per load case; dof order 3*v + i.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_coarse_solve(int m, size_t Nc, const double* __restrict__ Ainv,
+                                                      const float* __restrict__ f, float* __restrict__ u) {
+    extern __shared__ double fb[];
+    const int r = blockIdx.x;
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        const int v = k / 3, i = k % 3;
+        fb[k] = (double)f[(size_t)(r * 3 + i) * Nc + v];
+    }
+    __syncthreads();
+    for (int row = threadIdx.x; row < m; row += blockDim.x) {
+        double s = 0.0;
+        const double* a = Ainv + (size_t)row * m;
+        for (int k = 0; k < m; k++) s = fma(a[k], fb[k], s);
+        const int v = row / 3, i = row % 3;
+        u[(size_t)(r * 3 + i) * Nc + v] = (float)s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 reductions and vector updates (per load case)
+// ---------------------------------------------------------------------------------------------
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < BS / 64; i++) s += sh[i];
+    }
+    __syncthreads();
+    return s;
+}
+
+// partial[r * nblocks + b] = sum over block b's grid-stride share of a_r . b_r  (3N values per r)
+template <int R>
+__global__ __launch_bounds__(256) void k_dot(size_t M, const float* __restrict__ a, const float* __restrict__ b,
+                                             double* __restrict__ partial) {
+    __shared__ double sh[4];
+    for (int r = 0; r < R; r++) {
+        double s = 0.0;
+        const float* ar = a + (size_t)r * M;
+        const float* br = b + (size_t)r * M;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += (size_t)gridDim.x * blockDim.x)
+            s += (double)ar[i] * (double)br[i];
+        s = block_sum<256>(s, sh);
+        if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+__global__ void k_sum_partials(int R, int nb, const double* __restrict__ partial, double* __restrict__ out) {
+    __shared__ double sh[4];
+    for (int r = 0; r < R; r++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[r * nb + i];
+        s = block_sum<256>(s, sh);
+        if (threadIdx.x == 0) out[r] = s;
+    }
+}
+
+// PCG scalars per load case.  sc layout: [0..R) rz, [R..2R) pq, [2R..3R) alpha, [3R..4R) rz_new,
+// [4R..5R) beta, [5R..6R) active flag.
+__global__ void k_pcg_alpha(int R, double* sc) {
+    const int r = threadIdx.x;
+    if (r >= R) return;
+    const double pq = sc[R + r];
+    sc[2 * R + r] = (sc[5 * R + r] != 0.0 && pq != 0.0) ? sc[r] / pq : 0.0;
+}
+
+__global__ void k_pcg_beta(int R, double* sc) {
+    const int r = threadIdx.x;
+    if (r >= R) return;
+    const double rz = sc[r], rzn = sc[3 * R + r];
+    sc[4 * R + r] = (sc[5 * R + r] != 0.0 && rz != 0.0) ? rzn / rz : 0.0;
+    sc[r] = rzn;
+}
+
+// x += alpha p ; r -= alpha q  (per load case alpha)
+__global__ __launch_bounds__(256) void k_pcg_update_xr(int R, size_t M, const double* __restrict__ sc,
+                                                       const float* __restrict__ p, const float* __restrict__ q,
+                                                       float* __restrict__ x, float* __restrict__ r) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    for (int k = 0; k < R; k++) {
+        const float a = (float)sc[2 * R + k];
+        const size_t o = (size_t)k * M + i;
+        x[o] = fmaf(a, p[o], x[o]);
+        r[o] = fmaf(-a, q[o], r[o]);
+    }
+}
+
+// p = z + beta p (active load cases only)
+__global__ __launch_bounds__(256) void k_pcg_update_p(int R, size_t M, const double* __restrict__ sc,
+                                                      const float* __restrict__ z, float* __restrict__ p) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    for (int k = 0; k < R; k++) {
+        if (sc[5 * R + k] == 0.0) continue;
+        const float b = (float)sc[4 * R + k];
+        const size_t o = (size_t)k * M + i;
+        p[o] = fmaf(b, p[o], z[o]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// C^H and sensitivity (elasticMatrix_kernel_opt homogenization.cu:208-346;
+// Sensitivity_kernel_opt_2 homogenization.cu:697-858).  One thread per cell; the six solution
+// fields of the cell's 8 nodes are read in FP32 and the cell energies accumulated in FP32.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float macro_disp(int node, int s, int c) {
+    const float px = (float)(node & 1), py = (float)((node >> 1) & 1), pz = (float)(node >> 2);
+    switch (s) {
+        case 0: return c == 0 ? px : 0.f;
+        case 1: return c == 1 ? py : 0.f;
+        case 2: return c == 2 ? pz : 0.f;
+        case 3: return c == 1 ? 0.5f * pz : (c == 2 ? 0.5f * py : 0.f);
+        case 4: return c == 0 ? 0.5f * pz : (c == 2 ? 0.5f * px : 0.f);
+        default: return c == 0 ? 0.5f * py : (c == 1 ? 0.5f * px : 0.f);
+    }
+}
+
+__device__ __forceinline__ void cell_energies(int n, size_t N, long e, const float* __restrict__ ke,
+                                              const float* __restrict__ chi, float E[21]) {
+    const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+    const int xs[2] = {x, wrap_inc(x, n)}, ys[2] = {y, wrap_inc(y, n)}, zs[2] = {z, wrap_inc(z, n)};
+    float U[6][24];
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const int q = xs[b & 1] + n * (ys[(b >> 1) & 1] + n * zs[b >> 2]);
+#pragma unroll
+        for (int s = 0; s < 6; s++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) U[s][3 * b + c] = macro_disp(b, s, c) - chi[(size_t)(s * 3 + c) * N + q];
+    }
+#pragma unroll
+    for (int k = 0; k < 21; k++) E[k] = 0.f;
+#pragma unroll
+    for (int i = 0; i < 24; i++) {
+        float KU[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) KU[s] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 24; j++) {
+            const float kij = ke[i * 24 + j];
+#pragma unroll
+            for (int s = 0; s < 6; s++) KU[s] = fmaf(kij, U[s][j], KU[s]);
+        }
+        int k = 0;
+#pragma unroll
+        for (int a = 0; a < 6; a++)
+#pragma unroll
+            for (int b = a; b < 6; b++) {
+                E[k] = fmaf(U[a][i], KU[b], E[k]);
+                k++;
+            }
+    }
+}
+
+// partial[k * nblocks + block] = sum of w_e E_k,e over the block's cells (FP64)
+__global__ __launch_bounds__(256) void k_elastic_matrix(int n, size_t N, const float* __restrict__ ke,
+                                                        const float* __restrict__ w, const float* __restrict__ chi,
+                                                        double* __restrict__ partial) {
+    __shared__ double sh[4];
+    double acc[21];
+#pragma unroll
+    for (int k = 0; k < 21; k++) acc[k] = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (size_t)gridDim.x * blockDim.x) {
+        float E[21];
+        cell_energies(n, N, (long)e, ke, chi, E);
+        const double we = w[e];
+#pragma unroll
+        for (int k = 0; k < 21; k++) acc[k] += we * (double)E[k];
+    }
+    for (int k = 0; k < 21; k++) {
+        const double s = block_sum<256>(acc[k], sh);
+        if (threadIdx.x == 0) partial[k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// sens[x + (y + z*n)*pitch] = (1/N) sum_{i<=j} E_ij (dC_ij + [i!=j] dC_ji)
+__global__ __launch_bounds__(256) void k_sensitivity(int n, size_t N, int pitch, const float* __restrict__ ke,
+                                                     const float* __restrict__ chi, const float* __restrict__ wdc,
+                                                     float* __restrict__ sens) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)N) return;
+    float E[21];
+    cell_energies(n, N, e, ke, chi, E);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 21; k++) s = fmaf(E[k], wdc[k], s);
+    const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+    sens[x + ((size_t)y + (size_t)z * n) * pitch] = s / (float)N;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Density utilities
+// ---------------------------------------------------------------------------------------------
+
+// Symmetry projection (symetrize_tensor_kernel, TensorExpression.cuh:72-262): each cell receives
+// the FP64 average over the group applied to its orbit's smallest-index member, so every member of
+// an orbit receives the same bits.  gperm/gflip list the group elements; n cubic.
+__global__ __launch_bounds__(256) void k_symmetrize(int n, int pitch, int ng, const int* __restrict__ gperm,
+                                                    const int* __restrict__ gflip, const float* __restrict__ in,
+                                                    float* __restrict__ out) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const int p[3] = {(int)(e % n), (int)((e / n) % n), (int)(e / ((long)n * n))};
+    long best = e;
+    int bq[3] = {p[0], p[1], p[2]};
+    for (int g = 0; g < ng; g++) {
+        int q[3];
+        for (int k = 0; k < 3; k++) {
+            const int a = p[(gperm[g] >> (2 * k)) & 3];
+            q[k] = ((gflip[g] >> k) & 1) ? n - 1 - a : a;
+        }
+        const long id = q[0] + (long)n * (q[1] + (long)n * q[2]);
+        if (id < best) {
+            best = id;
+            bq[0] = q[0]; bq[1] = q[1]; bq[2] = q[2];
+        }
+    }
+    double s = 0.0;
+    for (int g = 0; g < ng; g++) {
+        int q[3];
+        for (int k = 0; k < 3; k++) {
+            const int a = bq[(gperm[g] >> (2 * k)) & 3];
+            q[k] = ((gflip[g] >> k) & 1) ? n - 1 - a : a;
+        }
+        s += (double)in[q[0] + ((size_t)q[1] + (size_t)q[2] * n) * pitch];
+    }
+    out[p[0] + ((size_t)p[1] + (size_t)p[2] * n) * pitch] = (float)(s / (double)ng);
+}
+
+// Sigmund filter (filterSens_kernel, OCOptimizer.cu:76-153): wk holds the normalised weights of the
+// (2h+1)^3 box.
+__global__ __launch_bounds__(256) void k_oc_filter(int n, int pitch, int h, int periodic, const float* __restrict__ wk,
+                                                   const float* __restrict__ sens, const float* __restrict__ rho,
+                                                   float* __restrict__ out) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const int p[3] = {(int)(e % n), (int)((e / n) % n), (int)(e / ((long)n * n))};
+    const int side = 2 * h + 1;
+    double sum = 0.0, ws = 0.0;
+    for (int i = 0; i < side * side * side; i++) {
+        int q[3] = {p[0] + i % side - h, p[1] + (i / side) % side - h, p[2] + i / (side * side) - h};
+        bool inb = true;
+        for (int k = 0; k < 3; k++) {
+            if (periodic) q[k] = ((q[k] % n) + n) % n;
+            if (q[k] < 0 || q[k] >= n) inb = false;
+        }
+        if (!inb) continue;
+        const size_t id = q[0] + ((size_t)q[1] + (size_t)q[2] * n) * pitch;
+        sum += (double)sens[id] * (double)rho[id] * (double)wk[i];
+        ws += (double)wk[i];
+    }
+    const size_t me = p[0] + ((size_t)p[1] + (size_t)p[2] * n) * pitch;
+    out[me] = (float)(sum / (ws * (double)rho[me]));
+}
+
+// OC trial update for multiplier g (update_kernel, OCOptimizer.cu:12-34), FP64 arithmetic rounded
+// to FP32; per-block FP64 partial sums of the new density.
+__global__ __launch_bounds__(256) void k_oc_trial(int n, int pitch, double g, double min_rho, double step,
+                                                  double damp, const float* __restrict__ sens,
+                                                  const float* __restrict__ rho, float* __restrict__ out,
+                                                  double* __restrict__ partial) {
+    __shared__ double sh[4];
+    const long N = (long)n * n * n;
+    double s = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+        const size_t id = x + ((size_t)y + (size_t)z * n) * pitch;
+        const double r0 = rho[id];
+        double B = -(double)sens[id] / g;
+        if (B < 0) B = 0.01;
+        double xv = (damp == 0.5 ? sqrt(B) : pow(B, damp)) * r0;
+        if (xv - r0 < -step) xv = r0 - step;
+        if (xv - r0 > step) xv = r0 + step;
+        if (xv < min_rho) xv = min_rho;
+        if (xv > 1.0) xv = 1.0;
+        const float o = (float)xv;
+        out[id] = o;
+        s += (double)o;
+    }
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_absmax(long N, int n, int pitch, const float* __restrict__ a,
+                                                float* __restrict__ partial) {
+    __shared__ float sh[256];
+    float m = 0.f;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+        m = fmaxf(m, fabsf(a[x + ((size_t)y + (size_t)z * n) * pitch]));
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// pitched lexicographic density -> dense cell weights w = emin + x
+__global__ __launch_bounds__(256) void k_set_weights(int n, int pitch, float emin, const float* __restrict__ x,
+                                                     float* __restrict__ w) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const int xx = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+    w[e] = emin + x[xx + ((size_t)y + (size_t)z * n) * pitch];
+}
+
+__global__ __launch_bounds__(256) void k_copy_pitched(int n, int pitch_in, int pitch_out, const float* __restrict__ in,
+                                                      float* __restrict__ out) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+    out[x + ((size_t)y + (size_t)z * n) * pitch_out] = in[x + ((size_t)y + (size_t)z * n) * pitch_in];
+}
+
+inline unsigned nblk(size_t work, int bs = 256) { return (unsigned)((work + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------------------------
+void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const float* u, const float* f,
+                       float* out) {
+    const size_t N = (size_t)n * n * n;
+    if (f)
+        hipLaunchKernelGGL((k_fine_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, n, N, ke, w, u, f, out);
+    else
+        hipLaunchKernelGGL((k_fine_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, n, N, ke, w, u, f, out);
+}
+
+void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* ke, const float* w,
+                    float* u, const float* f) {
+    const size_t N = (size_t)n * n * n;
+    const size_t h3 = N / 8;
+    if (reverse)
+        hipLaunchKernelGGL((k_fine_gs<6, true>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, ke, w, u, f);
+    else
+        hipLaunchKernelGGL((k_fine_gs<6, false>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, ke, w, u, f);
+}
+
+void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_macro_rhs, dim3(nblk(N)), dim3(256), 0, s, n, N, kchi, w, f);
+}
+
+void launch_stencil_apply(hipStream_t s, int n, const float* st, const float* u, const float* f, float* out) {
+    const size_t N = (size_t)n * n * n;
+    if (f)
+        hipLaunchKernelGGL((k_stencil_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, n, N, st, u, f, out);
+    else
+        hipLaunchKernelGGL((k_stencil_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, n, N, st, u, f, out);
+}
+
+void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* st, float* u,
+                       const float* f) {
+    const size_t N = (size_t)n * n * n;
+    const size_t h3 = N / 8;
+    if (reverse)
+        hipLaunchKernelGGL((k_stencil_gs<6, true>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, st, u, f);
+    else
+        hipLaunchKernelGGL((k_stencil_gs<6, false>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, st, u, f);
+}
+
+void launch_restrict(hipStream_t s, int nf, const float* rf, float* fc) {
+    const size_t Nf = (size_t)nf * nf * nf;
+    hipLaunchKernelGGL((k_restrict<6>), dim3(nblk(Nf / 8)), dim3(256), 0, s, nf, Nf, rf, fc);
+}
+
+void launch_prolong(hipStream_t s, int nf, const float* uc, float* uf, int pin) {
+    const size_t Nf = (size_t)nf * nf * nf;
+    hipLaunchKernelGGL((k_prolong<6>), dim3(nblk(Nf)), dim3(256), 0, s, nf, Nf, uc, uf, pin);
+}
+
+void launch_galerkin_fine(hipStream_t s, int nf, const float* G, const float* ke, const float* w, float* stc,
+                          float clamp) {
+    const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
+    hipLaunchKernelGGL(k_galerkin_fine, dim3(nblk(Nc)), dim3(256), 0, s, nf, G, w, stc, clamp);
+    hipLaunchKernelGGL(k_galerkin_fine_pin, dim3(nblk(27 * 27)), dim3(256), 0, s, nf, ke, w, stc, clamp);
+}
+
+void launch_galerkin_coarse(hipStream_t s, int nf, const float* stf, float* stc, float clamp) {
+    const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
+    hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(Nc)), dim3(256), 0, s, nf, stf, stc, clamp);
+}
+
+void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float* f, float* u) {
+    const size_t Nc = (size_t)nc * nc * nc;
+    const int m = (int)(3 * Nc);
+    hipLaunchKernelGGL(k_coarse_solve, dim3(6), dim3(256), sizeof(double) * m, s, m, Nc, Ainv, f, u);
+}
+
+void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks,
+                 double* out) {
+    hipLaunchKernelGGL((k_dot<6>), dim3(nblocks), dim3(256), 0, s, M, a, b, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, nblocks, partial, out);
+}
+
+void launch_sum_partials(hipStream_t s, int R, int nblocks, const double* partial, double* out) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, R, nblocks, partial, out);
+}
+
+void launch_pcg_alpha(hipStream_t s, double* sc) { hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(64), 0, s, 6, sc); }
+void launch_pcg_beta(hipStream_t s, double* sc) { hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(64), 0, s, 6, sc); }
+
+void launch_pcg_update_xr(hipStream_t s, size_t M, const double* sc, const float* p, const float* q, float* x,
+                          float* r) {
+    hipLaunchKernelGGL(k_pcg_update_xr, dim3(nblk(M)), dim3(256), 0, s, 6, M, sc, p, q, x, r);
+}
+
+void launch_pcg_update_p(hipStream_t s, size_t M, const double* sc, const float* z, float* p) {
+    hipLaunchKernelGGL(k_pcg_update_p, dim3(nblk(M)), dim3(256), 0, s, 6, M, sc, z, p);
+}
+
+void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,
+                           int nblocks, double* out21) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_elastic_matrix, dim3(nblocks), dim3(256), 0, s, n, N, ke, w, chi, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 21, nblocks, partial, out21);
+}
+
+void launch_sensitivity(hipStream_t s, int n, int pitch, const float* ke, const float* chi, const float* wdc,
+                        float* sens) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_sensitivity, dim3(nblk(N)), dim3(256), 0, s, n, N, pitch, ke, chi, wdc, sens);
+}
+
+void launch_symmetrize(hipStream_t s, int n, int pitch, int ng, const int* gperm, const int* gflip, const float* in,
+                       float* out) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblk(N)), dim3(256), 0, s, n, pitch, ng, gperm, gflip, in, out);
+}
+
+void launch_oc_filter(hipStream_t s, int n, int pitch, int h, int periodic, const float* wk, const float* sens,
+                      const float* rho, float* out) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_oc_filter, dim3(nblk(N)), dim3(256), 0, s, n, pitch, h, periodic, wk, sens, rho, out);
+}
+
+void launch_oc_trial(hipStream_t s, int n, int pitch, double g, double min_rho, double step, double damp,
+                     const float* sens, const float* rho, float* out, double* partial, int nblocks, double* sum) {
+    hipLaunchKernelGGL(k_oc_trial, dim3(nblocks), dim3(256), 0, s, n, pitch, g, min_rho, step, damp, sens, rho, out,
+                       partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 1, nblocks, partial, sum);
+}
+
+void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* partial, int nblocks) {
+    const long N = (long)n * n * n;
+    hipLaunchKernelGGL(k_absmax, dim3(nblocks), dim3(256), 0, s, N, n, pitch, a, partial);
+}
+
+void launch_set_weights(hipStream_t s, int n, int pitch, float emin, const float* x, float* w) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_set_weights, dim3(nblk(N)), dim3(256), 0, s, n, pitch, emin, x, w);
+}
+
+void launch_copy_pitched(hipStream_t s, int n, int pitch_in, int pitch_out, const float* in, float* out) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_copy_pitched, dim3(nblk(N)), dim3(256), 0, s, n, pitch_in, pitch_out, in, out);
+}
+
+}  // namespace homo3d

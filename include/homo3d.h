@@ -1,0 +1,134 @@
+/*
+ * homo3d.h - C ABI of the MI355X-native periodic homogenization solver.
+ *
+ * Plain C types only (pointers, sizes, doubles).  Every entry point returns an int status
+ * (HOMO3D_OK = 0) and records a message retrievable with homo3d_last_error().
+ *
+ * Reference interfaces each entry point replaces (lavenklau/homo3d):
+ *   homo3d_create            homo::Homogenization::Homogenization(cfg::HomoConfig) / build
+ *                            (homogenization/homogenization.h:20-22, homogenization.cpp:40-66)
+ *                            and MG::build (homogenization/MG.h:22, MG.cpp:13-44)
+ *   homo3d_update_density    Homogenization::update(float* rho, int pitchT)
+ *                            (homogenization.h:28, homogenization.cpp:79-85) incl. MG::updateStencils
+ *                            (MG.cpp:341-354)
+ *   homo3d_solve             the six-load-case loop of Homogenization::elasticMatrix
+ *                            (homogenization.cu:471-477): Grid::useFchar / useUchar /
+ *                            MG::solveEquation / Grid::setUchar (grid.h:257-262, MG.h:30)
+ *   homo3d_elastic_matrix    Homogenization::elasticMatrix(double C[6][6]) (homogenization.h:24,
+ *                            homogenization.cu:478-575)
+ *   homo3d_sensitivity       Homogenization::Sensitivity(float dC[6][6], float* sens, int pitchT,
+ *                            bool lexiOrder) (homogenization.h:38, homogenization.cu:860-948)
+ *   homo3d_macro_rhs         Grid::enforce_unit_macro_strain (grid.h:255, grid.cu:1921-1935)
+ *   homo3d_residual          Grid::update_residual (grid.h:253, grid.cu:1566-1593)
+ *   homo3d_apply_K           Grid::v3_stencilOnLeft (grid.h:355, grid.cu:3686-3857)
+ *   homo3d_smooth            Grid::gs_relaxation (grid.h:241, grid.cu:1192-1236)
+ *   homo3d_restrict          Grid::restrict_residual (grid.h:249, grid.cu:1257-1275)
+ *   homo3d_prolong           Grid::prolongate_correction (grid.h:247, grid.cu:1238-1255)
+ *   homo3d_coarse_solve      Grid::solveHostEquation (grid.h:236, grid.cpp:349-374)
+ *   homo3d_vcycle            MG::v_cycle (MG.h:26, MG.cpp:46-70)
+ *   homo3d_field_copy        Grid::v3_upload / v3_download (grid.h:339-340)
+ *   homo3d_get_stencil       Grid::stencil_g read-back (grid.h:126)
+ *   homo3d_oc_filter         OCOptimizer::filterSens (optimization/OCOptimizer.h:17-18)
+ *   homo3d_oc_update         OCOptimizer::update (optimization/OCOptimizer.h:19-20)
+ *   homo3d_symmetrize        symmetrizeField / Tensor::symmetrize (homogenization/Framework.cuh:221-235,
+ *                            AutoDiff/TensorExpression.h:425-447)
+ *   homo3d_mma_*             homo::MMAOptimizer (optimization/mmaOptimizer.h:12-34) and
+ *                            mmasubDevice (optimization/mmaOpt.h:17-19)
+ *   homo3d_randc_density     initDensity "randc" (Framework.cu:61-134, Framework.cuh:237-310), replaced
+ *                            by a seeded generator
+ *
+ * Field layout: a displacement/force/residual field of one level holds 6 load cases, each three
+ * components of n^3 floats: index ((rhs * 3 + comp) * n^3 + v), v = x + n*(y + n*z).  Densities and
+ * sensitivities cross the boundary as the reference's lexicographic row-pitched arrays:
+ * element (x,y,z) at x + (y + z*n) * pitch (AutoDiff/TensorExpression.h:137-150).
+ * C is a row-major double[36] in Voigt order xx, yy, zz, yz, xz, xy.
+ */
+#ifndef HOMO3D_H
+#define HOMO3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HOMO3D_OK = 0,
+    HOMO3D_NOT_CONVERGED = 1,
+    HOMO3D_DIVERGED = 2,
+    HOMO3D_BAD_ARG = 3,
+    HOMO3D_HIP_ERROR = 4,
+    HOMO3D_NO_SUPPORT = 5
+};
+
+enum { HOMO3D_SOLVER_MG = 0, HOMO3D_SOLVER_PCG = 1 };
+enum { HOMO3D_FIELD_U = 0, HOMO3D_FIELD_F = 1, HOMO3D_FIELD_R = 2 };
+enum { HOMO3D_SYM_REFLECT3 = 0, HOMO3D_SYM_REFLECT6 = 1, HOMO3D_SYM_ROTATE3 = 2, HOMO3D_SYM_NONE = 3 };
+
+typedef struct homo3d_handle homo3d_handle;
+
+/* Mirrors cfg::HomoConfig (cmdline.h:39-60) for the fields the solve path reads. */
+typedef struct {
+    int reso[3];            /* cells per axis; cubic grids, 8 <= n <= 512 */
+    double youngs_modulus;  /* E (reference default 10) */
+    double poisson_ratio;   /* nu (0.3) */
+    double emin;            /* stiffness floor added to x_e (reference rhoPenalMin = 1e-9) */
+    double fem_rel_thres;   /* default stop tolerance (reference femRelThres, 1e-2) */
+    int max_iter;           /* max V-cycles / PCG iterations per solve (200) */
+    int solver;             /* HOMO3D_SOLVER_MG (reference stand-alone V-cycles) or HOMO3D_SOLVER_PCG */
+    int device;             /* HIP device ordinal */
+    double sor_omega;       /* smoother relaxation factor (1.0) */
+    double diag_clamp;      /* coarse-stencil diagonal floor (1e-4) */
+} homo3d_config;
+
+typedef struct {
+    int status;             /* worst status over the six load cases */
+    int iters[6];           /* iterations per load case */
+    double relres[6];       /* final ||f - K u|| / ||f|| per load case (FP64, unique DOFs) */
+    double seconds;         /* wall time of the solve call */
+} homo3d_solve_stats;
+
+void homo3d_default_config(homo3d_config* cfg);
+int homo3d_create(const homo3d_config* cfg, homo3d_handle** out);
+int homo3d_destroy(homo3d_handle* h);
+const char* homo3d_last_error(const homo3d_handle* h);
+int homo3d_levels(const homo3d_handle* h, int* sizes, int max_levels);
+/* Opaque hipStream_t the handle launches on (for event timing by the caller). */
+void* homo3d_stream(homo3d_handle* h);
+
+int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_device);
+int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* stats);
+int homo3d_elastic_matrix(homo3d_handle* h, double C[36]);
+int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int on_device);
+
+/* Grid / MG level operations (all six load cases at once). */
+int homo3d_field_copy(homo3d_handle* h, int level, int field, float* host, int to_device);
+int homo3d_macro_rhs(homo3d_handle* h);
+int homo3d_residual(homo3d_handle* h, int level);
+int homo3d_apply_K(homo3d_handle* h);              /* r = K u on the finest level (pinned) */
+int homo3d_smooth(homo3d_handle* h, int level, double omega, int reverse);
+int homo3d_restrict(homo3d_handle* h, int level);  /* f[level] = P^T r[level-1] */
+int homo3d_prolong(homo3d_handle* h, int level);   /* u[level] += P u[level+1] */
+int homo3d_coarse_solve(homo3d_handle* h);
+int homo3d_vcycle(homo3d_handle* h, int symmetric);
+int homo3d_get_stencil(homo3d_handle* h, int level, float* host); /* [N][27][3][3] */
+
+/* Density-update kernels.  Arrays are lexicographic with row pitch `pitch` (elements). */
+int homo3d_symmetrize(homo3d_handle* h, float* field, int pitch, int sym, int on_device);
+int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch, double radius, int periodic,
+                     int on_device);
+int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch, double vol, double min_rho,
+                     double step, double damp, int on_device);
+
+/* Seeded randc density (project generator), written to host memory, n^3 floats, pitch n. */
+int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, float* out);
+
+/* Kernel timing: enable event pairs around the named kernel's launches, then read totals. */
+int homo3d_profile_enable(homo3d_handle* h, int enable);
+int homo3d_profile_get(homo3d_handle* h, const char* kernel, long* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HOMO3D_H */
