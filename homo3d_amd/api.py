@@ -47,6 +47,7 @@ EXPORTS = {
                                           ctypes.c_int, ctypes.c_int]),
     "homo3d_field_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_int]),
+    "homo3d_solution_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "homo3d_macro_rhs": (ctypes.c_int, [ctypes.c_void_p]),
     "homo3d_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_apply_K": (ctypes.c_int, [ctypes.c_void_p]),
@@ -233,6 +234,15 @@ class Homogenization:
     def set_field(self, level, field, arr):
         a = np.ascontiguousarray(arr, dtype=np.float32).reshape(self.level_shape(level))
         self._chk(lib().homo3d_field_copy(self._h, level, field, _ptr(a), 1), "field_copy")
+
+    def get_solution64(self):
+        out = np.zeros(self.level_shape(0), dtype=np.float64)
+        self._chk(lib().homo3d_solution_copy(self._h, _ptr(out), 0), "solution_copy")
+        return out
+
+    def set_solution64(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.level_shape(0))
+        self._chk(lib().homo3d_solution_copy(self._h, _ptr(a), 1), "solution_copy")
 
     def macro_rhs(self):
         self._chk(lib().homo3d_macro_rhs(self._h), "macro_rhs")

@@ -53,6 +53,7 @@ struct homo3d_handle {
     float* d_kchi = nullptr;
     float* d_G = nullptr;
     float* d_w = nullptr;
+    float* d_x = nullptr;       // raw cell values x (FP32 input), for FP64 weights
     float* d_stage = nullptr;   // pitched staging buffer for host densities / sensitivities
     size_t stage_elems = 0;
     double* d_Ainv = nullptr;
@@ -60,6 +61,11 @@ struct homo3d_handle {
     // solution fields (warm start) and PCG work vectors, level-0 sized, 6 load cases
     float* X = nullptr;
     float* F = nullptr;
+    double* U64 = nullptr;      // FP64 solution of the refinement loop
+    double* d_ke64 = nullptr;
+    double* d_kchi64 = nullptr;
+    size_t partial_elems = 0;
+    bool x_dirty = false;       // X was written by the caller: widen it before a warm refined solve
     float* P = nullptr;
     float* Q = nullptr;
     float* Z = nullptr;
@@ -267,6 +273,7 @@ void prof_end(homo3d_handle* h, ProfEvent& ev) {
 // ---------------------------------------------------------------------------------------------
 void op_smooth(homo3d_handle* h, int l, float* u, const float* f, float omega, int reverse) {
     const Level& L = h->lv[l];
+    if (l == 0) launch_zero_pin(h->stream, L.N, u);
     for (int cc = 0; cc < 8; cc++) {
         const int c = reverse ? 7 - cc : cc;
         if (l == 0) {
@@ -342,6 +349,71 @@ int stage_in(homo3d_handle* h, const float* a, int pitch, int on_device, const f
     return HOMO3D_OK;
 }
 
+constexpr int kMaxOuter = 12;
+constexpr double kInnerTol = 1e-3;
+
+// MG-preconditioned CG on K x = r for the load cases not marked done: x (h->X) starts at zero, r
+// (h->Rr) holds the right-hand side and is updated in place to the recursive residual.  A load case
+// stops when ||r|| <= tol[r] * sqrt(r0n2[r]).  Symmetric V(1,1) preconditioner, FP64 inner products.
+// iters[] accumulates the iterations spent per load case.
+int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const bool* done, int maxit, int* iters) {
+    const Level& L0 = h->lv[0];
+    const size_t M = 3 * L0.N;
+    const size_t vb = sizeof(float) * kRhs * M;
+    double sc[6 * kRhs];
+    bool act[kRhs];
+    bool any = false;
+    for (int r = 0; r < kRhs; r++) {
+        act[r] = !done[r] && r0n2[r] > 0;
+        sc[5 * kRhs + r] = act[r] ? 1.0 : 0.0;
+        any = any || act[r];
+    }
+    if (!any) return HOMO3D_OK;
+    HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
+    vcycle(h, h->Z, h->Rr, 1, nullptr);
+    HIPCHK(h, hipMemcpyAsync(h->P, h->Z, vb, hipMemcpyDeviceToDevice, h->stream));
+    launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc);  // rz
+    double rr[kRhs];
+    for (int it = 0; it < maxit; it++) {
+        {
+            ProfEvent ev;
+            prof_begin(h, 1, ev);
+            launch_fine_apply(h->stream, L0.n, h->d_ke, h->d_w, h->P, nullptr, h->Q);
+            prof_end(h, ev);
+        }
+        launch_dot6(h->stream, M, h->P, h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
+        launch_pcg_alpha(h->stream, h->d_sc);
+        launch_pcg_update_xr(h->stream, M, h->d_sc, h->P, h->Q, h->X, h->Rr);
+        launch_dot6(h->stream, M, h->Rr, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
+        HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        bool changed = false, all = true;
+        for (int r = 0; r < kRhs; r++) {
+            if (!act[r]) continue;
+            iters[r]++;
+            if (rr[r] <= tol[r] * tol[r] * r0n2[r]) {
+                act[r] = false;
+                sc[5 * kRhs + r] = 0.0;
+                changed = true;
+            } else {
+                all = false;
+            }
+        }
+        if (all) break;
+        if (changed)
+            HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
+                                     h->stream));
+        HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
+        vcycle(h, h->Z, h->Rr, 1, nullptr);
+        launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
+        launch_pcg_beta(h->stream, h->d_sc);
+        launch_pcg_update_p(h->stream, M, h->d_sc, h->Z, h->P);
+    }
+    return HOMO3D_OK;
+}
+
 bool check_handle(homo3d_handle* h) { return h != nullptr; }
 
 int require_density(homo3d_handle* h) {
@@ -402,11 +474,15 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
     }
     const size_t N0 = h->lv[0].N, vb = sizeof(float) * 3 * kRhs * N0;
     if (hipMalloc(&h->X, vb) || hipMalloc(&h->F, vb) || hipMalloc(&h->P, vb) || hipMalloc(&h->Q, vb) ||
-        hipMalloc(&h->Z, vb) || hipMalloc(&h->Rr, vb) || hipMalloc(&h->d_w, sizeof(float) * N0))
+        hipMalloc(&h->Z, vb) || hipMalloc(&h->Rr, vb) || hipMalloc(&h->d_w, sizeof(float) * N0) ||
+        hipMalloc(&h->d_x, sizeof(float) * N0))
         return fail(HOMO3D_HIP_ERROR);
     hipMemset(h->X, 0, vb);
     hipMemset(h->F, 0, vb);
-    if (hipMalloc(&h->d_partial, sizeof(double) * 21 * kDotBlocks) || hipMalloc(&h->d_sc, sizeof(double) * 64) ||
+    h->partial_elems = std::max((size_t)21 * kDotBlocks, 6 * ((N0 + 255) / 256));
+    if (hipMalloc(&h->U64, sizeof(double) * 3 * kRhs * N0)) return fail(HOMO3D_HIP_ERROR);
+    hipMemset(h->U64, 0, sizeof(double) * 3 * kRhs * N0);
+    if (hipMalloc(&h->d_partial, sizeof(double) * h->partial_elems) || hipMalloc(&h->d_sc, sizeof(double) * 64) ||
         hipMalloc(&h->d_tmp, sizeof(double) * 64) || hipMalloc(&h->d_ftmp, sizeof(float) * 4096))
         return fail(HOMO3D_HIP_ERROR);
     // element matrix, macro loads and the level-1 Galerkin table
@@ -439,6 +515,20 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
             for (int k = 0; k < 9; k++) G[(c * 27 + D) * 9 + k] = (float)S[k];
         }
     }
+    if (hipMalloc(&h->d_ke64, sizeof(double) * 576) || hipMalloc(&h->d_kchi64, sizeof(double) * 144))
+        return fail(HOMO3D_HIP_ERROR);
+    {
+        std::vector<double> kchi64(144);
+        for (int i = 0; i < 144; i++) {
+            const int s_ = i / 24, r_ = i % 24;
+            double t = 0;
+            for (int b = 0; b < 8; b++)
+                for (int c = 0; c < 3; c++) t += h->ke[r_ * 24 + 3 * b + c] * macro_disp(b, s_, c);
+            kchi64[i] = t;
+        }
+        hipMemcpy(h->d_ke64, h->ke, sizeof(double) * 576, hipMemcpyHostToDevice);
+        hipMemcpy(h->d_kchi64, kchi64.data(), sizeof(double) * 144, hipMemcpyHostToDevice);
+    }
     if (hipMalloc(&h->d_ke, sizeof(float) * 576) || hipMalloc(&h->d_kchi, sizeof(float) * 144) ||
         hipMalloc(&h->d_G, sizeof(float) * G.size()) || hipMalloc(&h->d_gperm, sizeof(int) * 48) ||
         hipMalloc(&h->d_gflip, sizeof(int) * 48))
@@ -464,7 +554,7 @@ int homo3d_destroy(homo3d_handle* h) {
         hipFree(h->lv[l].r);
         hipFree(h->lv[l].st);
     }
-    void* ptrs[] = {h->d_ke, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->X, h->F, h->P, h->Q, h->Z,
+    void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->X, h->F, h->P, h->Q, h->Z,
                     h->Rr, h->d_partial, h->d_sc, h->d_tmp, h->d_ftmp, h->d_gperm, h->d_gflip};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -492,7 +582,7 @@ int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_de
     const float* dx = nullptr;
     int rc = stage_in(h, x, pitch, on_device, &dx);
     if (rc) return rc;
-    launch_set_weights(h->stream, n, pitch, (float)h->cfg.emin, dx, h->d_w);
+    launch_set_weights(h->stream, n, pitch, h->cfg.emin, dx, h->d_w, h->d_x);
     const float clamp = (float)h->cfg.diag_clamp;
     launch_galerkin_fine(h->stream, n, h->d_G, h->d_ke, h->d_w, h->lv[1].st, clamp);
     for (int l = 2; l < h->nl; l++) launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l - 1].st, h->lv[l].st, clamp);
@@ -580,67 +670,54 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
         for (int r = 0; r < kRhs; r++)
             if (!done[r]) status = std::max(status, (int)HOMO3D_NOT_CONVERGED);
     } else {
-        // r = F - K X ; z = M r ; p = z
-        op_residual(h, 0, h->X, h->F, h->Rr);
-        double sc[6 * kRhs];
-        for (int r = 0; r < kRhs; r++) sc[5 * kRhs + r] = (fn2[r] > 0) ? 1.0 : 0.0;
-        double rr[kRhs];
-        launch_dot6(h->stream, M, h->Rr, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
-        HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        for (int r = 0; r < kRhs; r++) {
-            if (fn2[r] <= 0 || std::sqrt(rr[r] / fn2[r]) <= tol) {
-                done[r] = true;
-                sc[5 * kRhs + r] = 0.0;
-            }
-        }
-        HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
-                                 h->stream));
-        HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
-        vcycle(h, h->Z, h->Rr, 1, nullptr);
-        HIPCHK(h, hipMemcpyAsync(h->P, h->Z, vb, hipMemcpyDeviceToDevice, h->stream));
-        launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc);  // rz
-        int it = 0;
-        bool all = std::all_of(done, done + kRhs, [](bool b) { return b; });
-        while (!all && it < maxit) {
-            it++;
-            {
-                ProfEvent ev;
-                prof_begin(h, 1, ev);
-                launch_fine_apply(h->stream, L0.n, h->d_ke, h->d_w, h->P, nullptr, h->Q);
-                prof_end(h, ev);
-            }
-            launch_dot6(h->stream, M, h->P, h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
-            launch_pcg_alpha(h->stream, h->d_sc);
-            launch_pcg_update_xr(h->stream, M, h->d_sc, h->P, h->Q, h->X, h->Rr);
-            launch_dot6(h->stream, M, h->Rr, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
-            HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
+        // FP64 iterative refinement: r = f(w) - K U64 in FP64, inner FP32 MG-PCG on K d = r, U64 += d.
+        const size_t M6 = kRhs * M;
+        launch_rhs_norm64(h->stream, L0.n, h->d_kchi64, h->d_x, h->cfg.emin, h->d_partial, h->d_tmp);
+        HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
+        if (!warm)
+            HIPCHK(h, hipMemsetAsync(h->U64, 0, sizeof(double) * M6, h->stream));
+        else if (h->x_dirty)
+            launch_widen_f64(h->stream, M6, h->X, h->U64);
+        h->x_dirty = false;
+        launch_zero_pin64(h->stream, L0.N, h->U64);
+        double rel[kRhs];
+        for (int outer = 0;; outer++) {
+            double rn2[kRhs];
+            launch_fine_residual64(h->stream, L0.n, h->d_ke64, h->d_kchi64, h->d_x, h->cfg.emin, h->U64, h->Rr,
+                                   h->d_partial, h->d_tmp, 1);
+            HIPCHK(h, hipMemcpyAsync(rn2, h->d_tmp, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(h, hipStreamSynchronize(h->stream));
-            bool changed = false;
-            all = true;
+            bool all = true;
             for (int r = 0; r < kRhs; r++) {
-                if (done[r]) continue;
-                iters[r] = it;
-                if (std::sqrt(rr[r] / fn2[r]) <= tol) {
-                    done[r] = true;
-                    sc[5 * kRhs + r] = 0.0;
-                    changed = true;
-                } else {
-                    all = false;
-                }
+                rel[r] = fn2[r] > 0 ? std::sqrt(rn2[r] / fn2[r]) : 0.0;
+                done[r] = rel[r] <= tol;
+                all = all && done[r];
             }
-            if (changed)
-                HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs,
-                                         hipMemcpyHostToDevice, h->stream));
-            if (all) break;
-            HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
-            vcycle(h, h->Z, h->Rr, 1, nullptr);
-            launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
-            launch_pcg_beta(h->stream, h->d_sc);
-            launch_pcg_update_p(h->stream, M, h->d_sc, h->Z, h->P);
+            if (all || outer >= kMaxOuter) break;
+            int total = 0;
+            for (int r = 0; r < kRhs; r++) total += iters[r];
+            if (total >= maxit * kRhs) break;
+            double tol_in[kRhs];
+            for (int r = 0; r < kRhs; r++) tol_in[r] = std::max(kInnerTol, 0.5 * tol / std::max(rel[r], 1e-300));
+            HIPCHK(h, hipMemsetAsync(h->X, 0, sizeof(float) * M6, h->stream));
+            rc = pcg_inner(h, rn2, tol_in, done, maxit, iters);
+            if (rc) return rc;
+            launch_add_correction(h->stream, M6, h->X, h->U64);
         }
         for (int r = 0; r < kRhs; r++)
             if (!done[r]) status = std::max(status, (int)HOMO3D_NOT_CONVERGED);
+        launch_round_f32(h->stream, M6, h->U64, h->X);
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        prof_flush(h);
+        if (stats) {
+            stats->status = status;
+            for (int r = 0; r < kRhs; r++) {
+                stats->iters[r] = iters[r];
+                stats->relres[r] = rel[r];
+            }
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return status;
     }
     // true final residual per load case
     op_residual(h, 0, h->X, h->F, h->Rr);
@@ -719,10 +796,27 @@ int homo3d_field_copy(homo3d_handle* h, int level, int field, float* host, int t
     if (!h || !host || level < 0 || level >= h->nl || field < 0 || field > 2) return HOMO3D_BAD_ARG;
     float* d = field_ptr(h, level, field);
     const size_t bytes = sizeof(float) * 3 * kRhs * h->lv[level].N;
-    if (to_device)
+    if (to_device) {
         HIPCHK(h, hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, h->stream));
+        if (level == 0 && field == HOMO3D_FIELD_U) h->x_dirty = true;
+    }
     else
         HIPCHK(h, hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_solution_copy(homo3d_handle* h, double* host, int to_device) {
+    if (!h || !host) return HOMO3D_BAD_ARG;
+    const size_t bytes = sizeof(double) * 3 * kRhs * h->lv[0].N;
+    if (to_device) {
+        HIPCHK(h, hipMemcpyAsync(h->U64, host, bytes, hipMemcpyHostToDevice, h->stream));
+        launch_zero_pin64(h->stream, h->lv[0].N, h->U64);
+        launch_round_f32(h->stream, 3 * kRhs * h->lv[0].N, h->U64, h->X);
+        h->x_dirty = false;
+    } else {
+        HIPCHK(h, hipMemcpyAsync(host, h->U64, bytes, hipMemcpyDeviceToHost, h->stream));
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
@@ -740,6 +834,7 @@ int homo3d_residual(homo3d_handle* h, int level) {
     if (!h || level < 0 || level >= h->nl) return HOMO3D_BAD_ARG;
     int rc = require_density(h);
     if (rc) return rc;
+    if (level == 0) launch_zero_pin(h->stream, h->lv[0].N, h->X);
     op_residual(h, level, field_ptr(h, level, 0), field_ptr(h, level, 1), h->lv[level].r);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
@@ -750,6 +845,7 @@ int homo3d_apply_K(homo3d_handle* h) {
     if (!h) return HOMO3D_BAD_ARG;
     int rc = require_density(h);
     if (rc) return rc;
+    launch_zero_pin(h->stream, h->lv[0].N, h->X);
     ProfEvent ev;
     prof_begin(h, 1, ev);
     launch_fine_apply(h->stream, h->lv[0].n, h->d_ke, h->d_w, h->X, nullptr, h->lv[0].r);

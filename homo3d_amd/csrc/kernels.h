@@ -10,6 +10,15 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, co
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* ke, const float* w,
                     float* u, const float* f);
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f);
+void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
+                            const double* u, float* r32, double* partial, double* out6, int do_f);
+void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
+                       double* out6);
+void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u);
+void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x);
+void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u);
+void launch_zero_pin(hipStream_t s, size_t N, float* u);
+void launch_zero_pin64(hipStream_t s, size_t N, double* u);
 void launch_stencil_apply(hipStream_t s, int n, const float* st, const float* u, const float* f, float* out);
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* st, float* u,
                        const float* f);
@@ -37,7 +46,7 @@ void launch_oc_filter(hipStream_t s, int n, int pitch, int h, int periodic, cons
 void launch_oc_trial(hipStream_t s, int n, int pitch, double g, double min_rho, double step, double damp,
                      const float* sens, const float* rho, float* out, double* partial, int nblocks, double* sum);
 void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* partial, int nblocks);
-void launch_set_weights(hipStream_t s, int n, int pitch, float emin, const float* x, float* w);
+void launch_set_weights(hipStream_t s, int n, int pitch, double emin, const float* x, float* w, float* xc);
 void launch_copy_pitched(hipStream_t s, int n, int pitch_in, int pitch_out, const float* in, float* out);
 
 }  // namespace homo3d

@@ -21,30 +21,65 @@ namespace {
 __device__ __forceinline__ int wrap_dec(int a, int n) { return a == 0 ? n - 1 : a - 1; }
 __device__ __forceinline__ int wrap_inc(int a, int n) { return a == n - 1 ? 0 : a + 1; }
 
+// Buffer-resource loads: one SGPR descriptor per array, a 32-bit VGPR byte offset per element and an
+// SGPR byte offset per (load case, component) plane.  Offsets past the array read as zero.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc make_rsrc(const void* p, size_t bytes) {
+    const uint32_t nrec = bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nrec, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(Rsrc r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float bload<float>(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
 // 3x3 block of the assembled fine operator between vertex v and its neighbour v+d:
 // S = sum over the cells containing both of w_e * KE[node(v)][node(v+d)].
 // Cell e (0..7) sits at offset (e&1, e>>1&1, e>>2) - 1 from v; v is its node 7-e.
-template <int D>
-__device__ __forceinline__ void fine_block(const float wc[8], const float* __restrict__ ke, float S[9]) {
+template <int D, typename T, typename W>
+__device__ __forceinline__ void fine_block(const W wc[8], const T* ke, T S[9]) {
     constexpr int dx = D % 3 - 1, dy = D / 3 % 3 - 1, dz = D / 9 - 1;
 #pragma unroll
-    for (int k = 0; k < 9; k++) S[k] = 0.f;
+    for (int k = 0; k < 9; k++) S[k] = T(0);
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
         const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
         if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
         const int a = 7 - e, b = bx + 2 * by + 4 * bz;
+        const T we = (T)wc[e];  // FP32 weights; the FP64 path passes its own array
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) S[i * 3 + j] = fmaf(wc[e], ke[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
+            for (int j = 0; j < 3; j++) S[i * 3 + j] = fma(we, ke[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
     }
+}
+
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < BS / 64; i++) s += sh[i];
+    }
+    __syncthreads();
+    return s;
 }
 
 struct FineCtx {
     int xs[3], ys[3], zs[3];
     float wc[8];
+    double wd[8];  // FP64 weights Emin + x (refinement residual only)
     __device__ __forceinline__ int nb(int d, int n) const {
         return xs[d % 3] + n * (ys[d / 3 % 3] + n * zs[d / 9]);
     }
@@ -54,43 +89,93 @@ __device__ __forceinline__ void fine_ctx(FineCtx& c, int x, int y, int z, int n,
     c.xs[0] = wrap_dec(x, n); c.xs[1] = x; c.xs[2] = wrap_inc(x, n);
     c.ys[0] = wrap_dec(y, n); c.ys[1] = y; c.ys[2] = wrap_inc(y, n);
     c.zs[0] = wrap_dec(z, n); c.zs[1] = z; c.zs[2] = wrap_inc(z, n);
+    const Rsrc rw = make_rsrc(w, (size_t)n * n * n * sizeof(float));
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         // cell at offset (ex-1, ey-1, ez-1): index xs[ex], ys[ey], zs[ez]
-        c.wc[e] = w[c.xs[e & 1] + n * (c.ys[(e >> 1) & 1] + n * c.zs[e >> 2])];
+        const uint32_t q = c.xs[e & 1] + n * (c.ys[(e >> 1) & 1] + n * c.zs[e >> 2]);
+        c.wc[e] = bload<float>(rw, q * 4u, 0);
     }
 }
 
-// acc[r][i] += sum_{d in mask} S_d u_{v+d}; pinned origin read as zero.
-template <int R, bool SKIP_SELF>
-__device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, const float* __restrict__ ke,
-                                            const float* __restrict__ u, float acc[R][3]) {
+// acc[r][i] += sum_{d in mask} S_d u_{v+d}.  The pinned origin's entries of u are zero by
+// construction (every entry point zeroes them before the fine operator reads u).
+// FP64 cell weights w = Emin + x from the raw cell values x (FP32 input data, exact in FP64).
+__device__ __forceinline__ void fine_ctx64(FineCtx& c, int x, int y, int z, int n, const float* __restrict__ xc,
+                                           double emin) {
+    c.xs[0] = wrap_dec(x, n); c.xs[1] = x; c.xs[2] = wrap_inc(x, n);
+    c.ys[0] = wrap_dec(y, n); c.ys[1] = y; c.ys[2] = wrap_inc(y, n);
+    c.zs[0] = wrap_dec(z, n); c.zs[1] = z; c.zs[2] = wrap_inc(z, n);
+    const Rsrc rw = make_rsrc(xc, (size_t)n * n * n * sizeof(float));
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const uint32_t q = c.xs[e & 1] + n * (c.ys[(e >> 1) & 1] + n * c.zs[e >> 2]);
+        c.wd[e] = emin + (double)bload<float>(rw, q * 4u, 0);
+        c.wc[e] = (float)c.wd[e];
+    }
+}
+
+// Neighbour sequence of a row: d = 0..26, skipping the vertex itself (13) when SKIP_SELF.
+template <int D, bool SKIP_SELF>
+struct NextNb {
+    static constexpr int value = (SKIP_SELF && D + 1 == 13) ? D + 2 : D + 1;
+};
+
+template <int R, typename A>
+__device__ __forceinline__ void load_nb(A (&dst)[R][3], Rsrc ru, uint32_t qo, uint32_t plane, int r0) {
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
-#define HOMO3D_GATHER_D(DD)                                                                        \
-    if (!(SKIP_SELF && DD == 13)) {                                                                \
-        float S[9];                                                                                \
-        fine_block<DD>(c.wc, ke, S);                                                               \
-        const int q = c.nb(DD, n);                                                                 \
-        const float pm = (q == 0) ? 0.f : 1.f;                                                     \
-        _Pragma("unroll") for (int r = 0; r < R; r++) {                                            \
-            const float u0 = pm * u[(size_t)(r * 3 + 0) * N + q];                                  \
-            const float u1 = pm * u[(size_t)(r * 3 + 1) * N + q];                                  \
-            const float u2 = pm * u[(size_t)(r * 3 + 2) * N + q];                                  \
-            acc[r][0] = fmaf(S[0], u0, fmaf(S[1], u1, fmaf(S[2], u2, acc[r][0])));                 \
-            acc[r][1] = fmaf(S[3], u0, fmaf(S[4], u1, fmaf(S[5], u2, acc[r][1])));                 \
-            acc[r][2] = fmaf(S[6], u0, fmaf(S[7], u1, fmaf(S[8], u2, acc[r][2])));                 \
-        }                                                                                          \
+        for (int i = 0; i < 3; i++) dst[r][i] = bload<A>(ru, qo, (uint32_t)((r0 + r) * 3 + i) * plane);
+}
+
+// One neighbour of the row: issue the loads of the next neighbour, build the 3x3 block of this one
+// from the cell weights and KE (in LDS), accumulate, recurse.
+template <int R, bool SKIP_SELF, int D, typename T, typename A>
+__device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke, Rsrc ru, uint32_t plane, int r0,
+                                            const A (&cur)[R][3], T (&acc)[R][3]) {
+    constexpr int DN = NextNb<D, SKIP_SELF>::value;
+    // keep each neighbour's block build next to its use (bounds register pressure)
+    __builtin_amdgcn_sched_barrier(0);
+    A nxt[R][3];
+    if constexpr (DN < 27) load_nb<R, A>(nxt, ru, (uint32_t)c.nb(DN, n) * (uint32_t)sizeof(A), plane, r0);
+    T S[9];
+    if constexpr (sizeof(T) == 8)
+        fine_block<D>(c.wd, ke, S);
+    else
+        fine_block<D>(c.wc, ke, S);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const T u0 = (T)cur[r][0], u1 = (T)cur[r][1], u2 = (T)cur[r][2];
+        acc[r][0] = fma(S[0], u0, fma(S[1], u1, fma(S[2], u2, acc[r][0])));
+        acc[r][1] = fma(S[3], u0, fma(S[4], u1, fma(S[5], u2, acc[r][1])));
+        acc[r][2] = fma(S[6], u0, fma(S[7], u1, fma(S[8], u2, acc[r][2])));
     }
-    HOMO3D_GATHER_D(0) HOMO3D_GATHER_D(1) HOMO3D_GATHER_D(2) HOMO3D_GATHER_D(3) HOMO3D_GATHER_D(4)
-    HOMO3D_GATHER_D(5) HOMO3D_GATHER_D(6) HOMO3D_GATHER_D(7) HOMO3D_GATHER_D(8) HOMO3D_GATHER_D(9)
-    HOMO3D_GATHER_D(10) HOMO3D_GATHER_D(11) HOMO3D_GATHER_D(12) HOMO3D_GATHER_D(13) HOMO3D_GATHER_D(14)
-    HOMO3D_GATHER_D(15) HOMO3D_GATHER_D(16) HOMO3D_GATHER_D(17) HOMO3D_GATHER_D(18) HOMO3D_GATHER_D(19)
-    HOMO3D_GATHER_D(20) HOMO3D_GATHER_D(21) HOMO3D_GATHER_D(22) HOMO3D_GATHER_D(23) HOMO3D_GATHER_D(24)
-    HOMO3D_GATHER_D(25) HOMO3D_GATHER_D(26)
-#undef HOMO3D_GATHER_D
+    if constexpr (DN < 27) gather_step<R, SKIP_SELF, DN, T, A>(c, n, ke, ru, plane, r0, nxt, acc);
+}
+
+// acc[r][i] = sum_d S_d u_{v+d} over the row (without d = 13 when SKIP_SELF), load cases r0..r0+R-1.
+// ke points to the element matrix staged in LDS.  The pinned origin's entries of u are zero by
+// construction (every entry point zeroes them before the fine operator reads u).
+template <int R, bool SKIP_SELF, typename T, typename A>
+__device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, const T* ke, const A* __restrict__ u,
+                                            T (&acc)[R][3], int r0 = 0) {
+    const Rsrc ru = make_rsrc(u, (size_t)18 * N * sizeof(A));
+    const uint32_t plane = (uint32_t)(N * sizeof(A));
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc[r][i] = T(0);
+    A first[R][3];
+    load_nb<R, A>(first, ru, (uint32_t)c.nb(0, n) * (uint32_t)sizeof(A), plane, r0);
+    gather_step<R, SKIP_SELF, 0, T, A>(c, n, ke, ru, plane, r0, first, acc);
+}
+
+// stage the element matrix in LDS (all threads of the block take part)
+template <typename T>
+__device__ __forceinline__ void stage_ke(T* ke_s, const T* __restrict__ ke) {
+    for (int i = threadIdx.x; i < 576; i += blockDim.x) ke_s[i] = ke[i];
+    __syncthreads();
 }
 
 // Block Gauss-Seidel / SOR update of one vertex: x, y, z in turn (z, y, x when REVERSE)
@@ -126,16 +211,18 @@ __device__ __forceinline__ void colour_vertex(long t, int n, int colour, int& x,
 // MODE 0: out = K u; MODE 1: out = f - K u.  Pinned origin row is zero.
 // (update_residual_otf_kernel_1 grid.cu:1277-1461; v3_stencilOnLeft_kernel grid.cu:3686-3857)
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_fine_apply(int n, size_t N, const float* __restrict__ ke,
+__global__ __launch_bounds__(256, 4) void k_fine_apply(int n, size_t N, const float* __restrict__ ke,
                                                     const float* __restrict__ w, const float* __restrict__ u,
                                                     const float* __restrict__ f, float* __restrict__ out) {
+    __shared__ float ke_s[576];
+    stage_ke(ke_s, ke);
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (long)N) return;
     const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
     FineCtx c;
     fine_ctx(c, x, y, z, n, w);
     float acc[R][3];
-    fine_gather<R, false>(c, n, N, ke, u, acc);
+    fine_gather<R, false, float, float>(c, n, N, ke_s, u, acc);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -148,9 +235,11 @@ __global__ __launch_bounds__(256) void k_fine_apply(int n, size_t N, const float
 
 // One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
 template <int R, bool REVERSE>
-__global__ __launch_bounds__(256) void k_fine_gs(int n, size_t N, int colour, float omega,
+__global__ __launch_bounds__(256, 4) void k_fine_gs(int n, size_t N, int colour, float omega,
                                                  const float* __restrict__ ke, const float* __restrict__ w,
                                                  float* __restrict__ u, const float* __restrict__ f) {
+    __shared__ float ke_s[576];
+    stage_ke(ke_s, ke);
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long h = n >> 1;
     if (t >= h * h * h) return;
@@ -167,9 +256,9 @@ __global__ __launch_bounds__(256) void k_fine_gs(int n, size_t N, int colour, fl
     FineCtx c;
     fine_ctx(c, x, y, z, n, w);
     float acc[R][3];
-    fine_gather<R, true>(c, n, N, ke, u, acc);
+    fine_gather<R, true, float, float>(c, n, N, ke_s, u, acc);
     float Ks[9];
-    fine_block<13>(c.wc, ke, Ks);
+    fine_block<13, float>(c.wc, ke_s, Ks);
 #pragma unroll
     for (int r = 0; r < R; r++) {
         float uu[3], ff[3];
@@ -204,6 +293,100 @@ __global__ __launch_bounds__(256) void k_macro_rhs(int n, size_t N, const float*
         }
 }
 
+// Iterative-refinement residual in FP64: r = f(w) - K u with u the FP64 solution, f the macro-strain
+// loads rebuilt from the cell weights in FP64 (kchi64 = KE chi0 in FP64) and every product
+// accumulated in FP64.  Writes r rounded to FP32 (the inner solve's right-hand side) and per-block
+// FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0 (r = -K u).
+template <int R, bool DO_F>
+__global__ __launch_bounds__(256, 2) void k_fine_residual64(int n, size_t N, int r0, const double* __restrict__ ke,
+                                                         const double* __restrict__ kchi, const float* __restrict__ xc,
+                                                         double emin,
+                                                         const double* __restrict__ u, float* __restrict__ r32,
+                                                         double* __restrict__ partial) {
+    __shared__ double sh[4];
+    __shared__ double ke_s[576];
+    stage_ke(ke_s, ke);
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double nrm[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) nrm[r] = 0.0;
+    if (v < (long)N) {
+        const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+        FineCtx c;
+        fine_ctx64(c, x, y, z, n, xc, emin);
+        double acc[R][3];
+        fine_gather<R, false, double, double>(c, n, N, ke_s, u, acc, r0);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double f = 0.0;
+                if (DO_F) {
+#pragma unroll
+                    for (int e = 0; e < 8; e++) f = fma(c.wd[e], kchi[(r0 + r) * 24 + 3 * (7 - e) + i], f);
+                }
+                const double val = (v == 0) ? 0.0 : f - acc[r][i];
+                r32[(size_t)((r0 + r) * 3 + i) * N + v] = (float)val;
+                nrm[r] = fma(val, val, nrm[r]);
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const double s = block_sum<256>(nrm[r], sh);
+        if (threadIdx.x == 0) partial[(r0 + r) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// |f|^2 per load case of the FP64 macro-strain loads (pinned row excluded).
+__global__ __launch_bounds__(256) void k_rhs_norm64(int n, size_t N, const double* __restrict__ kchi,
+                                                    const float* __restrict__ xc, double emin,
+                                                    double* __restrict__ partial) {
+    __shared__ double sh[4];
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double nrm[6] = {0, 0, 0, 0, 0, 0};
+    if (v < (long)N && v != 0) {
+        const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+        FineCtx c;
+        fine_ctx64(c, x, y, z, n, xc, emin);
+#pragma unroll
+        for (int s = 0; s < 6; s++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double f = 0.0;
+#pragma unroll
+                for (int e = 0; e < 8; e++) f = fma(c.wd[e], kchi[s * 24 + 3 * (7 - e) + i], f);
+                nrm[s] = fma(f, f, nrm[s]);
+            }
+    }
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        const double t = block_sum<256>(nrm[s], sh);
+        if (threadIdx.x == 0) partial[s * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// U64 += d (FP32 correction), all load cases
+__global__ __launch_bounds__(256) void k_add_correction(size_t M, const float* __restrict__ d, double* __restrict__ u) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) u[i] += (double)d[i];
+}
+
+__global__ __launch_bounds__(256) void k_round_f32(size_t M, const double* __restrict__ u, float* __restrict__ x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) x[i] = (float)u[i];
+}
+
+__global__ __launch_bounds__(256) void k_widen_f64(size_t M, const float* __restrict__ x, double* __restrict__ u) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) u[i] = (double)x[i];
+}
+
+// zero the pinned origin's entries of a 6-load-case field
+template <typename T>
+__global__ void k_zero_pin(size_t N, T* __restrict__ u) {
+    if (threadIdx.x < 18) u[(size_t)threadIdx.x * N] = T(0);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Coarse levels (stored 27-point stencils)
 // ---------------------------------------------------------------------------------------------
@@ -215,6 +398,10 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, int n, size_
     const int xs[3] = {wrap_dec(x, n), x, wrap_inc(x, n)};
     const int ys[3] = {wrap_dec(y, n), y, wrap_inc(y, n)};
     const int zs[3] = {wrap_dec(z, n), z, wrap_inc(z, n)};
+    const Rsrc rs = make_rsrc(st, (size_t)243 * N * sizeof(float));
+    const Rsrc ru = make_rsrc(u, (size_t)18 * N * sizeof(float));
+    const uint32_t plane = (uint32_t)(N * sizeof(float));
+    const uint32_t vo = (uint32_t)v * 4u;
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -224,13 +411,13 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, int n, size_
         if (SKIP_SELF && d == 13) continue;
         float S[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) S[k] = st[(size_t)(d * 9 + k) * N + v];
-        const int q = xs[d % 3] + n * (ys[d / 3 % 3] + n * zs[d / 9]);
+        for (int k = 0; k < 9; k++) S[k] = bload<float>(rs, vo, (uint32_t)(d * 9 + k) * plane);
+        const uint32_t qo = (uint32_t)(xs[d % 3] + n * (ys[d / 3 % 3] + n * zs[d / 9])) * 4u;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const float u0 = u[(size_t)(r * 3 + 0) * N + q];
-            const float u1 = u[(size_t)(r * 3 + 1) * N + q];
-            const float u2 = u[(size_t)(r * 3 + 2) * N + q];
+            const float u0 = bload<float>(ru, qo, (uint32_t)(r * 3 + 0) * plane);
+            const float u1 = bload<float>(ru, qo, (uint32_t)(r * 3 + 1) * plane);
+            const float u2 = bload<float>(ru, qo, (uint32_t)(r * 3 + 2) * plane);
             acc[r][0] = fmaf(S[0], u0, fmaf(S[1], u1, fmaf(S[2], u2, acc[r][0])));
             acc[r][1] = fmaf(S[3], u0, fmaf(S[4], u1, fmaf(S[5], u2, acc[r][1])));
             acc[r][2] = fmaf(S[6], u0, fmaf(S[7], u1, fmaf(S[8], u2, acc[r][2])));
@@ -525,21 +712,6 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int m, size_t Nc, const do
 // ---------------------------------------------------------------------------------------------
 // FP64 reductions and vector updates (per load case)
 // ---------------------------------------------------------------------------------------------
-template <int BS>
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < BS / 64; i++) s += sh[i];
-    }
-    __syncthreads();
-    return s;
-}
-
 // partial[r * nblocks + b] = sum over block b's grid-stride share of a_r . b_r  (3N values per r)
 template <int R>
 __global__ __launch_bounds__(256) void k_dot(size_t M, const float* __restrict__ a, const float* __restrict__ b,
@@ -813,13 +985,15 @@ __global__ __launch_bounds__(256) void k_absmax(long N, int n, int pitch, const 
 }
 
 // pitched lexicographic density -> dense cell weights w = emin + x
-__global__ __launch_bounds__(256) void k_set_weights(int n, int pitch, float emin, const float* __restrict__ x,
-                                                     float* __restrict__ w) {
+__global__ __launch_bounds__(256) void k_set_weights(int n, int pitch, double emin, const float* __restrict__ x,
+                                                     float* __restrict__ w, float* __restrict__ xc) {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long N = (long)n * n * n;
     if (e >= N) return;
     const int xx = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
-    w[e] = emin + x[xx + ((size_t)y + (size_t)z * n) * pitch];
+    const float xv = x[xx + ((size_t)y + (size_t)z * n) * pitch];
+    xc[e] = xv;
+    w[e] = (float)(emin + (double)xv);
 }
 
 __global__ __launch_bounds__(256) void k_copy_pitched(int n, int pitch_in, int pitch_out, const float* __restrict__ in,
@@ -860,6 +1034,49 @@ void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, 
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {
     const size_t N = (size_t)n * n * n;
     hipLaunchKernelGGL(k_macro_rhs, dim3(nblk(N)), dim3(256), 0, s, n, N, kchi, w, f);
+}
+
+void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
+                            const double* u, float* r32, double* partial, double* out6, int do_f) {
+    const size_t N = (size_t)n * n * n;
+    const unsigned nb = nblk(N);
+    for (int r0 = 0; r0 < 6; r0 += 1) {
+        if (do_f)
+            hipLaunchKernelGGL((k_fine_residual64<1, true>), dim3(nb), dim3(256), 0, s, n, N, r0, ke, kchi, xc, emin, u,
+                               r32, partial);
+        else
+            hipLaunchKernelGGL((k_fine_residual64<1, false>), dim3(nb), dim3(256), 0, s, n, N, r0, ke, kchi, xc, emin, u,
+                               r32, partial);
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)nb, partial, out6);
+}
+
+void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
+                       double* out6) {
+    const size_t N = (size_t)n * n * n;
+    const unsigned nb = nblk(N);
+    hipLaunchKernelGGL(k_rhs_norm64, dim3(nb), dim3(256), 0, s, n, N, kchi, xc, emin, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)nb, partial, out6);
+}
+
+void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u) {
+    hipLaunchKernelGGL(k_add_correction, dim3(nblk(M)), dim3(256), 0, s, M, d, u);
+}
+
+void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x) {
+    hipLaunchKernelGGL(k_round_f32, dim3(nblk(M)), dim3(256), 0, s, M, u, x);
+}
+
+void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u) {
+    hipLaunchKernelGGL(k_widen_f64, dim3(nblk(M)), dim3(256), 0, s, M, x, u);
+}
+
+void launch_zero_pin(hipStream_t s, size_t N, float* u) {
+    hipLaunchKernelGGL((k_zero_pin<float>), dim3(1), dim3(64), 0, s, N, u);
+}
+
+void launch_zero_pin64(hipStream_t s, size_t N, double* u) {
+    hipLaunchKernelGGL((k_zero_pin<double>), dim3(1), dim3(64), 0, s, N, u);
 }
 
 void launch_stencil_apply(hipStream_t s, int n, const float* st, const float* u, const float* f, float* out) {
@@ -967,9 +1184,9 @@ void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* parti
     hipLaunchKernelGGL(k_absmax, dim3(nblocks), dim3(256), 0, s, N, n, pitch, a, partial);
 }
 
-void launch_set_weights(hipStream_t s, int n, int pitch, float emin, const float* x, float* w) {
+void launch_set_weights(hipStream_t s, int n, int pitch, double emin, const float* x, float* w, float* xc) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_set_weights, dim3(nblk(N)), dim3(256), 0, s, n, pitch, emin, x, w);
+    hipLaunchKernelGGL(k_set_weights, dim3(nblk(N)), dim3(256), 0, s, n, pitch, emin, x, w, xc);
 }
 
 void launch_copy_pitched(hipStream_t s, int n, int pitch_in, int pitch_out, const float* in, float* out) {

@@ -27,6 +27,8 @@
  *   homo3d_coarse_solve      Grid::solveHostEquation (grid.h:236, grid.cpp:349-374)
  *   homo3d_vcycle            MG::v_cycle (MG.h:26, MG.cpp:46-70)
  *   homo3d_field_copy        Grid::v3_upload / v3_download (grid.h:339-340)
+ *   homo3d_solution_copy     Grid::uchar_h read/write, the stored load-case solutions used as warm start
+ *                            (grid.h:142, grid.cpp:341-345, 459-462), here the FP64 refinement iterate
  *   homo3d_get_stencil       Grid::stencil_g read-back (grid.h:126)
  *   homo3d_oc_filter         OCOptimizer::filterSens (optimization/OCOptimizer.h:17-18)
  *   homo3d_oc_update         OCOptimizer::update (optimization/OCOptimizer.h:19-20)
@@ -103,6 +105,8 @@ int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pi
 
 /* Grid / MG level operations (all six load cases at once). */
 int homo3d_field_copy(homo3d_handle* h, int level, int field, float* host, int to_device);
+/* FP64 solution of the six load cases ([6][3][n^3] doubles) kept by the refined MG-PCG solver. */
+int homo3d_solution_copy(homo3d_handle* h, double* host, int to_device);
 int homo3d_macro_rhs(homo3d_handle* h);
 int homo3d_residual(homo3d_handle* h, int level);
 int homo3d_apply_K(homo3d_handle* h);              /* r = K u on the finest level (pinned) */
