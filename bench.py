@@ -22,6 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bench_256cube_pcg_traffic.json")
+PROF_KERNEL = {"fine_gs": "k_fine_gs", "fine_apply": "k_fine_apply<6, 0>", "fine_residual": "k_fine_apply<6, 1>"}
 METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; stencil GB/s"
 
 
@@ -39,6 +41,16 @@ def kernel_bytes(name, n, R=6, s_u=4, s_f=4, s_w=4):
     if name == "fine_gs":         # one colour = 1/8 of an in-place sweep: u read+write, f read, w read
         return N * (3 * R * (2 * s_u + s_f) + s_w) / 8.0
     raise KeyError(name)
+
+
+def measured_traffic(name, n):
+    """HBM bytes per launch of `name` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
+    if n != 256 or not os.path.exists(TRAFFIC_JSON):
+        return None
+    data = json.load(open(TRAFFIC_JSON))["kernels"]
+    key = PROF_KERNEL[name]
+    vals = [v["traffic_bytes"] for k, v in data.items() if k.startswith(key)]
+    return sum(vals) / len(vals) if vals else None
 
 
 def cpu_baseline(n_sample, n_target, tol):
@@ -118,7 +130,7 @@ def main():
     avg_s = ms / cnt * 1e-3
     achieved = kernel_bytes(dom, n) / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dom, n), "kernel": dom,
                 "avg_launch_ms": round(ms / cnt, 5), "algorithmic_bytes_per_launch": kernel_bytes(dom, n)}
     out = {
         "metric": METRIC,

@@ -8,6 +8,7 @@ The classes mirror the reference's host API (lavenklau/homo3d):
   * ``HomoConfig``        cfg::HomoConfig (cmdline.h:39-60)
   * ``Homogenization``    homo::Homogenization (homogenization/homogenization.h:17-61)
   * ``OCOptimizer``       homo::OCOptimizer (optimization/OCOptimizer.h:7-20)
+  * ``MMAOptimizer``      homo::MMAOptimizer (optimization/mmaOptimizer.h:12-34)
 """
 import ctypes
 import os
@@ -16,8 +17,9 @@ import numpy as np
 
 from .api import (HOMO3D_OK, HOMO3D_NOT_CONVERGED, HOMO3D_DIVERGED, HOMO3D_BAD_ARG, HOMO3D_HIP_ERROR,
                   HOMO3D_NO_SUPPORT, SOLVER_MG, SOLVER_PCG, FIELD_U, FIELD_F, FIELD_R, SYM, Homo3dError,
-                  lib, lib_path, HomoConfig, Homogenization, OCOptimizer, randc_density)
+                  lib, lib_path, HomoConfig, Homogenization, OCOptimizer, MMAOptimizer, randc_density)
 
 __all__ = ["HOMO3D_OK", "HOMO3D_NOT_CONVERGED", "HOMO3D_DIVERGED", "HOMO3D_BAD_ARG", "HOMO3D_HIP_ERROR",
            "HOMO3D_NO_SUPPORT", "SOLVER_MG", "SOLVER_PCG", "FIELD_U", "FIELD_F", "FIELD_R", "SYM",
-           "Homo3dError", "lib", "lib_path", "HomoConfig", "Homogenization", "OCOptimizer", "randc_density"]
+           "Homo3dError", "lib", "lib_path", "HomoConfig", "Homogenization", "OCOptimizer", "MMAOptimizer",
+           "randc_density"]

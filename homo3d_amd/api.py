@@ -64,6 +64,16 @@ EXPORTS = {
     "homo3d_oc_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int]),
+    "homo3d_mma_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "homo3d_mma_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_mma_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "homo3d_mma_set_bound": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "homo3d_mma_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    "homo3d_mma_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_int)]),
     "homo3d_randc_density": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_void_p]),
     "homo3d_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -314,3 +324,52 @@ class OCOptimizer:
                                              float(self.minRho), float(self.step_limit), float(self.damp), 0),
                       "oc_update")
         return r
+
+
+class MMAOptimizer:
+    """homo::MMAOptimizer(int m, int n, double a0, double a, double c, double d)
+    (optimization/mmaOptimizer.h:12-34): Svanberg's MMA step on the GPU in FP64."""
+
+    def __init__(self, m, n, a0=1.0, a=0.0, c=1000.0, d=1.0, device=0):
+        self.m, self.n = int(m), int(n)
+        self._h = ctypes.c_void_p()
+        rc = lib().homo3d_mma_create(self.m, self.n, float(a0), float(a), float(c), float(d), int(device),
+                                     ctypes.byref(self._h))
+        if rc:
+            raise Homo3dError(rc, "homo3d_mma_create")
+
+    def close(self):
+        if self._h:
+            lib().homo3d_mma_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise Homo3dError(rc, "%s: %s" % (what, lib().homo3d_mma_last_error(self._h).decode()))
+
+    def setBound(self, xmin, xmax):
+        self._chk(lib().homo3d_mma_set_bound(self._h, float(xmin), float(xmax)), "setBound")
+
+    def update(self, itn, x, dfdx, gval, dgdx):
+        """One MMA step; returns the new design (float32).  dgdx: sequence of m gradient arrays."""
+        xv = np.ascontiguousarray(x, dtype=np.float32).ravel().copy()
+        df = np.ascontiguousarray(dfdx, dtype=np.float32).ravel()
+        gv = np.ascontiguousarray(gval, dtype=np.float32).ravel()
+        dg = [np.ascontiguousarray(g, dtype=np.float32).ravel() for g in dgdx]
+        ptrs = (ctypes.c_void_p * self.m)(*[g.ctypes.data for g in dg])
+        self._chk(lib().homo3d_mma_update(self._h, int(itn), _ptr(xv), _ptr(df), _ptr(gv), ptrs, 0), "update")
+        return xv
+
+    def state(self):
+        x = np.zeros(self.n)
+        lo = np.zeros(self.n)
+        up = np.zeros(self.n)
+        it = ctypes.c_int()
+        self._chk(lib().homo3d_mma_state(self._h, _ptr(x), _ptr(lo), _ptr(up), ctypes.byref(it)), "state")
+        return x, lo, up, it.value

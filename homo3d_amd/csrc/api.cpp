@@ -27,6 +27,7 @@ constexpr int kDotBlocks = 1024;
 struct Level {
     int n = 0;
     size_t N = 0;
+    int lex = 0;  // 1: x + n(y + n z) (coarsest level); 0: parity-blocked (see store_index)
     float* u = nullptr;
     float* f = nullptr;
     float* r = nullptr;
@@ -172,6 +173,32 @@ int level_sizes(int n, int* sizes) {
     return k + 1;
 }
 
+// Storage index of vertex (x, y, z) on a level.  All levels but the coarsest are parity-blocked: the
+// eight classes (x&1, y&1, z&1) are contiguous (n/2)^3 blocks, block b = (x&1) | (y&1)<<1 | (z&1)<<2,
+// lexicographic inside; a GS colour is one block.  The coarsest level is lexicographic.
+size_t store_index(const Level& L, int x, int y, int z) {
+    if (L.lex) return x + (size_t)L.n * (y + (size_t)L.n * z);
+    const size_t h = L.n / 2;
+    const size_t b = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+    return b * h * h * h + (x >> 1) + h * ((y >> 1) + h * (z >> 1));
+}
+
+// Copy `planes` planes of L.N values between lexicographic order (the C ABI's) and storage order.
+template <typename T>
+void permute_planes(const Level& L, const T* in, T* out, int planes, bool to_storage) {
+    for (int z = 0; z < L.n; z++)
+        for (int y = 0; y < L.n; y++)
+            for (int x = 0; x < L.n; x++) {
+                const size_t v = x + (size_t)L.n * (y + (size_t)L.n * z), sv = store_index(L, x, y, z);
+                for (int p = 0; p < planes; p++) {
+                    if (to_storage)
+                        out[(size_t)p * L.N + sv] = in[(size_t)p * L.N + v];
+                    else
+                        out[(size_t)p * L.N + v] = in[(size_t)p * L.N + sv];
+                }
+            }
+}
+
 // Host FP64 Cholesky (lower) and explicit inverse of the pinned coarsest system.
 bool cholesky_inverse(int m, std::vector<double>& A) {
     for (int j = 0; j < m; j++) {
@@ -295,7 +322,7 @@ void op_residual(homo3d_handle* h, int l, const float* u, const float* f, float*
         launch_fine_apply(h->stream, L.n, h->d_ke, h->d_w, u, f, r);
         prof_end(h, ev);
     } else {
-        launch_stencil_apply(h->stream, L.n, L.st, u, f, r);
+        launch_stencil_apply(h->stream, L.n, L.lex, L.st, u, f, r);
     }
 }
 
@@ -311,7 +338,7 @@ void vcycle(homo3d_handle* h, float* u0, const float* f0, int symmetric, double*
         op_smooth(h, l, u, f, omega, 0);
         op_residual(h, l, u, f, L.r);
         if (l == 0 && rn2) launch_dot6(h->stream, 3 * L.N, L.r, L.r, h->d_partial, kDotBlocks, rn2);
-        launch_restrict(h->stream, L.n, L.r, h->lv[l + 1].f);
+        launch_restrict(h->stream, L.n, h->lv[l + 1].lex, L.r, h->lv[l + 1].f);
     }
     const Level& Lc = h->lv[h->nl - 1];
     launch_coarse_solve(h->stream, Lc.n, h->d_Ainv, Lc.f, Lc.u);
@@ -319,7 +346,7 @@ void vcycle(homo3d_handle* h, float* u0, const float* f0, int symmetric, double*
         Level& L = h->lv[l];
         float* u = l == 0 ? u0 : L.u;
         const float* f = l == 0 ? f0 : L.f;
-        launch_prolong(h->stream, L.n, h->lv[l + 1].u, u, l == 0 ? 1 : 0);
+        launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, u, l == 0 ? 1 : 0);
         op_smooth(h, l, u, f, omega, symmetric ? 1 : 0);
     }
 }
@@ -416,7 +443,20 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
 
 bool check_handle(homo3d_handle* h) { return h != nullptr; }
 
+// on_device < 0: classify the pointer (device or managed memory -> 1, anything else -> 0)
+int resolve_on_device(const void* p, int on_device) {
+    if (on_device >= 0) return on_device;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) ? 1 : 0;
+}
+
 int require_density(homo3d_handle* h) {
+    if (h->cfg.solver == HOMO3D_SOLVER_NONE)
+        return set_err(h, HOMO3D_BAD_ARG, "density-only handle: no solver hierarchy");
     if (!h->density_set) return set_err(h, HOMO3D_BAD_ARG, "density not set: call homo3d_update_density first");
     return HOMO3D_OK;
 }
@@ -449,22 +489,36 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
     const int n = cfg->reso[0];
     if (cfg->reso[1] != n || cfg->reso[2] != n) return HOMO3D_NO_SUPPORT;
     int sizes[kMaxLevels];
-    const int nl = level_sizes(n, sizes);
-    if (nl < 2) return HOMO3D_NO_SUPPORT;
+    const int nl = cfg->solver == HOMO3D_SOLVER_NONE ? (n >= 1 && n <= 1024 ? 1 : -1) : level_sizes(n, sizes);
+    if (cfg->solver == HOMO3D_SOLVER_NONE && nl == 1) sizes[0] = n;
+    if (nl < 1 || (cfg->solver != HOMO3D_SOLVER_NONE && nl < 2)) return HOMO3D_NO_SUPPORT;
     if (!(cfg->poisson_ratio > -1 && cfg->poisson_ratio < 0.5) || !(cfg->youngs_modulus > 0)) return HOMO3D_BAD_ARG;
     homo3d_handle* h = new homo3d_handle();
     h->cfg = *cfg;
     h->nl = nl;
+    const bool dens_only = cfg->solver == HOMO3D_SOLVER_NONE;
     auto fail = [&](int code) {
         homo3d_destroy(h);
         return code;
     };
     if (hipSetDevice(cfg->device) != hipSuccess) return fail(HOMO3D_HIP_ERROR);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(HOMO3D_HIP_ERROR);
+    if (dens_only) {
+        h->lv[0].n = n;
+        h->lv[0].N = (size_t)n * n * n;
+        h->partial_elems = 21 * kDotBlocks;
+        if (hipMalloc(&h->d_partial, sizeof(double) * h->partial_elems) || hipMalloc(&h->d_tmp, sizeof(double) * 64) ||
+            hipMalloc(&h->d_ftmp, sizeof(float) * 4096) || hipMalloc(&h->d_gperm, sizeof(int) * 48) ||
+            hipMalloc(&h->d_gflip, sizeof(int) * 48))
+            return fail(HOMO3D_HIP_ERROR);
+        *out = h;
+        return HOMO3D_OK;
+    }
     for (int l = 0; l < nl; l++) {
         Level& L = h->lv[l];
         L.n = sizes[l];
         L.N = (size_t)L.n * L.n * L.n;
+        L.lex = l == nl - 1;
         const size_t bytes = sizeof(float) * 3 * kRhs * L.N;
         if (hipMalloc(&L.u, bytes) || hipMalloc(&L.f, bytes) || hipMalloc(&L.r, bytes)) return fail(HOMO3D_HIP_ERROR);
         hipMemset(L.u, 0, bytes);
@@ -577,6 +631,8 @@ void* homo3d_stream(homo3d_handle* h) { return h ? (void*)h->stream : nullptr; }
 // every coarse level (MG::updateStencils, MG.cpp:341-354) and the coarsest FP64 factorisation.
 int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_device) {
     if (!h || !x) return HOMO3D_BAD_ARG;
+    if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
+    on_device = resolve_on_device(x, on_device);
     const int n = h->lv[0].n;
     if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
     const float* dx = nullptr;
@@ -584,8 +640,9 @@ int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_de
     if (rc) return rc;
     launch_set_weights(h->stream, n, pitch, h->cfg.emin, dx, h->d_w, h->d_x);
     const float clamp = (float)h->cfg.diag_clamp;
-    launch_galerkin_fine(h->stream, n, h->d_G, h->d_ke, h->d_w, h->lv[1].st, clamp);
-    for (int l = 2; l < h->nl; l++) launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l - 1].st, h->lv[l].st, clamp);
+    launch_galerkin_fine(h->stream, n, h->lv[1].lex, h->d_G, h->d_ke, h->d_w, h->lv[1].st, clamp);
+    for (int l = 2; l < h->nl; l++)
+        launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l].lex, h->lv[l - 1].st, h->lv[l].st, clamp);
     // coarsest dense system
     const Level& Lc = h->lv[h->nl - 1];
     std::vector<float> st(243 * Lc.N);
@@ -760,6 +817,7 @@ int homo3d_elastic_matrix(homo3d_handle* h, double C[36]) {
 // dJ/dx_e for dJ/dC = dC (Homogenization::Sensitivity, homogenization.cu:860-948), lexicographic pitched.
 int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int on_device) {
     if (!h || !dC || !sens) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(sens, on_device);
     int rc = require_density(h);
     if (rc) return rc;
     const int n = h->lv[0].n;
@@ -794,30 +852,42 @@ static float* field_ptr(homo3d_handle* h, int level, int field) {
 
 int homo3d_field_copy(homo3d_handle* h, int level, int field, float* host, int to_device) {
     if (!h || !host || level < 0 || level >= h->nl || field < 0 || field > 2) return HOMO3D_BAD_ARG;
+    if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
     float* d = field_ptr(h, level, field);
-    const size_t bytes = sizeof(float) * 3 * kRhs * h->lv[level].N;
+    const Level& L = h->lv[level];
+    const size_t bytes = sizeof(float) * 3 * kRhs * L.N;
+    std::vector<float> tmp(3 * kRhs * L.N);
     if (to_device) {
-        HIPCHK(h, hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, h->stream));
+        permute_planes(L, host, tmp.data(), 3 * kRhs, true);
+        HIPCHK(h, hipMemcpyAsync(d, tmp.data(), bytes, hipMemcpyHostToDevice, h->stream));
         if (level == 0 && field == HOMO3D_FIELD_U) h->x_dirty = true;
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    } else {
+        HIPCHK(h, hipMemcpyAsync(tmp.data(), d, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        permute_planes(L, tmp.data(), host, 3 * kRhs, false);
     }
-    else
-        HIPCHK(h, hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
 
 int homo3d_solution_copy(homo3d_handle* h, double* host, int to_device) {
     if (!h || !host) return HOMO3D_BAD_ARG;
-    const size_t bytes = sizeof(double) * 3 * kRhs * h->lv[0].N;
+    if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
+    const Level& L = h->lv[0];
+    const size_t bytes = sizeof(double) * 3 * kRhs * L.N;
+    std::vector<double> tmp(3 * kRhs * L.N);
     if (to_device) {
-        HIPCHK(h, hipMemcpyAsync(h->U64, host, bytes, hipMemcpyHostToDevice, h->stream));
-        launch_zero_pin64(h->stream, h->lv[0].N, h->U64);
-        launch_round_f32(h->stream, 3 * kRhs * h->lv[0].N, h->U64, h->X);
+        permute_planes(L, host, tmp.data(), 3 * kRhs, true);
+        HIPCHK(h, hipMemcpyAsync(h->U64, tmp.data(), bytes, hipMemcpyHostToDevice, h->stream));
+        launch_zero_pin64(h->stream, L.N, h->U64);
+        launch_round_f32(h->stream, 3 * kRhs * L.N, h->U64, h->X);
         h->x_dirty = false;
+        HIPCHK(h, hipStreamSynchronize(h->stream));
     } else {
-        HIPCHK(h, hipMemcpyAsync(host, h->U64, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(tmp.data(), h->U64, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        permute_planes(L, tmp.data(), host, 3 * kRhs, false);
     }
-    HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
 
@@ -867,14 +937,17 @@ int homo3d_smooth(homo3d_handle* h, int level, double omega, int reverse) {
 
 int homo3d_restrict(homo3d_handle* h, int level) {
     if (!h || level < 1 || level >= h->nl) return HOMO3D_BAD_ARG;
-    launch_restrict(h->stream, h->lv[level - 1].n, h->lv[level - 1].r, field_ptr(h, level, 1));
+    if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
+    launch_restrict(h->stream, h->lv[level - 1].n, h->lv[level].lex, h->lv[level - 1].r, field_ptr(h, level, 1));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
 
 int homo3d_prolong(homo3d_handle* h, int level) {
     if (!h || level < 0 || level >= h->nl - 1) return HOMO3D_BAD_ARG;
-    launch_prolong(h->stream, h->lv[level].n, field_ptr(h, level + 1, 0), field_ptr(h, level, 0), level == 0 ? 1 : 0);
+    if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
+    launch_prolong(h->stream, h->lv[level].n, h->lv[level + 1].lex, field_ptr(h, level + 1, 0), field_ptr(h, level, 0),
+                   level == 0 ? 1 : 0);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
@@ -901,17 +974,23 @@ int homo3d_vcycle(homo3d_handle* h, int symmetric) {
 
 int homo3d_get_stencil(homo3d_handle* h, int level, float* host) {
     if (!h || !host || level < 1 || level >= h->nl) return HOMO3D_BAD_ARG;
+    if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
     const Level& L = h->lv[level];
     std::vector<float> soa(243 * L.N);
     HIPCHK(h, hipMemcpyAsync(soa.data(), L.st, sizeof(float) * soa.size(), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    for (size_t v = 0; v < L.N; v++)
-        for (int k = 0; k < 243; k++) host[v * 243 + k] = soa[(size_t)k * L.N + v];
+    for (int z = 0; z < L.n; z++)
+        for (int y = 0; y < L.n; y++)
+            for (int x = 0; x < L.n; x++) {
+                const size_t v = x + (size_t)L.n * (y + (size_t)L.n * z), sv = store_index(L, x, y, z);
+                for (int k = 0; k < 243; k++) host[v * 243 + k] = soa[(size_t)k * L.N + sv];
+            }
     return HOMO3D_OK;
 }
 
 int homo3d_symmetrize(homo3d_handle* h, float* field, int pitch, int sym, int on_device) {
     if (!h || !field) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(field, on_device);
     if (sym == HOMO3D_SYM_NONE) return HOMO3D_OK;
     if (sym < 0 || sym > 3) return set_err(h, HOMO3D_BAD_ARG, "unknown symmetry %d", sym);
     const int n = h->lv[0].n;
@@ -941,6 +1020,7 @@ int homo3d_symmetrize(homo3d_handle* h, float* field, int pitch, int sym, int on
 int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch, double radius, int periodic,
                      int on_device) {
     if (!h || !sens || !rho) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(sens, on_device);
     const int n = h->lv[0].n;
     if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
     const int hw = (int)(radius + 0.5);
@@ -978,6 +1058,7 @@ int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch,
 int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch, double vol, double min_rho,
                      double step, double damp, int on_device) {
     if (!h || !sens || !rho) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(rho, on_device);
     const int n = h->lv[0].n;
     if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
     const size_t elems = (size_t)pitch * n * n;

@@ -19,14 +19,15 @@ void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x);
 void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u);
 void launch_zero_pin(hipStream_t s, size_t N, float* u);
 void launch_zero_pin64(hipStream_t s, size_t N, double* u);
-void launch_stencil_apply(hipStream_t s, int n, const float* st, const float* u, const float* f, float* out);
+void launch_stencil_apply(hipStream_t s, int n, int lex, const float* st, const float* u, const float* f,
+                          float* out);
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* st, float* u,
                        const float* f);
-void launch_restrict(hipStream_t s, int nf, const float* rf, float* fc);
-void launch_prolong(hipStream_t s, int nf, const float* uc, float* uf, int pin);
-void launch_galerkin_fine(hipStream_t s, int nf, const float* G, const float* ke, const float* w, float* stc,
+void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc);
+void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, float* uf, int pin);
+void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
                           float clamp);
-void launch_galerkin_coarse(hipStream_t s, int nf, const float* stf, float* stc, float clamp);
+void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const float* stf, float* stc, float clamp);
 void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float* f, float* u);
 void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks, double* out);
 void launch_sum_partials(hipStream_t s, int R, int nblocks, const double* partial, double* out);
@@ -49,4 +50,31 @@ void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* parti
 void launch_set_weights(hipStream_t s, int n, int pitch, double emin, const float* x, float* w, float* xc);
 void launch_copy_pitched(hipStream_t s, int n, int pitch_in, int pitch_out, const float* in, float* out);
 
+}  // namespace homo3d
+
+namespace homo3d {
+constexpr int kMmaMaxM = 4;
+constexpr int kMmaNewtonSums = kMmaMaxM + kMmaMaxM * (kMmaMaxM + 1) / 2;
+void launch_mma_setup(hipStream_t s, int m, long n, int iter, double move, const double* x, const double* xmin,
+                      const double* xmax, const double* xold1, const double* xold2, const double* df0,
+                      const double* dfdx, double* low, double* upp, double* alfa, double* beta, double* p0, double* q0,
+                      double* P, double* Q, double* part, int nb, double* out);
+void launch_mma_resid(hipStream_t s, int m, long n, const double* lam, double epsi, const double* x,
+                      const double* xsi, const double* eta, const double* low, const double* upp, const double* alfa,
+                      const double* beta, const double* p0, const double* q0, const double* P, const double* Q,
+                      double* part, double* partmax, int nb, double* out);
+void launch_mma_newton(hipStream_t s, int m, long n, const double* lam, double epsi, const double* x,
+                       const double* xsi, const double* eta, const double* low, const double* upp, const double* alfa,
+                       const double* beta, const double* p0, const double* q0, const double* P, const double* Q,
+                       double* delx, double* diagx, double* part, int nb, double* out);
+void launch_mma_dir(hipStream_t s, int m, long n, double epsi, const double* dlam, const double* x, const double* xsi,
+                    const double* eta, const double* low, const double* upp, const double* alfa, const double* beta,
+                    const double* P, const double* Q, const double* delx, const double* diagx, double* dx,
+                    double* dxsi, double* deta, double* partmax, int nb, double* out);
+void launch_mma_step(hipStream_t s, long n, double steg, const double* xo, const double* xsio, const double* etao,
+                     const double* dx, const double* dxsi, const double* deta, double* x, double* xsi, double* eta);
+void launch_mma_init(hipStream_t s, long n, const double* alfa, const double* beta, double* x, double* xsi,
+                     double* eta);
+void launch_f32_to_f64(hipStream_t s, long n, const float* a, double* b);
+void launch_f64_to_f32(hipStream_t s, long n, const double* a, float* b);
 }  // namespace homo3d

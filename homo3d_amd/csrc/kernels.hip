@@ -21,6 +21,38 @@ namespace {
 __device__ __forceinline__ int wrap_dec(int a, int n) { return a == 0 ? n - 1 : a - 1; }
 __device__ __forceinline__ int wrap_inc(int a, int n) { return a == n - 1 ? 0 : a + 1; }
 
+// Storage order of a level's vertices (and of the cells, which share the vertex index).  Even levels
+// are parity-blocked: the 8 coordinate-parity classes are contiguous (n/2)^3 blocks, block
+// b = (x&1) | (y&1)<<1 | (z&1)<<2, and inside a block (x>>1) + h((y>>1) + h(z>>1)).  A GS colour is
+// one block, so colour passes read their own vertices and every neighbour class contiguously.
+// The coarsest level (possibly odd) is lexicographic.  Vertex (0,0,0) has index 0 in both orders.
+struct Lay {
+    int n;        // points per axis
+    int h;        // n / 2
+    uint32_t h3;  // h^3
+    int lex;      // 1: x + n(y + n z)
+    // index contribution of (already wrapped) coordinate a on axis k
+    __device__ __forceinline__ uint32_t part(int a, int k) const {
+        if (lex) return k == 0 ? (uint32_t)a : (k == 1 ? (uint32_t)a * n : (uint32_t)a * n * n);
+        const uint32_t stride = k == 0 ? 1u : (k == 1 ? (uint32_t)h : (uint32_t)h * h);
+        return ((uint32_t)(a & 1) << k) * h3 + (uint32_t)(a >> 1) * stride;
+    }
+    __device__ __forceinline__ uint32_t idx(int x, int y, int z) const { return part(x, 0) + part(y, 1) + part(z, 2); }
+    __device__ __forceinline__ void pos(uint32_t t, int& x, int& y, int& z) const {
+        if (lex) {
+            x = (int)(t % n);
+            y = (int)((t / n) % n);
+            z = (int)(t / ((uint32_t)n * n));
+            return;
+        }
+        const uint32_t b = t / h3, l = t - b * h3;
+        x = 2 * (int)(l % h) + (int)(b & 1);
+        y = 2 * (int)((l / h) % h) + (int)((b >> 1) & 1);
+        z = 2 * (int)(l / ((uint32_t)h * h)) + (int)(b >> 2);
+    }
+    __device__ __forceinline__ int wrap(int a) const { return a < 0 ? a + n : (a >= n ? a - n : a); }
+};
+
 // Buffer-resource loads: one SGPR descriptor per array, a 32-bit VGPR byte offset per element and an
 // SGPR byte offset per (load case, component) plane.  Offsets past the array read as zero.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
@@ -77,40 +109,35 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 struct FineCtx {
-    int xs[3], ys[3], zs[3];
+    uint32_t X[3], Y[3], Z[3];  // index parts of the coordinates -1, 0, +1 per axis
     float wc[8];
     double wd[8];  // FP64 weights Emin + x (refinement residual only)
-    __device__ __forceinline__ int nb(int d, int n) const {
-        return xs[d % 3] + n * (ys[d / 3 % 3] + n * zs[d / 9]);
-    }
+    __device__ __forceinline__ uint32_t nb(int d) const { return X[d % 3] + Y[d / 3 % 3] + Z[d / 9]; }
+    // cell at offset (ex-1, ey-1, ez-1) from the vertex
+    __device__ __forceinline__ uint32_t cell(int e) const { return X[e & 1] + Y[(e >> 1) & 1] + Z[e >> 2]; }
 };
 
-__device__ __forceinline__ void fine_ctx(FineCtx& c, int x, int y, int z, int n, const float* __restrict__ w) {
-    c.xs[0] = wrap_dec(x, n); c.xs[1] = x; c.xs[2] = wrap_inc(x, n);
-    c.ys[0] = wrap_dec(y, n); c.ys[1] = y; c.ys[2] = wrap_inc(y, n);
-    c.zs[0] = wrap_dec(z, n); c.zs[1] = z; c.zs[2] = wrap_inc(z, n);
-    const Rsrc rw = make_rsrc(w, (size_t)n * n * n * sizeof(float));
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        // cell at offset (ex-1, ey-1, ez-1): index xs[ex], ys[ey], zs[ez]
-        const uint32_t q = c.xs[e & 1] + n * (c.ys[(e >> 1) & 1] + n * c.zs[e >> 2]);
-        c.wc[e] = bload<float>(rw, q * 4u, 0);
-    }
+__device__ __forceinline__ void fine_parts(FineCtx& c, int x, int y, int z, const Lay& L) {
+    c.X[0] = L.part(wrap_dec(x, L.n), 0); c.X[1] = L.part(x, 0); c.X[2] = L.part(wrap_inc(x, L.n), 0);
+    c.Y[0] = L.part(wrap_dec(y, L.n), 1); c.Y[1] = L.part(y, 1); c.Y[2] = L.part(wrap_inc(y, L.n), 1);
+    c.Z[0] = L.part(wrap_dec(z, L.n), 2); c.Z[1] = L.part(z, 2); c.Z[2] = L.part(wrap_inc(z, L.n), 2);
 }
 
-// acc[r][i] += sum_{d in mask} S_d u_{v+d}.  The pinned origin's entries of u are zero by
-// construction (every entry point zeroes them before the fine operator reads u).
+__device__ __forceinline__ void fine_ctx(FineCtx& c, int x, int y, int z, const Lay& L, const float* __restrict__ w) {
+    fine_parts(c, x, y, z, L);
+    const Rsrc rw = make_rsrc(w, (size_t)L.n * L.n * L.n * sizeof(float));
+#pragma unroll
+    for (int e = 0; e < 8; e++) c.wc[e] = bload<float>(rw, c.cell(e) * 4u, 0);
+}
+
 // FP64 cell weights w = Emin + x from the raw cell values x (FP32 input data, exact in FP64).
-__device__ __forceinline__ void fine_ctx64(FineCtx& c, int x, int y, int z, int n, const float* __restrict__ xc,
+__device__ __forceinline__ void fine_ctx64(FineCtx& c, int x, int y, int z, const Lay& L, const float* __restrict__ xc,
                                            double emin) {
-    c.xs[0] = wrap_dec(x, n); c.xs[1] = x; c.xs[2] = wrap_inc(x, n);
-    c.ys[0] = wrap_dec(y, n); c.ys[1] = y; c.ys[2] = wrap_inc(y, n);
-    c.zs[0] = wrap_dec(z, n); c.zs[1] = z; c.zs[2] = wrap_inc(z, n);
-    const Rsrc rw = make_rsrc(xc, (size_t)n * n * n * sizeof(float));
+    fine_parts(c, x, y, z, L);
+    const Rsrc rw = make_rsrc(xc, (size_t)L.n * L.n * L.n * sizeof(float));
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        const uint32_t q = c.xs[e & 1] + n * (c.ys[(e >> 1) & 1] + n * c.zs[e >> 2]);
-        c.wd[e] = emin + (double)bload<float>(rw, q * 4u, 0);
+        c.wd[e] = emin + (double)bload<float>(rw, c.cell(e) * 4u, 0);
         c.wc[e] = (float)c.wd[e];
     }
 }
@@ -138,7 +165,7 @@ __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke
     // keep each neighbour's block build next to its use (bounds register pressure)
     __builtin_amdgcn_sched_barrier(0);
     A nxt[R][3];
-    if constexpr (DN < 27) load_nb<R, A>(nxt, ru, (uint32_t)c.nb(DN, n) * (uint32_t)sizeof(A), plane, r0);
+    if constexpr (DN < 27) load_nb<R, A>(nxt, ru, c.nb(DN) * (uint32_t)sizeof(A), plane, r0);
     T S[9];
     if constexpr (sizeof(T) == 8)
         fine_block<D>(c.wd, ke, S);
@@ -167,7 +194,7 @@ __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, c
 #pragma unroll
         for (int i = 0; i < 3; i++) acc[r][i] = T(0);
     A first[R][3];
-    load_nb<R, A>(first, ru, (uint32_t)c.nb(0, n) * (uint32_t)sizeof(A), plane, r0);
+    load_nb<R, A>(first, ru, c.nb(0) * (uint32_t)sizeof(A), plane, r0);
     gather_step<R, SKIP_SELF, 0, T, A>(c, n, ke, ru, plane, r0, first, acc);
 }
 
@@ -194,15 +221,10 @@ __device__ __forceinline__ void block_update(float u[3], const float f[3], const
     }
 }
 
-// GS colour c covers vertices with coordinate parity (1-c&1, 1-c>>1&1, 1-c>>2): the reference colours
-// by the parity of the ghost-padded position (grid.cpp:155-198) and sweeps colours 0..7.
-__device__ __forceinline__ void colour_vertex(long t, int n, int colour, int& x, int& y, int& z) {
-    const int h = n >> 1;
-    const int px = 1 - (colour & 1), py = 1 - ((colour >> 1) & 1), pz = 1 - (colour >> 2);
-    x = 2 * (int)(t % h) + px;
-    y = 2 * (int)((t / h) % h) + py;
-    z = 2 * (int)(t / ((long)h * h)) + pz;
-}
+// GS colour c covers the vertices with coordinate parity (1-c&1, 1-c>>1&1, 1-c>>2): the reference
+// colours by the parity of the ghost-padded position (grid.cpp:155-198) and sweeps colours 0..7.
+// In the parity-blocked order that is block 7 - c.
+__device__ __forceinline__ uint32_t colour_block(int colour) { return (uint32_t)(7 - colour); }
 
 // ---------------------------------------------------------------------------------------------
 // Finest level (matrix-free)
@@ -211,18 +233,19 @@ __device__ __forceinline__ void colour_vertex(long t, int n, int colour, int& x,
 // MODE 0: out = K u; MODE 1: out = f - K u.  Pinned origin row is zero.
 // (update_residual_otf_kernel_1 grid.cu:1277-1461; v3_stencilOnLeft_kernel grid.cu:3686-3857)
 template <int R, int MODE>
-__global__ __launch_bounds__(256, 4) void k_fine_apply(int n, size_t N, const float* __restrict__ ke,
-                                                    const float* __restrict__ w, const float* __restrict__ u,
-                                                    const float* __restrict__ f, float* __restrict__ out) {
+__global__ __launch_bounds__(256, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
+                                                       const float* __restrict__ w, const float* __restrict__ u,
+                                                       const float* __restrict__ f, float* __restrict__ out) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= (long)N) return;
-    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    int x, y, z;
+    L.pos(v, x, y, z);
     FineCtx c;
-    fine_ctx(c, x, y, z, n, w);
+    fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, false, float, float>(c, n, N, ke_s, u, acc);
+    fine_gather<R, false, float, float>(c, L.n, N, ke_s, u, acc);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -235,17 +258,14 @@ __global__ __launch_bounds__(256, 4) void k_fine_apply(int n, size_t N, const fl
 
 // One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
 template <int R, bool REVERSE>
-__global__ __launch_bounds__(256, 4) void k_fine_gs(int n, size_t N, int colour, float omega,
-                                                 const float* __restrict__ ke, const float* __restrict__ w,
-                                                 float* __restrict__ u, const float* __restrict__ f) {
+__global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour, float omega,
+                                                    const float* __restrict__ ke, const float* __restrict__ w,
+                                                    float* __restrict__ u, const float* __restrict__ f) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long h = n >> 1;
-    if (t >= h * h * h) return;
-    int x, y, z;
-    colour_vertex(t, n, colour, x, y, z);
-    const long v = x + (long)n * (y + (long)n * z);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.h3) return;
+    const uint32_t v = colour_block(colour) * L.h3 + t;
     if (v == 0) {
 #pragma unroll
         for (int r = 0; r < R; r++)
@@ -253,10 +273,12 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(int n, size_t N, int colour,
             for (int i = 0; i < 3; i++) u[(size_t)(r * 3 + i) * N] = 0.f;
         return;
     }
+    int x, y, z;
+    L.pos(v, x, y, z);
     FineCtx c;
-    fine_ctx(c, x, y, z, n, w);
+    fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, true, float, float>(c, n, N, ke_s, u, acc);
+    fine_gather<R, true, float, float>(c, L.n, N, ke_s, u, acc);
     float Ks[9];
     fine_block<13, float>(c.wc, ke_s, Ks);
 #pragma unroll
@@ -275,13 +297,14 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(int n, size_t N, int colour,
 
 // Macro-strain right-hand sides f_s = sum_e w_e KE chi0_s for the 6 strains (grid.cu:1818-1919).
 // kchi: 6 x 24 table of KE * chi0_s.  Pinned row zero.
-__global__ __launch_bounds__(256) void k_macro_rhs(int n, size_t N, const float* __restrict__ kchi,
+__global__ __launch_bounds__(256) void k_macro_rhs(Lay L, size_t N, const float* __restrict__ kchi,
                                                    const float* __restrict__ w, float* __restrict__ f) {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= (long)N) return;
-    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    int x, y, z;
+    L.pos(v, x, y, z);
     FineCtx c;
-    fine_ctx(c, x, y, z, n, w);
+    fine_ctx(c, x, y, z, L, w);
 #pragma unroll
     for (int s = 0; s < 6; s++)
 #pragma unroll
@@ -298,24 +321,25 @@ __global__ __launch_bounds__(256) void k_macro_rhs(int n, size_t N, const float*
 // accumulated in FP64.  Writes r rounded to FP32 (the inner solve's right-hand side) and per-block
 // FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0 (r = -K u).
 template <int R, bool DO_F>
-__global__ __launch_bounds__(256, 2) void k_fine_residual64(int n, size_t N, int r0, const double* __restrict__ ke,
-                                                         const double* __restrict__ kchi, const float* __restrict__ xc,
-                                                         double emin,
-                                                         const double* __restrict__ u, float* __restrict__ r32,
-                                                         double* __restrict__ partial) {
+__global__ __launch_bounds__(256, 2) void k_fine_residual64(Lay L, size_t N, int r0, const double* __restrict__ ke,
+                                                            const double* __restrict__ kchi,
+                                                            const float* __restrict__ xc, double emin,
+                                                            const double* __restrict__ u, float* __restrict__ r32,
+                                                            double* __restrict__ partial) {
     __shared__ double sh[4];
     __shared__ double ke_s[576];
     stage_ke(ke_s, ke);
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     double nrm[R];
 #pragma unroll
     for (int r = 0; r < R; r++) nrm[r] = 0.0;
-    if (v < (long)N) {
-        const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    if (v < N) {
+        int x, y, z;
+        L.pos(v, x, y, z);
         FineCtx c;
-        fine_ctx64(c, x, y, z, n, xc, emin);
+        fine_ctx64(c, x, y, z, L, xc, emin);
         double acc[R][3];
-        fine_gather<R, false, double, double>(c, n, N, ke_s, u, acc, r0);
+        fine_gather<R, false, double, double>(c, L.n, N, ke_s, u, acc, r0);
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
@@ -338,16 +362,17 @@ __global__ __launch_bounds__(256, 2) void k_fine_residual64(int n, size_t N, int
 }
 
 // |f|^2 per load case of the FP64 macro-strain loads (pinned row excluded).
-__global__ __launch_bounds__(256) void k_rhs_norm64(int n, size_t N, const double* __restrict__ kchi,
+__global__ __launch_bounds__(256) void k_rhs_norm64(Lay L, size_t N, const double* __restrict__ kchi,
                                                     const float* __restrict__ xc, double emin,
                                                     double* __restrict__ partial) {
     __shared__ double sh[4];
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     double nrm[6] = {0, 0, 0, 0, 0, 0};
-    if (v < (long)N && v != 0) {
-        const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    if (v < N && v != 0) {
+        int x, y, z;
+        L.pos(v, x, y, z);
         FineCtx c;
-        fine_ctx64(c, x, y, z, n, xc, emin);
+        fine_ctx64(c, x, y, z, L, xc, emin);
 #pragma unroll
         for (int s = 0; s < 6; s++)
 #pragma unroll
@@ -392,16 +417,16 @@ __global__ void k_zero_pin(size_t N, T* __restrict__ u) {
 // ---------------------------------------------------------------------------------------------
 
 template <int R, bool SKIP_SELF>
-__device__ __forceinline__ void stencil_gather(int x, int y, int z, int n, size_t N, long v,
+__device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L, size_t N, uint32_t v,
                                                const float* __restrict__ st, const float* __restrict__ u,
                                                float acc[R][3]) {
-    const int xs[3] = {wrap_dec(x, n), x, wrap_inc(x, n)};
-    const int ys[3] = {wrap_dec(y, n), y, wrap_inc(y, n)};
-    const int zs[3] = {wrap_dec(z, n), z, wrap_inc(z, n)};
+    const uint32_t X[3] = {L.part(wrap_dec(x, L.n), 0), L.part(x, 0), L.part(wrap_inc(x, L.n), 0)};
+    const uint32_t Y[3] = {L.part(wrap_dec(y, L.n), 1), L.part(y, 1), L.part(wrap_inc(y, L.n), 1)};
+    const uint32_t Z[3] = {L.part(wrap_dec(z, L.n), 2), L.part(z, 2), L.part(wrap_inc(z, L.n), 2)};
     const Rsrc rs = make_rsrc(st, (size_t)243 * N * sizeof(float));
     const Rsrc ru = make_rsrc(u, (size_t)18 * N * sizeof(float));
     const uint32_t plane = (uint32_t)(N * sizeof(float));
-    const uint32_t vo = (uint32_t)v * 4u;
+    const uint32_t vo = v * 4u;
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -412,7 +437,7 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, int n, size_
         float S[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) S[k] = bload<float>(rs, vo, (uint32_t)(d * 9 + k) * plane);
-        const uint32_t qo = (uint32_t)(xs[d % 3] + n * (ys[d / 3 % 3] + n * zs[d / 9])) * 4u;
+        const uint32_t qo = (X[d % 3] + Y[d / 3 % 3] + Z[d / 9]) * 4u;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const float u0 = bload<float>(ru, qo, (uint32_t)(r * 3 + 0) * plane);
@@ -427,14 +452,15 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, int n, size_
 
 // MODE 0: out = A u; MODE 1: out = f - A u (update_residual_kernel_1, grid.cu:1465-1564)
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_stencil_apply(int n, size_t N, const float* __restrict__ st,
+__global__ __launch_bounds__(256) void k_stencil_apply(Lay L, size_t N, const float* __restrict__ st,
                                                        const float* __restrict__ u, const float* __restrict__ f,
                                                        float* __restrict__ out) {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= (long)N) return;
-    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long)n * n));
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    int x, y, z;
+    L.pos(v, x, y, z);
     float acc[R][3];
-    stencil_gather<R, false>(x, y, z, n, N, v, st, u, acc);
+    stencil_gather<R, false>(x, y, z, L, N, v, st, u, acc);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -446,17 +472,16 @@ __global__ __launch_bounds__(256) void k_stencil_apply(int n, size_t N, const fl
 
 // One colour of the coarse block GS sweep (gs_relaxation_kernel, grid.cu:556-689); no pin on coarse levels.
 template <int R, bool REVERSE>
-__global__ __launch_bounds__(256) void k_stencil_gs(int n, size_t N, int colour, float omega,
+__global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour, float omega,
                                                     const float* __restrict__ st, float* __restrict__ u,
                                                     const float* __restrict__ f) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long h = n >> 1;
-    if (t >= h * h * h) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.h3) return;
+    const uint32_t v = colour_block(colour) * L.h3 + t;
     int x, y, z;
-    colour_vertex(t, n, colour, x, y, z);
-    const long v = x + (long)n * (y + (long)n * z);
+    L.pos(v, x, y, z);
     float acc[R][3];
-    stencil_gather<R, true>(x, y, z, n, N, v, st, u, acc);
+    stencil_gather<R, true>(x, y, z, L, N, v, st, u, acc);
     float Ks[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) Ks[k] = st[(size_t)(13 * 9 + k) * N + v];
@@ -479,16 +504,16 @@ __global__ __launch_bounds__(256) void k_stencil_gs(int n, size_t N, int colour,
 // grid.cu:1120-1190).  Weights prod_k (2-|o_k|)/2; R = P^T.
 // ---------------------------------------------------------------------------------------------
 template <int R>
-__global__ __launch_bounds__(256) void k_restrict(int nf, size_t Nf, const float* __restrict__ rf,
+__global__ __launch_bounds__(256) void k_restrict(Lay Lf, Lay Lc, const float* __restrict__ rf,
                                                   float* __restrict__ fc) {
-    const int nc = nf >> 1;
-    const size_t Nc = (size_t)nc * nc * nc;
-    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (V >= (long)Nc) return;
-    const int X = (int)(V % nc), Y = (int)((V / nc) % nc), Z = (int)(V / ((long)nc * nc));
-    const int xs[3] = {wrap_dec(2 * X, nf), 2 * X, 2 * X + 1};
-    const int ys[3] = {wrap_dec(2 * Y, nf), 2 * Y, 2 * Y + 1};
-    const int zs[3] = {wrap_dec(2 * Z, nf), 2 * Z, 2 * Z + 1};
+    const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n, Nf = (size_t)Lf.n * Lf.n * Lf.n;
+    const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= Nc) return;
+    int X, Y, Z;
+    Lc.pos(V, X, Y, Z);
+    const uint32_t xs[3] = {Lf.part(wrap_dec(2 * X, Lf.n), 0), Lf.part(2 * X, 0), Lf.part(2 * X + 1, 0)};
+    const uint32_t ys[3] = {Lf.part(wrap_dec(2 * Y, Lf.n), 1), Lf.part(2 * Y, 1), Lf.part(2 * Y + 1, 1)};
+    const uint32_t zs[3] = {Lf.part(wrap_dec(2 * Z, Lf.n), 2), Lf.part(2 * Z, 2), Lf.part(2 * Z + 1, 2)};
     float acc[R][3];
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -498,7 +523,7 @@ __global__ __launch_bounds__(256) void k_restrict(int nf, size_t Nf, const float
     for (int o = 0; o < 27; o++) {
         const int ox = o % 3, oy = o / 3 % 3, oz = o / 9;
         const float wgt = (float)((ox == 1 ? 2 : 1) * (oy == 1 ? 2 : 1) * (oz == 1 ? 2 : 1)) * 0.125f;
-        const int q = xs[ox] + nf * (ys[oy] + nf * zs[oz]);
+        const uint32_t q = xs[ox] + ys[oy] + zs[oz];
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
@@ -511,14 +536,15 @@ __global__ __launch_bounds__(256) void k_restrict(int nf, size_t Nf, const float
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_prolong(int nf, size_t Nf, const float* __restrict__ uc,
+__global__ __launch_bounds__(256) void k_prolong(Lay Lf, Lay Lc, const float* __restrict__ uc,
                                                  float* __restrict__ uf, int pin) {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= (long)Nf) return;
+    const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n, Nf = (size_t)Lf.n * Lf.n * Lf.n;
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Nf) return;
     if (pin && v == 0) return;
-    const int nc = nf >> 1;
-    const size_t Nc = (size_t)nc * nc * nc;
-    const int x = (int)(v % nf), y = (int)((v / nf) % nf), z = (int)(v / ((long)nf * nf));
+    const int nc = Lc.n;
+    int x, y, z;
+    Lf.pos(v, x, y, z);
     // per axis: even -> one parent weight 1, odd -> two parents weight 1/2
     int ix[2], iy[2], iz[2];
     float wx[2], wy[2], wz[2];
@@ -537,7 +563,7 @@ __global__ __launch_bounds__(256) void k_prolong(int nf, size_t Nf, const float*
     for (int p = 0; p < 8; p++) {
         const float wgt = wx[p & 1] * wy[(p >> 1) & 1] * wz[p >> 2];
         if (wgt == 0.f) continue;
-        const int q = ix[p & 1] + nc * (iy[(p >> 1) & 1] + nc * iz[p >> 2]);
+        const uint32_t q = Lc.idx(ix[p & 1], iy[(p >> 1) & 1], iz[p >> 2]);
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
@@ -558,14 +584,15 @@ __device__ __forceinline__ float hat(int o) { return o == 0 ? 1.f : ((o == 1 || 
 
 // Level-1 stencil away from the pinned origin: st[V][D] = sum_{c} w_{2V+c} G[c][D], c in {-2..1}^3.
 // G (64 x 27 x 9) is the density-independent table of P-weighted KE blocks.
-__global__ __launch_bounds__(256) void k_galerkin_fine(int nf, const float* __restrict__ G,
+__global__ __launch_bounds__(256) void k_galerkin_fine(Lay Lf, Lay Lc, const float* __restrict__ G,
                                                        const float* __restrict__ w, float* __restrict__ stc,
                                                        float clamp) {
-    const int nc = nf >> 1;
-    const size_t Nc = (size_t)nc * nc * nc;
-    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (V >= (long)Nc) return;
-    const int X = (int)(V % nc), Y = (int)((V / nc) % nc), Z = (int)(V / ((long)nc * nc));
+    const int nf = Lf.n;
+    const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n;
+    const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= Nc) return;
+    int X, Y, Z;
+    Lc.pos(V, X, Y, Z);
     float wc[64];
 #pragma unroll
     for (int c = 0; c < 64; c++) {
@@ -573,7 +600,7 @@ __global__ __launch_bounds__(256) void k_galerkin_fine(int nf, const float* __re
         cx = cx < 0 ? cx + nf : (cx >= nf ? cx - nf : cx);
         cy = cy < 0 ? cy + nf : (cy >= nf ? cy - nf : cy);
         cz = cz < 0 ? cz + nf : (cz >= nf ? cz - nf : cz);
-        wc[c] = w[cx + nf * (cy + nf * cz)];
+        wc[c] = w[Lf.idx(cx, cy, cz)];
     }
     for (int D = 0; D < 27; D++) {
         float S[9];
@@ -597,9 +624,9 @@ __global__ __launch_bounds__(256) void k_galerkin_fine(int nf, const float* __re
 
 // Level-1 stencil of the 27 coarse vertices around the pinned origin: node pairs touching the
 // origin contribute identity when they are the same node and nothing otherwise.
-__global__ void k_galerkin_fine_pin(int nf, const float* __restrict__ ke, const float* __restrict__ w,
+__global__ void k_galerkin_fine_pin(Lay Lf, Lay Lc, const float* __restrict__ ke, const float* __restrict__ w,
                                     float* __restrict__ stc, float clamp) {
-    const int nc = nf >> 1;
+    const int nf = Lf.n, nc = Lc.n;
     const size_t Nc = (size_t)nc * nc * nc;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (t%3-1, t/3%3-1, t/9-1) around the origin
     if (t >= 27 * 27) return;
@@ -611,7 +638,7 @@ __global__ void k_galerkin_fine_pin(int nf, const float* __restrict__ ke, const 
         const int ox = (c & 3) - 2, oy = ((c >> 2) & 3) - 2, oz = (c >> 4) - 2;  // cell offset from 2V
         int cx = 2 * X + ox, cy = 2 * Y + oy, cz = 2 * Z + oz;
         const int cxw = (cx % nf + nf) % nf, cyw = (cy % nf + nf) % nf, czw = (cz % nf + nf) % nf;
-        const float we = w[cxw + nf * (cyw + nf * czw)];
+        const float we = w[Lf.idx(cxw, cyw, czw)];
         for (int a = 0; a < 8; a++) {
             const int ax = ox + (a & 1), ay = oy + ((a >> 1) & 1), az = oz + (a >> 2);
             const float wa = hat(ax) * hat(ay) * hat(az);
@@ -642,22 +669,22 @@ __global__ void k_galerkin_fine_pin(int nf, const float* __restrict__ ke, const 
         if (fabsf(S[4]) < clamp) S[4] = clamp;
         if (fabsf(S[8]) < clamp) S[8] = clamp;
     }
-    const int Xw = (X + nc) % nc, Yw = (Y + nc) % nc, Zw = (Z + nc) % nc;
-    const size_t V = Xw + (size_t)nc * (Yw + (size_t)nc * Zw);
+    const uint32_t V = Lc.idx((X + nc) % nc, (Y + nc) % nc, (Z + nc) % nc);
     for (int k = 0; k < 9; k++) stc[(size_t)(D * 9 + k) * Nc + V] = S[k];
 }
 
 // Coarser stencil from a stored one: st_c[V][D] = sum_{o,d} hat(o) A_f[2V+o][d] hat3(o+d-2D).
-__global__ __launch_bounds__(256) void k_galerkin_coarse(int nf, const float* __restrict__ stf,
+__global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const float* __restrict__ stf,
                                                          float* __restrict__ stc, float clamp) {
-    const int nc = nf >> 1;
-    const size_t Nc = (size_t)nc * nc * nc, Nf = (size_t)nf * nf * nf;
-    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (V >= (long)Nc) return;
-    const int X = (int)(V % nc), Y = (int)((V / nc) % nc), Z = (int)(V / ((long)nc * nc));
-    const int xs[3] = {wrap_dec(2 * X, nf), 2 * X, 2 * X + 1};
-    const int ys[3] = {wrap_dec(2 * Y, nf), 2 * Y, 2 * Y + 1};
-    const int zs[3] = {wrap_dec(2 * Z, nf), 2 * Z, 2 * Z + 1};
+    const int nf = Lf.n;
+    const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n, Nf = (size_t)nf * nf * nf;
+    const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= Nc) return;
+    int X, Y, Z;
+    Lc.pos(V, X, Y, Z);
+    const uint32_t xs[3] = {Lf.part(wrap_dec(2 * X, nf), 0), Lf.part(2 * X, 0), Lf.part(2 * X + 1, 0)};
+    const uint32_t ys[3] = {Lf.part(wrap_dec(2 * Y, nf), 1), Lf.part(2 * Y, 1), Lf.part(2 * Y + 1, 1)};
+    const uint32_t zs[3] = {Lf.part(wrap_dec(2 * Z, nf), 2), Lf.part(2 * Z, 2), Lf.part(2 * Z + 1, 2)};
     for (int D = 0; D < 27; D++) {
         const int Dx = D % 3 - 1, Dy = D / 3 % 3 - 1, Dz = D / 9 - 1;
         float S[9];
@@ -666,7 +693,7 @@ __global__ __launch_bounds__(256) void k_galerkin_coarse(int nf, const float* __
         for (int o = 0; o < 27; o++) {
             const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
             const float wo = hat(ox) * hat(oy) * hat(oz);
-            const size_t vf = xs[ox + 1] + (size_t)nf * (ys[oy + 1] + (size_t)nf * zs[oz + 1]);
+            const size_t vf = xs[ox + 1] + ys[oy + 1] + zs[oz + 1];
             for (int d = 0; d < 27; d++) {
                 const int dx = d % 3 - 1, dy = d / 3 % 3 - 1, dz = d / 9 - 1;
                 const float wd = hat(ox + dx - 2 * Dx) * hat(oy + dy - 2 * Dy) * hat(oz + dz - 2 * Dz);
@@ -799,14 +826,16 @@ __device__ __forceinline__ float macro_disp(int node, int s, int c) {
     }
 }
 
-__device__ __forceinline__ void cell_energies(int n, size_t N, long e, const float* __restrict__ ke,
-                                              const float* __restrict__ chi, float E[21]) {
-    const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
-    const int xs[2] = {x, wrap_inc(x, n)}, ys[2] = {y, wrap_inc(y, n)}, zs[2] = {z, wrap_inc(z, n)};
+__device__ __forceinline__ void cell_energies(const Lay& L, size_t N, uint32_t e, const float* __restrict__ ke,
+                                              const float* __restrict__ chi, float E[21], int& x, int& y, int& z) {
+    L.pos(e, x, y, z);
+    const uint32_t xs[2] = {L.part(x, 0), L.part(wrap_inc(x, L.n), 0)};
+    const uint32_t ys[2] = {L.part(y, 1), L.part(wrap_inc(y, L.n), 1)};
+    const uint32_t zs[2] = {L.part(z, 2), L.part(wrap_inc(z, L.n), 2)};
     float U[6][24];
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-        const int q = xs[b & 1] + n * (ys[(b >> 1) & 1] + n * zs[b >> 2]);
+        const uint32_t q = xs[b & 1] + ys[(b >> 1) & 1] + zs[b >> 2];
 #pragma unroll
         for (int s = 0; s < 6; s++)
 #pragma unroll
@@ -837,16 +866,17 @@ __device__ __forceinline__ void cell_energies(int n, size_t N, long e, const flo
 }
 
 // partial[k * nblocks + block] = sum of w_e E_k,e over the block's cells (FP64)
-__global__ __launch_bounds__(256) void k_elastic_matrix(int n, size_t N, const float* __restrict__ ke,
+__global__ __launch_bounds__(256) void k_elastic_matrix(Lay L, size_t N, const float* __restrict__ ke,
                                                         const float* __restrict__ w, const float* __restrict__ chi,
                                                         double* __restrict__ partial) {
     __shared__ double sh[4];
     double acc[21];
 #pragma unroll
     for (int k = 0; k < 21; k++) acc[k] = 0.0;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (size_t)gridDim.x * blockDim.x) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < N; e += gridDim.x * blockDim.x) {
         float E[21];
-        cell_energies(n, N, (long)e, ke, chi, E);
+        int x, y, z;
+        cell_energies(L, N, e, ke, chi, E, x, y, z);
         const double we = w[e];
 #pragma unroll
         for (int k = 0; k < 21; k++) acc[k] += we * (double)E[k];
@@ -858,18 +888,18 @@ __global__ __launch_bounds__(256) void k_elastic_matrix(int n, size_t N, const f
 }
 
 // sens[x + (y + z*n)*pitch] = (1/N) sum_{i<=j} E_ij (dC_ij + [i!=j] dC_ji)
-__global__ __launch_bounds__(256) void k_sensitivity(int n, size_t N, int pitch, const float* __restrict__ ke,
+__global__ __launch_bounds__(256) void k_sensitivity(Lay L, size_t N, int pitch, const float* __restrict__ ke,
                                                      const float* __restrict__ chi, const float* __restrict__ wdc,
                                                      float* __restrict__ sens) {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long)N) return;
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N) return;
     float E[21];
-    cell_energies(n, N, e, ke, chi, E);
+    int x, y, z;
+    cell_energies(L, N, e, ke, chi, E, x, y, z);
     float s = 0.f;
 #pragma unroll
     for (int k = 0; k < 21; k++) s = fmaf(E[k], wdc[k], s);
-    const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
-    sens[x + ((size_t)y + (size_t)z * n) * pitch] = s / (float)N;
+    sens[x + ((size_t)y + (size_t)z * L.n) * pitch] = s / (float)N;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -985,13 +1015,15 @@ __global__ __launch_bounds__(256) void k_absmax(long N, int n, int pitch, const 
 }
 
 // pitched lexicographic density -> dense cell weights w = emin + x
-__global__ __launch_bounds__(256) void k_set_weights(int n, int pitch, double emin, const float* __restrict__ x,
+// pitched lexicographic density -> cell weights w = emin + x and raw x in level storage order
+__global__ __launch_bounds__(256) void k_set_weights(Lay L, int pitch, double emin, const float* __restrict__ x,
                                                      float* __restrict__ w, float* __restrict__ xc) {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long N = (long)n * n * n;
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = (uint32_t)L.n * L.n * L.n;
     if (e >= N) return;
-    const int xx = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
-    const float xv = x[xx + ((size_t)y + (size_t)z * n) * pitch];
+    int xx, y, z;
+    L.pos(e, xx, y, z);
+    const float xv = x[xx + ((size_t)y + (size_t)z * L.n) * pitch];
     xc[e] = xv;
     w[e] = (float)(emin + (double)xv);
 }
@@ -1012,41 +1044,52 @@ inline unsigned nblk(size_t work, int bs = 256) { return (unsigned)((work + bs -
 // ---------------------------------------------------------------------------------------------
 // Launch wrappers
 // ---------------------------------------------------------------------------------------------
+static Lay make_lay(int n, int lex) {
+    Lay L;
+    L.n = n;
+    L.h = n / 2;
+    L.h3 = (uint32_t)L.h * L.h * L.h;
+    L.lex = lex;
+    return L;
+}
+
 void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const float* u, const float* f,
                        float* out) {
     const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
     if (f)
-        hipLaunchKernelGGL((k_fine_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, n, N, ke, w, u, f, out);
+        hipLaunchKernelGGL((k_fine_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, L, N, ke, w, u, f, out);
     else
-        hipLaunchKernelGGL((k_fine_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, n, N, ke, w, u, f, out);
+        hipLaunchKernelGGL((k_fine_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, L, N, ke, w, u, f, out);
 }
 
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* ke, const float* w,
                     float* u, const float* f) {
     const size_t N = (size_t)n * n * n;
-    const size_t h3 = N / 8;
+    const Lay L = make_lay(n, 0);
     if (reverse)
-        hipLaunchKernelGGL((k_fine_gs<6, true>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, ke, w, u, f);
+        hipLaunchKernelGGL((k_fine_gs<6, true>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, ke, w, u, f);
     else
-        hipLaunchKernelGGL((k_fine_gs<6, false>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, ke, w, u, f);
+        hipLaunchKernelGGL((k_fine_gs<6, false>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, ke, w, u, f);
 }
 
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_macro_rhs, dim3(nblk(N)), dim3(256), 0, s, n, N, kchi, w, f);
+    hipLaunchKernelGGL(k_macro_rhs, dim3(nblk(N)), dim3(256), 0, s, make_lay(n, 0), N, kchi, w, f);
 }
 
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
                             const double* u, float* r32, double* partial, double* out6, int do_f) {
     const size_t N = (size_t)n * n * n;
     const unsigned nb = nblk(N);
+    const Lay L = make_lay(n, 0);
     for (int r0 = 0; r0 < 6; r0 += 1) {
         if (do_f)
-            hipLaunchKernelGGL((k_fine_residual64<1, true>), dim3(nb), dim3(256), 0, s, n, N, r0, ke, kchi, xc, emin, u,
+            hipLaunchKernelGGL((k_fine_residual64<1, true>), dim3(nb), dim3(256), 0, s, L, N, r0, ke, kchi, xc, emin, u,
                                r32, partial);
         else
-            hipLaunchKernelGGL((k_fine_residual64<1, false>), dim3(nb), dim3(256), 0, s, n, N, r0, ke, kchi, xc, emin, u,
-                               r32, partial);
+            hipLaunchKernelGGL((k_fine_residual64<1, false>), dim3(nb), dim3(256), 0, s, L, N, r0, ke, kchi, xc, emin,
+                               u, r32, partial);
     }
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)nb, partial, out6);
 }
@@ -1055,7 +1098,7 @@ void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc
                        double* out6) {
     const size_t N = (size_t)n * n * n;
     const unsigned nb = nblk(N);
-    hipLaunchKernelGGL(k_rhs_norm64, dim3(nb), dim3(256), 0, s, n, N, kchi, xc, emin, partial);
+    hipLaunchKernelGGL(k_rhs_norm64, dim3(nb), dim3(256), 0, s, make_lay(n, 0), N, kchi, xc, emin, partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)nb, partial, out6);
 }
 
@@ -1079,44 +1122,50 @@ void launch_zero_pin64(hipStream_t s, size_t N, double* u) {
     hipLaunchKernelGGL((k_zero_pin<double>), dim3(1), dim3(64), 0, s, N, u);
 }
 
-void launch_stencil_apply(hipStream_t s, int n, const float* st, const float* u, const float* f, float* out) {
+void launch_stencil_apply(hipStream_t s, int n, int lex, const float* st, const float* u, const float* f,
+                          float* out) {
     const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, lex);
     if (f)
-        hipLaunchKernelGGL((k_stencil_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, n, N, st, u, f, out);
+        hipLaunchKernelGGL((k_stencil_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, L, N, st, u, f, out);
     else
-        hipLaunchKernelGGL((k_stencil_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, n, N, st, u, f, out);
+        hipLaunchKernelGGL((k_stencil_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, L, N, st, u, f, out);
 }
 
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* st, float* u,
                        const float* f) {
     const size_t N = (size_t)n * n * n;
-    const size_t h3 = N / 8;
+    const Lay L = make_lay(n, 0);
     if (reverse)
-        hipLaunchKernelGGL((k_stencil_gs<6, true>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, st, u, f);
+        hipLaunchKernelGGL((k_stencil_gs<6, true>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, st, u, f);
     else
-        hipLaunchKernelGGL((k_stencil_gs<6, false>), dim3(nblk(h3)), dim3(256), 0, s, n, N, colour, omega, st, u, f);
+        hipLaunchKernelGGL((k_stencil_gs<6, false>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, st, u, f);
 }
 
-void launch_restrict(hipStream_t s, int nf, const float* rf, float* fc) {
+void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc) {
+    const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
+    hipLaunchKernelGGL((k_restrict<6>), dim3(nblk(Nc)), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc), rf,
+                       fc);
+}
+
+void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, float* uf, int pin) {
     const size_t Nf = (size_t)nf * nf * nf;
-    hipLaunchKernelGGL((k_restrict<6>), dim3(nblk(Nf / 8)), dim3(256), 0, s, nf, Nf, rf, fc);
+    hipLaunchKernelGGL((k_prolong<6>), dim3(nblk(Nf)), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc), uc, uf,
+                       pin);
 }
 
-void launch_prolong(hipStream_t s, int nf, const float* uc, float* uf, int pin) {
-    const size_t Nf = (size_t)nf * nf * nf;
-    hipLaunchKernelGGL((k_prolong<6>), dim3(nblk(Nf)), dim3(256), 0, s, nf, Nf, uc, uf, pin);
-}
-
-void launch_galerkin_fine(hipStream_t s, int nf, const float* G, const float* ke, const float* w, float* stc,
+void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
                           float clamp) {
     const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
-    hipLaunchKernelGGL(k_galerkin_fine, dim3(nblk(Nc)), dim3(256), 0, s, nf, G, w, stc, clamp);
-    hipLaunchKernelGGL(k_galerkin_fine_pin, dim3(nblk(27 * 27)), dim3(256), 0, s, nf, ke, w, stc, clamp);
+    const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
+    hipLaunchKernelGGL(k_galerkin_fine, dim3(nblk(Nc)), dim3(256), 0, s, Lf, Lc, G, w, stc, clamp);
+    hipLaunchKernelGGL(k_galerkin_fine_pin, dim3(nblk(27 * 27)), dim3(256), 0, s, Lf, Lc, ke, w, stc, clamp);
 }
 
-void launch_galerkin_coarse(hipStream_t s, int nf, const float* stf, float* stc, float clamp) {
+void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const float* stf, float* stc, float clamp) {
     const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
-    hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(Nc)), dim3(256), 0, s, nf, stf, stc, clamp);
+    hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(Nc)), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc), stf,
+                       stc, clamp);
 }
 
 void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float* f, float* u) {
@@ -1150,14 +1199,14 @@ void launch_pcg_update_p(hipStream_t s, size_t M, const double* sc, const float*
 void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,
                            int nblocks, double* out21) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_elastic_matrix, dim3(nblocks), dim3(256), 0, s, n, N, ke, w, chi, partial);
+    hipLaunchKernelGGL(k_elastic_matrix, dim3(nblocks), dim3(256), 0, s, make_lay(n, 0), N, ke, w, chi, partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 21, nblocks, partial, out21);
 }
 
 void launch_sensitivity(hipStream_t s, int n, int pitch, const float* ke, const float* chi, const float* wdc,
                         float* sens) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_sensitivity, dim3(nblk(N)), dim3(256), 0, s, n, N, pitch, ke, chi, wdc, sens);
+    hipLaunchKernelGGL(k_sensitivity, dim3(nblk(N)), dim3(256), 0, s, make_lay(n, 0), N, pitch, ke, chi, wdc, sens);
 }
 
 void launch_symmetrize(hipStream_t s, int n, int pitch, int ng, const int* gperm, const int* gflip, const float* in,
@@ -1186,12 +1235,317 @@ void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* parti
 
 void launch_set_weights(hipStream_t s, int n, int pitch, double emin, const float* x, float* w, float* xc) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_set_weights, dim3(nblk(N)), dim3(256), 0, s, n, pitch, emin, x, w, xc);
+    hipLaunchKernelGGL(k_set_weights, dim3(nblk(N)), dim3(256), 0, s, make_lay(n, 0), pitch, emin, x, w, xc);
 }
 
 void launch_copy_pitched(hipStream_t s, int n, int pitch_in, int pitch_out, const float* in, float* out) {
     const size_t N = (size_t)n * n * n;
     hipLaunchKernelGGL(k_copy_pitched, dim3(nblk(N)), dim3(256), 0, s, n, pitch_in, pitch_out, in, out);
+}
+
+}  // namespace homo3d
+
+// =============================================================================================
+// MMA (Svanberg's method of moving asymptotes; mmasub_ker + subsolv_g, mmaOpt.cu:429-1019)
+// FP64 per-variable kernels; m <= kMmaMaxM constraints; dfdx is m x n row-major.
+// =============================================================================================
+namespace homo3d {
+namespace {
+
+struct MmaDual {          // small dual vectors of subsolv, read by every thread
+    double lam[kMmaMaxM];
+    double epsi;
+};
+
+template <int BS>
+__device__ __forceinline__ double block_max(double v, double* sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = -1e300;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < BS / 64; i++) s = fmax(s, sh[i]);
+    __syncthreads();
+    return s;
+}
+
+// asymptotes, move limits and the convex approximation terms (mmasub_ker, mmaOpt.cu:864-989);
+// part[i*nb + b] = block sums of P_ij/(upp-x) + Q_ij/(x-low)
+__global__ __launch_bounds__(256) void k_mma_setup(int m, long n, int iter, double move, const double* __restrict__ x,
+                                                   const double* __restrict__ xmin, const double* __restrict__ xmax,
+                                                   const double* __restrict__ xold1, const double* __restrict__ xold2,
+                                                   const double* __restrict__ df0, const double* __restrict__ dfdx,
+                                                   double* __restrict__ low, double* __restrict__ upp,
+                                                   double* __restrict__ alfa, double* __restrict__ beta,
+                                                   double* __restrict__ p0, double* __restrict__ q0,
+                                                   double* __restrict__ P, double* __restrict__ Q,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[4];
+    const double raa0 = 1e-5, albefa = 0.1, asyinit = 0.5, asyincr = 1.2, asydecr = 0.7;
+    double bs[kMmaMaxM] = {0, 0, 0, 0};
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double xj = x[j], dx = xmax[j] - xmin[j];
+        double lo, up;
+        if (iter <= 2) {
+            lo = xj - asyinit * dx;
+            up = xj + asyinit * dx;
+        } else {
+            const double zzz = (xj - xold1[j]) * (xold1[j] - xold2[j]);
+            const double fac = zzz > 0 ? asyincr : (zzz < 0 ? asydecr : 1.0);
+            lo = xj - fac * (xold1[j] - low[j]);
+            up = xj + fac * (upp[j] - xold1[j]);
+            lo = fmax(lo, xj - 10 * dx);
+            lo = fmin(lo, xj - 0.01 * dx);
+            up = fmin(up, xj + 10 * dx);
+            up = fmax(up, xj + 0.01 * dx);
+        }
+        low[j] = lo;
+        upp[j] = up;
+        alfa[j] = fmax(fmax(lo + albefa * (xj - lo), xj - move * dx), xmin[j]);
+        beta[j] = fmin(fmin(up - albefa * (up - xj), xj + move * dx), xmax[j]);
+        const double xmamiinv = 1.0 / fmax(dx, 1e-5);
+        const double ux1 = up - xj, xl1 = xj - lo, ux2 = ux1 * ux1, xl2 = xl1 * xl1;
+        const double dp = fmax(df0[j], 0.0), dm = fmax(-df0[j], 0.0);
+        const double pq0 = 1e-3 * (dp + dm) + raa0 * xmamiinv;
+        p0[j] = (dp + pq0) * ux2;
+        q0[j] = (dm + pq0) * xl2;
+        for (int i = 0; i < m; i++) {
+            const double g = dfdx[(size_t)i * n + j];
+            const double pp = fmax(g, 0.0), qq = fmax(-g, 0.0), pq = 1e-3 * (pp + qq) + raa0 * xmamiinv;
+            const double Pv = (pp + pq) * ux2, Qv = (qq + pq) * xl2;
+            P[(size_t)i * n + j] = Pv;
+            Q[(size_t)i * n + j] = Qv;
+            bs[i] += Pv / ux1 + Qv / xl1;
+        }
+    }
+    for (int i = 0; i < m; i++) {
+        const double s = block_sum<256>(bs[i], sh);
+        if (threadIdx.x == 0) part[i * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// per-variable primal residual terms of subsolv at (x, xsi, eta; lam, epsi):
+// part layout per block b: [0] sum sq, [1] max abs (in part2), [2..2+m) gvec partials
+__global__ __launch_bounds__(256) void k_mma_resid(int m, long n, const MmaDual dual, const double* __restrict__ x,
+                                                   const double* __restrict__ xsi, const double* __restrict__ eta,
+                                                   const double* __restrict__ low, const double* __restrict__ upp,
+                                                   const double* __restrict__ alfa, const double* __restrict__ beta,
+                                                   const double* __restrict__ p0, const double* __restrict__ q0,
+                                                   const double* __restrict__ P, const double* __restrict__ Q,
+                                                   double* __restrict__ part, double* __restrict__ partmax) {
+    __shared__ double sh[4];
+    double ssq = 0, smax = 0;
+    double gv[kMmaMaxM] = {0, 0, 0, 0};
+    const double epsi = dual.epsi;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double xj = x[j], ux1 = upp[j] - xj, xl1 = xj - low[j];
+        double pl = p0[j], ql = q0[j];
+        for (int i = 0; i < m; i++) {
+            const double Pv = P[(size_t)i * n + j], Qv = Q[(size_t)i * n + j];
+            pl += Pv * dual.lam[i];
+            ql += Qv * dual.lam[i];
+            gv[i] += Pv / ux1 + Qv / xl1;
+        }
+        const double dpsidx = pl / (ux1 * ux1) - ql / (xl1 * xl1);
+        const double rex = dpsidx - xsi[j] + eta[j];
+        const double rexsi = xsi[j] * (xj - alfa[j]) - epsi;
+        const double reeta = eta[j] * (beta[j] - xj) - epsi;
+        ssq += rex * rex + rexsi * rexsi + reeta * reeta;
+        smax = fmax(smax, fmax(fabs(rex), fmax(fabs(rexsi), fabs(reeta))));
+    }
+    double t = block_sum<256>(ssq, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    for (int i = 0; i < m; i++) {
+        t = block_sum<256>(gv[i], sh);
+        if (threadIdx.x == 0) part[(1 + i) * gridDim.x + blockIdx.x] = t;
+    }
+    t = block_max<256>(smax, sh);
+    if (threadIdx.x == 0) partmax[blockIdx.x] = t;
+}
+
+// Newton system terms: delx, diagx per variable; block sums of GG_i*delx/diagx and
+// GG_i*GG_k/diagx (k >= i) for the (m+1) x (m+1) system (subsolv_g, mmaOpt.cu:560-640)
+__global__ __launch_bounds__(256) void k_mma_newton(int m, long n, const MmaDual dual, const double* __restrict__ x,
+                                                    const double* __restrict__ xsi, const double* __restrict__ eta,
+                                                    const double* __restrict__ low, const double* __restrict__ upp,
+                                                    const double* __restrict__ alfa, const double* __restrict__ beta,
+                                                    const double* __restrict__ p0, const double* __restrict__ q0,
+                                                    const double* __restrict__ P, const double* __restrict__ Q,
+                                                    double* __restrict__ delx, double* __restrict__ diagx,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[4];
+    constexpr int NS = kMmaMaxM + kMmaMaxM * (kMmaMaxM + 1) / 2;
+    double acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) acc[k] = 0;
+    const double epsi = dual.epsi;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double xj = x[j], ux1 = upp[j] - xj, xl1 = xj - low[j];
+        const double ux2 = ux1 * ux1, xl2 = xl1 * xl1, ux3 = ux1 * ux2, xl3 = xl1 * xl2;
+        double pl = p0[j], ql = q0[j], gg[kMmaMaxM];
+        for (int i = 0; i < m; i++) {
+            const double Pv = P[(size_t)i * n + j], Qv = Q[(size_t)i * n + j];
+            pl += Pv * dual.lam[i];
+            ql += Qv * dual.lam[i];
+            gg[i] = Pv / ux2 - Qv / xl2;
+        }
+        const double dpsidx = pl / ux2 - ql / xl2;
+        const double dl = dpsidx - epsi / (xj - alfa[j]) + epsi / (beta[j] - xj);
+        const double dg = 2 * (pl / ux3 + ql / xl3) + xsi[j] / (xj - alfa[j]) + eta[j] / (beta[j] - xj);
+        delx[j] = dl;
+        diagx[j] = dg;
+        int k = kMmaMaxM;
+        for (int i = 0; i < m; i++) {
+            acc[i] += gg[i] * dl / dg;
+            for (int q = i; q < m; q++) acc[k + i * kMmaMaxM - i * (i - 1) / 2 + (q - i)] += gg[i] * gg[q] / dg;
+        }
+    }
+    for (int k = 0; k < NS; k++) {
+        const double t = block_sum<256>(acc[k], sh);
+        if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// Newton direction per variable and block maxima of the step-length ratios
+// (stepxx/stepalfa/stepbeta of subsolv_g, mmaOpt.cu:680-700)
+__global__ __launch_bounds__(256) void k_mma_dir(int m, long n, const MmaDual dual, const double* __restrict__ dlam,
+                                                 const double* __restrict__ x, const double* __restrict__ xsi,
+                                                 const double* __restrict__ eta, const double* __restrict__ low,
+                                                 const double* __restrict__ upp, const double* __restrict__ alfa,
+                                                 const double* __restrict__ beta, const double* __restrict__ P,
+                                                 const double* __restrict__ Q, const double* __restrict__ delx,
+                                                 const double* __restrict__ diagx, double* __restrict__ dx,
+                                                 double* __restrict__ dxsi, double* __restrict__ deta,
+                                                 double* __restrict__ partmax) {
+    __shared__ double sh[4];
+    double stm = -1e300;
+    const double epsi = dual.epsi;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double xj = x[j], ux2 = (upp[j] - xj) * (upp[j] - xj), xl2 = (xj - low[j]) * (xj - low[j]);
+        double gt = 0;
+        for (int i = 0; i < m; i++) gt += (P[(size_t)i * n + j] / ux2 - Q[(size_t)i * n + j] / xl2) * dlam[i];
+        const double d = -delx[j] / diagx[j] - gt / diagx[j];
+        const double xa = xj - alfa[j], bx = beta[j] - xj;
+        const double ds = -xsi[j] + epsi / xa - (xsi[j] * d) / xa;
+        const double de = -eta[j] + epsi / bx + (eta[j] * d) / bx;
+        dx[j] = d;
+        dxsi[j] = ds;
+        deta[j] = de;
+        stm = fmax(stm, fmax(fmax(-1.01 * ds / xsi[j], -1.01 * de / eta[j]), fmax(-1.01 * d / xa, 1.01 * d / bx)));
+    }
+    const double t = block_max<256>(stm, sh);
+    if (threadIdx.x == 0) partmax[blockIdx.x] = t;
+}
+
+// x = xold + steg dx (and xsi, eta)
+__global__ __launch_bounds__(256) void k_mma_step(long n, double steg, const double* __restrict__ xo,
+                                                  const double* __restrict__ xsio, const double* __restrict__ etao,
+                                                  const double* __restrict__ dx, const double* __restrict__ dxsi,
+                                                  const double* __restrict__ deta, double* __restrict__ x,
+                                                  double* __restrict__ xsi, double* __restrict__ eta) {
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    x[j] = xo[j] + steg * dx[j];
+    xsi[j] = xsio[j] + steg * dxsi[j];
+    eta[j] = etao[j] + steg * deta[j];
+}
+
+__global__ __launch_bounds__(256) void k_mma_init(long n, const double* __restrict__ alfa, const double* __restrict__ beta,
+                                                  double* __restrict__ x, double* __restrict__ xsi,
+                                                  double* __restrict__ eta) {
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double xv = 0.5 * (alfa[j] + beta[j]);
+    x[j] = xv;
+    xsi[j] = fmax(1.0 / (xv - alfa[j]), 1.0);
+    eta[j] = fmax(1.0 / (beta[j] - xv), 1.0);
+}
+
+// pitched FP32 design array <-> dense FP64 vector
+__global__ __launch_bounds__(256) void k_f32_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) b[j] = (double)a[j];
+}
+__global__ __launch_bounds__(256) void k_f64_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) b[j] = (float)a[j];
+}
+
+__global__ void k_max_partials(int nb, const double* __restrict__ partial, double* __restrict__ out) {
+    __shared__ double sh[4];
+    double s = -1e300;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s = fmax(s, partial[i]);
+    s = block_max<256>(s, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+}  // namespace
+
+void launch_mma_setup(hipStream_t s, int m, long n, int iter, double move, const double* x, const double* xmin,
+                      const double* xmax, const double* xold1, const double* xold2, const double* df0,
+                      const double* dfdx, double* low, double* upp, double* alfa, double* beta, double* p0, double* q0,
+                      double* P, double* Q, double* part, int nb, double* out) {
+    hipLaunchKernelGGL(k_mma_setup, dim3(nb), dim3(256), 0, s, m, n, iter, move, x, xmin, xmax, xold1, xold2, df0, dfdx,
+                       low, upp, alfa, beta, p0, q0, P, Q, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, m, nb, part, out);
+}
+
+void launch_mma_resid(hipStream_t s, int m, long n, const double* lam, double epsi, const double* x,
+                      const double* xsi, const double* eta, const double* low, const double* upp, const double* alfa,
+                      const double* beta, const double* p0, const double* q0, const double* P, const double* Q,
+                      double* part, double* partmax, int nb, double* out) {
+    MmaDual d{};
+    for (int i = 0; i < m; i++) d.lam[i] = lam[i];
+    d.epsi = epsi;
+    hipLaunchKernelGGL(k_mma_resid, dim3(nb), dim3(256), 0, s, m, n, d, x, xsi, eta, low, upp, alfa, beta, p0, q0, P,
+                       Q, part, partmax);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 1 + m, nb, part, out);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, s, nb, partmax, out + 1 + m);
+}
+
+void launch_mma_newton(hipStream_t s, int m, long n, const double* lam, double epsi, const double* x,
+                       const double* xsi, const double* eta, const double* low, const double* upp, const double* alfa,
+                       const double* beta, const double* p0, const double* q0, const double* P, const double* Q,
+                       double* delx, double* diagx, double* part, int nb, double* out) {
+    MmaDual d{};
+    for (int i = 0; i < m; i++) d.lam[i] = lam[i];
+    d.epsi = epsi;
+    const int NS = kMmaMaxM + kMmaMaxM * (kMmaMaxM + 1) / 2;
+    hipLaunchKernelGGL(k_mma_newton, dim3(nb), dim3(256), 0, s, m, n, d, x, xsi, eta, low, upp, alfa, beta, p0, q0, P,
+                       Q, delx, diagx, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, NS, nb, part, out);
+}
+
+void launch_mma_dir(hipStream_t s, int m, long n, double epsi, const double* dlam, const double* x, const double* xsi,
+                    const double* eta, const double* low, const double* upp, const double* alfa, const double* beta,
+                    const double* P, const double* Q, const double* delx, const double* diagx, double* dx,
+                    double* dxsi, double* deta, double* partmax, int nb, double* out) {
+    MmaDual d{};
+    d.epsi = epsi;
+    hipLaunchKernelGGL(k_mma_dir, dim3(nb), dim3(256), 0, s, m, n, d, dlam, x, xsi, eta, low, upp, alfa, beta, P, Q,
+                       delx, diagx, dx, dxsi, deta, partmax);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, s, nb, partmax, out);
+}
+
+void launch_mma_step(hipStream_t s, long n, double steg, const double* xo, const double* xsio, const double* etao,
+                     const double* dx, const double* dxsi, const double* deta, double* x, double* xsi, double* eta) {
+    hipLaunchKernelGGL(k_mma_step, dim3(nblk(n)), dim3(256), 0, s, n, steg, xo, xsio, etao, dx, dxsi, deta, x, xsi,
+                       eta);
+}
+
+void launch_mma_init(hipStream_t s, long n, const double* alfa, const double* beta, double* x, double* xsi,
+                     double* eta) {
+    hipLaunchKernelGGL(k_mma_init, dim3(nblk(n)), dim3(256), 0, s, n, alfa, beta, x, xsi, eta);
+}
+
+void launch_f32_to_f64(hipStream_t s, long n, const float* a, double* b) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(nblk(n)), dim3(256), 0, s, n, a, b);
+}
+
+void launch_f64_to_f32(hipStream_t s, long n, const double* a, float* b) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nblk(n)), dim3(256), 0, s, n, a, b);
 }
 
 }  // namespace homo3d

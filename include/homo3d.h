@@ -44,6 +44,9 @@
  * sensitivities cross the boundary as the reference's lexicographic row-pitched arrays:
  * element (x,y,z) at x + (y + z*n) * pitch (AutoDiff/TensorExpression.h:137-150).
  * C is a row-major double[36] in Voigt order xx, yy, zz, yz, xz, xy.
+ *
+ * Arguments named on_device take 1 for device (or managed) memory, 0 for host memory and -1 to let
+ * the library inspect the pointer (hipPointerGetAttributes).
  */
 #ifndef HOMO3D_H
 #define HOMO3D_H
@@ -63,7 +66,8 @@ enum {
     HOMO3D_NO_SUPPORT = 5
 };
 
-enum { HOMO3D_SOLVER_MG = 0, HOMO3D_SOLVER_PCG = 1 };
+/* HOMO3D_SOLVER_NONE creates a density-only handle (symmetry, filter, OC) without a solver hierarchy. */
+enum { HOMO3D_SOLVER_MG = 0, HOMO3D_SOLVER_PCG = 1, HOMO3D_SOLVER_NONE = 2 };
 enum { HOMO3D_FIELD_U = 0, HOMO3D_FIELD_F = 1, HOMO3D_FIELD_R = 2 };
 enum { HOMO3D_SYM_REFLECT3 = 0, HOMO3D_SYM_REFLECT6 = 1, HOMO3D_SYM_ROTATE3 = 2, HOMO3D_SYM_NONE = 3 };
 
@@ -123,6 +127,20 @@ int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch,
                      int on_device);
 int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch, double vol, double min_rho,
                      double step, double damp, int on_device);
+
+/* MMA (homo::MMAOptimizer, optimization/mmaOptimizer.h:12-34; mmasubDevice, mmaOpt.h:17-19).
+ * FP64 state on the GPU; m <= 4 constraints; constants of mmasub_ker (mmaOpt.cu:846-851), move = 1. */
+typedef struct homo3d_mma homo3d_mma;
+int homo3d_mma_create(int m, long n, double a0, double a, double c, double d, int device, homo3d_mma** out);
+int homo3d_mma_destroy(homo3d_mma* h);
+const char* homo3d_mma_last_error(const homo3d_mma* h);
+int homo3d_mma_set_bound(homo3d_mma* h, double xmin, double xmax);
+/* x: n floats updated in place; dfdx: n floats; gval: m floats on the host; dgdx: m arrays of n floats.
+ * on_device selects device (1) or host (0) for x, dfdx and the dgdx arrays. */
+int homo3d_mma_update(homo3d_mma* h, int itn, float* x, const float* dfdx, const float* gval, const float* const* dgdx,
+                      int on_device);
+/* Read back the FP64 iterate and asymptotes (host arrays of n doubles, any may be NULL). */
+int homo3d_mma_state(homo3d_mma* h, double* x, double* low, double* upp, int* newton_iters);
 
 /* Seeded randc density (project generator), written to host memory, n^3 floats, pitch n. */
 int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, float* out);

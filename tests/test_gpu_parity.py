@@ -246,3 +246,28 @@ def test_oc_filter_and_update(case16, orc, h3d):
     ulp = np.abs(new.view(np.int32).astype(np.int64) - refn.view(np.int32).astype(np.int64))
     assert ulp.max() <= 1
     assert abs(new.astype(np.float64).mean() - 0.3) <= 1.01e-4
+
+
+@pytest.mark.parametrize("n,m,seed", [(20, 1, 0), (4096, 2, 1)])
+def test_mma_matches_oracle(h3d, orc, n, m, seed):
+    say("mma: n=%d m=%d" % (n, m))
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.2, 0.8, n)
+    gpu = h3d.MMAOptimizer(m, n, 1.0, 0.0, 1000.0, 1.0)
+    gpu.setBound(0.001, 1.0)
+    st = {"x": x.copy(), "xmin": np.full(n, 0.001), "xmax": np.ones(n), "xold1": x.copy(), "xold2": x.copy(),
+          "low": np.zeros(n), "upp": np.zeros(n)}
+    w = rng.uniform(0.5, 2.0, (m, n))
+    for it in range(6):
+        xin = st["x"].astype(np.float32).astype(np.float64)
+        st["x"][:] = xin
+        df0 = (-(1.0 + np.sin(3 * xin))).astype(np.float32)
+        g = np.array([(w[i] * xin).sum() / n - 0.4 - 0.1 * i for i in range(m)], dtype=np.float32)
+        dg = (w / n).astype(np.float32)
+        xg = gpu.update(it, xin.astype(np.float32), df0, g, list(dg))
+        orc.mma_update(st, it, df0.astype(np.float64), g.astype(np.float64), dg.astype(np.float64), 1.0,
+                       np.zeros(m), np.full(m, 1000.0), np.ones(m), 1.0)
+        x64, lo, up, nit = gpu.state()
+        assert np.abs(x64 - st["x"]).max() <= 1e-9 * max(1.0, np.abs(st["x"]).max()), it
+        assert np.abs(lo - st["low"]).max() <= 1e-12 and np.abs(up - st["upp"]).max() <= 1e-12
+        assert np.array_equal(xg, x64.astype(np.float32))
