@@ -298,18 +298,21 @@ void prof_end(homo3d_handle* h, ProfEvent& ev) {
 // ---------------------------------------------------------------------------------------------
 // Level operations
 // ---------------------------------------------------------------------------------------------
-void op_smooth(homo3d_handle* h, int l, float* u, const float* f, float omega, int reverse) {
+// One 8-colour sweep.  zero_start: u is taken as zero on entry whatever it holds (forward sweeps
+// only); every colour pass then writes its block without reading later colours.
+void op_smooth(homo3d_handle* h, int l, float* u, const float* f, float omega, int reverse, bool zero_start = false) {
     const Level& L = h->lv[l];
-    if (l == 0) launch_zero_pin(h->stream, L.N, u);
+    if (l == 0 && !zero_start) launch_zero_pin(h->stream, L.N, u);
+    const int zs = zero_start && !reverse ? 1 : 0;
     for (int cc = 0; cc < 8; cc++) {
         const int c = reverse ? 7 - cc : cc;
         if (l == 0) {
             ProfEvent ev;
             prof_begin(h, 0, ev);
-            launch_fine_gs(h->stream, L.n, c, omega, reverse, h->d_ke, h->d_w, u, f);
+            launch_fine_gs(h->stream, L.n, c, omega, reverse, zs, h->d_ke, h->d_w, u, f);
             prof_end(h, ev);
         } else {
-            launch_stencil_gs(h->stream, L.n, c, omega, reverse, L.st, u, f);
+            launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, u, f);
         }
     }
 }
@@ -327,15 +330,15 @@ void op_residual(homo3d_handle* h, int l, const float* u, const float* f, float*
 }
 
 // One V(1,1) cycle on the level-0 pair (u0, f0) (MG::v_cycle, MG.cpp:46-70).  When rn2 is given it
-// receives, per load case, the squared norm of the level-0 residual after pre-smoothing.
-void vcycle(homo3d_handle* h, float* u0, const float* f0, int symmetric, double* rn2) {
+// receives, per load case, the squared norm of the level-0 residual after pre-smoothing.  u0_zero:
+// the cycle starts from u0 = 0 (u0's contents are ignored), as the coarse levels always do.
+void vcycle(homo3d_handle* h, float* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false) {
     const float omega = (float)h->cfg.sor_omega;
     for (int l = 0; l < h->nl - 1; l++) {
         Level& L = h->lv[l];
         float* u = l == 0 ? u0 : L.u;
         const float* f = l == 0 ? f0 : L.f;
-        if (l > 0) hipMemsetAsync(u, 0, sizeof(float) * 3 * kRhs * L.N, h->stream);
-        op_smooth(h, l, u, f, omega, 0);
+        op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero);
         op_residual(h, l, u, f, L.r);
         if (l == 0 && rn2) launch_dot6(h->stream, 3 * L.N, L.r, L.r, h->d_partial, kDotBlocks, rn2);
         launch_restrict(h->stream, L.n, h->lv[l + 1].lex, L.r, h->lv[l + 1].f);
@@ -398,8 +401,7 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     if (!any) return HOMO3D_OK;
     HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                              h->stream));
-    HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
-    vcycle(h, h->Z, h->Rr, 1, nullptr);
+    vcycle(h, h->Z, h->Rr, 1, nullptr, true);
     HIPCHK(h, hipMemcpyAsync(h->P, h->Z, vb, hipMemcpyDeviceToDevice, h->stream));
     launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc);  // rz
     double rr[kRhs];
@@ -432,8 +434,7 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         if (changed)
             HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                                      h->stream));
-        HIPCHK(h, hipMemsetAsync(h->Z, 0, vb, h->stream));
-        vcycle(h, h->Z, h->Rr, 1, nullptr);
+        vcycle(h, h->Z, h->Rr, 1, nullptr, true);
         launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
         launch_pcg_beta(h->stream, h->d_sc);
         launch_pcg_update_p(h->stream, M, h->d_sc, h->Z, h->P);
@@ -728,22 +729,25 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             if (!done[r]) status = std::max(status, (int)HOMO3D_NOT_CONVERGED);
     } else {
         // FP64 iterative refinement: r = f(w) - K U64 in FP64, inner FP32 MG-PCG on K d = r, U64 += d.
+        // A cold start keeps U64 = 0 implicit: the first residual is f itself (which also gives |f|^2)
+        // and the first correction is copied rather than added.
         const size_t M6 = kRhs * M;
-        launch_rhs_norm64(h->stream, L0.n, h->d_kchi64, h->d_x, h->cfg.emin, h->d_partial, h->d_tmp);
-        HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
-        if (!warm)
-            HIPCHK(h, hipMemsetAsync(h->U64, 0, sizeof(double) * M6, h->stream));
-        else if (h->x_dirty)
-            launch_widen_f64(h->stream, M6, h->X, h->U64);
+        bool u_zero = !warm;
+        if (warm) {
+            launch_rhs_norm64(h->stream, L0.n, h->d_kchi64, h->d_x, h->cfg.emin, h->d_partial, h->d_tmp);
+            HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
+            if (h->x_dirty) launch_widen_f64(h->stream, M6, h->X, h->U64);
+            launch_zero_pin64(h->stream, L0.N, h->U64);
+        }
         h->x_dirty = false;
-        launch_zero_pin64(h->stream, L0.N, h->U64);
         double rel[kRhs];
         for (int outer = 0;; outer++) {
             double rn2[kRhs];
             launch_fine_residual64(h->stream, L0.n, h->d_ke64, h->d_kchi64, h->d_x, h->cfg.emin, h->U64, h->Rr,
-                                   h->d_partial, h->d_tmp, 1);
+                                   h->d_partial, h->d_tmp, 1, u_zero ? 0 : 1);
             HIPCHK(h, hipMemcpyAsync(rn2, h->d_tmp, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(h, hipStreamSynchronize(h->stream));
+            if (outer == 0 && u_zero) std::copy(rn2, rn2 + kRhs, fn2);
             bool all = true;
             for (int r = 0; r < kRhs; r++) {
                 rel[r] = fn2[r] > 0 ? std::sqrt(rn2[r] / fn2[r]) : 0.0;
@@ -759,8 +763,13 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             HIPCHK(h, hipMemsetAsync(h->X, 0, sizeof(float) * M6, h->stream));
             rc = pcg_inner(h, rn2, tol_in, done, maxit, iters);
             if (rc) return rc;
-            launch_add_correction(h->stream, M6, h->X, h->U64);
+            if (u_zero)
+                launch_widen_f64(h->stream, M6, h->X, h->U64);
+            else
+                launch_add_correction(h->stream, M6, h->X, h->U64);
+            u_zero = false;
         }
+        if (u_zero) HIPCHK(h, hipMemsetAsync(h->U64, 0, sizeof(double) * M6, h->stream));
         for (int r = 0; r < kRhs; r++)
             if (!done[r]) status = std::max(status, (int)HOMO3D_NOT_CONVERGED);
         launch_round_f32(h->stream, M6, h->U64, h->X);

@@ -7,11 +7,12 @@ namespace homo3d {
 
 void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const float* u, const float* f,
                        float* out);
-void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* ke, const float* w,
-                    float* u, const float* f);
+// zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
+void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
+                    const float* w, float* u, const float* f);
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f);
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
-                            const double* u, float* r32, double* partial, double* out6, int do_f);
+                            const double* u, float* r32, double* partial, double* out6, int do_f, int do_k);
 void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
                        double* out6);
 void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u);
@@ -21,8 +22,8 @@ void launch_zero_pin(hipStream_t s, size_t N, float* u);
 void launch_zero_pin64(hipStream_t s, size_t N, double* u);
 void launch_stencil_apply(hipStream_t s, int n, int lex, const float* st, const float* u, const float* f,
                           float* out);
-void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* st, float* u,
-                       const float* f);
+void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* st,
+                       float* u, const float* f);
 void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc);
 void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, float* uf, int pin);
 void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace homo3d {
@@ -52,6 +54,28 @@ struct Lay {
     }
     __device__ __forceinline__ int wrap(int a) const { return a < 0 ? a + n : (a >= n ? a - n : a); }
 };
+
+// Workgroup id -> work chunk.  The hardware deals workgroup g to XCD g % 8; this gives each XCD a
+// contiguous run of chunks, so the neighbour rows that adjacent chunks share stay in one L2.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t g, uint32_t G) {
+    if (G & 7u) return g;
+    return (g & 7u) * (G >> 3) + (g >> 3);
+}
+
+// Full-level kernels run 512-thread workgroups.  On a parity-blocked level the 8 waves of a
+// workgroup take the same 64 local positions in the 8 parity blocks, so a workgroup covers a
+// 128 x 2 x 2 vertex tile and the neighbour reads of its waves overlap in L1/L2; on a
+// lexicographic level the 512 threads take consecutive vertices.  False past the end of the level.
+constexpr int kFullThreads = 512;
+__device__ __forceinline__ bool full_vertex(const Lay& L, uint32_t chunk, uint32_t& v) {
+    if (L.lex) {
+        v = chunk * (uint32_t)kFullThreads + threadIdx.x;
+        return v < (uint32_t)L.n * L.n * L.n;
+    }
+    const uint32_t t = chunk * 64u + (threadIdx.x & 63u);
+    v = (threadIdx.x >> 6) * L.h3 + t;
+    return t < L.h3;
+}
 
 // Buffer-resource loads: one SGPR descriptor per array, a 32-bit VGPR byte offset per element and an
 // SGPR byte offset per (load case, component) plane.  Offsets past the array read as zero.
@@ -142,11 +166,32 @@ __device__ __forceinline__ void fine_ctx64(FineCtx& c, int x, int y, int z, cons
     }
 }
 
-// Neighbour sequence of a row: d = 0..26, skipping the vertex itself (13) when SKIP_SELF.
-template <int D, bool SKIP_SELF>
-struct NextNb {
-    static constexpr int value = (SKIP_SELF && D + 1 == 13) ? D + 2 : D + 1;
-};
+// Neighbour masks: bit d (d = 0..26, offset (d%3-1, d/3%3-1, d/9-1)) selects the neighbours a row
+// gathers.  kRowAll is the whole row, kRowOffDiag the row without the vertex itself (d = 13).
+constexpr uint32_t kRowAll = (1u << 27) - 1u;
+constexpr uint32_t kRowOffDiag = kRowAll & ~(1u << 13);
+
+// Colour c of a sweep that starts from u = 0: the neighbours of colours above c still hold zero, so
+// only those of colours below c are gathered.  A neighbour at offset d has colour c ^ flip(d), where
+// flip sets bit k when the offset is odd along axis k.
+__host__ __device__ constexpr uint32_t zero_start_mask(int c) {
+    uint32_t m = 0;
+    for (int d = 0; d < 27; d++) {
+        const int f = (d % 3 != 1 ? 1 : 0) | (d / 3 % 3 != 1 ? 2 : 0) | (d / 9 != 1 ? 4 : 0);
+        if ((c ^ f) < c) m |= 1u << d;
+    }
+    return m;
+}
+
+// first selected neighbour after d (27 when none)
+constexpr int next_nb(uint32_t mask, int d) {
+    for (int k = d + 1; k < 27; k++)
+        if ((mask >> k) & 1u) return k;
+    return 27;
+}
+
+// zero-start colour of a GS pass (-1: ordinary pass) -> gathered neighbours
+constexpr uint32_t gs_mask(int zc) { return zc < 0 ? kRowOffDiag : zero_start_mask(zc); }
 
 template <int R, typename A>
 __device__ __forceinline__ void load_nb(A (&dst)[R][3], Rsrc ru, uint32_t qo, uint32_t plane, int r0) {
@@ -158,10 +203,10 @@ __device__ __forceinline__ void load_nb(A (&dst)[R][3], Rsrc ru, uint32_t qo, ui
 
 // One neighbour of the row: issue the loads of the next neighbour, build the 3x3 block of this one
 // from the cell weights and KE (in LDS), accumulate, recurse.
-template <int R, bool SKIP_SELF, int D, typename T, typename A>
+template <int R, uint32_t MASK, int D, typename T, typename A>
 __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke, Rsrc ru, uint32_t plane, int r0,
                                             const A (&cur)[R][3], T (&acc)[R][3]) {
-    constexpr int DN = NextNb<D, SKIP_SELF>::value;
+    constexpr int DN = next_nb(MASK, D);
     // keep each neighbour's block build next to its use (bounds register pressure)
     __builtin_amdgcn_sched_barrier(0);
     A nxt[R][3];
@@ -178,13 +223,13 @@ __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke
         acc[r][1] = fma(S[3], u0, fma(S[4], u1, fma(S[5], u2, acc[r][1])));
         acc[r][2] = fma(S[6], u0, fma(S[7], u1, fma(S[8], u2, acc[r][2])));
     }
-    if constexpr (DN < 27) gather_step<R, SKIP_SELF, DN, T, A>(c, n, ke, ru, plane, r0, nxt, acc);
+    if constexpr (DN < 27) gather_step<R, MASK, DN, T, A>(c, n, ke, ru, plane, r0, nxt, acc);
 }
 
-// acc[r][i] = sum_d S_d u_{v+d} over the row (without d = 13 when SKIP_SELF), load cases r0..r0+R-1.
+// acc[r][i] = sum_d S_d u_{v+d} over the neighbours d selected by MASK, load cases r0..r0+R-1.
 // ke points to the element matrix staged in LDS.  The pinned origin's entries of u are zero by
 // construction (every entry point zeroes them before the fine operator reads u).
-template <int R, bool SKIP_SELF, typename T, typename A>
+template <int R, uint32_t MASK, typename T, typename A>
 __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, const T* ke, const A* __restrict__ u,
                                             T (&acc)[R][3], int r0 = 0) {
     const Rsrc ru = make_rsrc(u, (size_t)18 * N * sizeof(A));
@@ -193,9 +238,12 @@ __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, c
     for (int r = 0; r < R; r++)
 #pragma unroll
         for (int i = 0; i < 3; i++) acc[r][i] = T(0);
-    A first[R][3];
-    load_nb<R, A>(first, ru, c.nb(0) * (uint32_t)sizeof(A), plane, r0);
-    gather_step<R, SKIP_SELF, 0, T, A>(c, n, ke, ru, plane, r0, first, acc);
+    constexpr int D0 = next_nb(MASK, -1);
+    if constexpr (D0 < 27) {
+        A first[R][3];
+        load_nb<R, A>(first, ru, c.nb(D0) * (uint32_t)sizeof(A), plane, r0);
+        gather_step<R, MASK, D0, T, A>(c, n, ke, ru, plane, r0, first, acc);
+    }
 }
 
 // stage the element matrix in LDS (all threads of the block take part)
@@ -233,19 +281,20 @@ __device__ __forceinline__ uint32_t colour_block(int colour) { return (uint32_t)
 // MODE 0: out = K u; MODE 1: out = f - K u.  Pinned origin row is zero.
 // (update_residual_otf_kernel_1 grid.cu:1277-1461; v3_stencilOnLeft_kernel grid.cu:3686-3857)
 template <int R, int MODE>
-__global__ __launch_bounds__(256, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
-                                                       const float* __restrict__ w, const float* __restrict__ u,
-                                                       const float* __restrict__ f, float* __restrict__ out) {
+__global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
+                                                                const float* __restrict__ w,
+                                                                const float* __restrict__ u,
+                                                                const float* __restrict__ f, float* __restrict__ out) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
+    uint32_t v;
+    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
     int x, y, z;
     L.pos(v, x, y, z);
     FineCtx c;
     fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, false, float, float>(c, L.n, N, ke_s, u, acc);
+    fine_gather<R, kRowAll, float, float>(c, L.n, N, ke_s, u, acc);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -257,13 +306,15 @@ __global__ __launch_bounds__(256, 4) void k_fine_apply(Lay L, size_t N, const fl
 }
 
 // One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
-template <int R, bool REVERSE>
+// ZC >= 0: colour ZC of a forward sweep that starts from u = 0 (colours above ZC are never read, so
+// u needs no clearing beforehand; the old value of the vertex itself is zero).
+template <int R, bool REVERSE, int ZC>
 __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour, float omega,
                                                     const float* __restrict__ ke, const float* __restrict__ w,
                                                     float* __restrict__ u, const float* __restrict__ f) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.h3) return;
     const uint32_t v = colour_block(colour) * L.h3 + t;
     if (v == 0) {
@@ -278,7 +329,7 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
     FineCtx c;
     fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, true, float, float>(c, L.n, N, ke_s, u, acc);
+    fine_gather<R, gs_mask(ZC), float, float>(c, L.n, N, ke_s, u, acc);
     float Ks[9];
     fine_block<13, float>(c.wc, ke_s, Ks);
 #pragma unroll
@@ -286,7 +337,7 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
         float uu[3], ff[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            uu[i] = u[(size_t)(r * 3 + i) * N + v];
+            uu[i] = ZC >= 0 ? 0.f : u[(size_t)(r * 3 + i) * N + v];
             ff[i] = f[(size_t)(r * 3 + i) * N + v];
         }
         block_update<REVERSE>(uu, ff, acc[r], Ks, omega);
@@ -297,10 +348,10 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
 
 // Macro-strain right-hand sides f_s = sum_e w_e KE chi0_s for the 6 strains (grid.cu:1818-1919).
 // kchi: 6 x 24 table of KE * chi0_s.  Pinned row zero.
-__global__ __launch_bounds__(256) void k_macro_rhs(Lay L, size_t N, const float* __restrict__ kchi,
-                                                   const float* __restrict__ w, float* __restrict__ f) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
+__global__ __launch_bounds__(kFullThreads) void k_macro_rhs(Lay L, size_t N, const float* __restrict__ kchi,
+                                                            const float* __restrict__ w, float* __restrict__ f) {
+    uint32_t v;
+    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
     int x, y, z;
     L.pos(v, x, y, z);
     FineCtx c;
@@ -319,27 +370,36 @@ __global__ __launch_bounds__(256) void k_macro_rhs(Lay L, size_t N, const float*
 // Iterative-refinement residual in FP64: r = f(w) - K u with u the FP64 solution, f the macro-strain
 // loads rebuilt from the cell weights in FP64 (kchi64 = KE chi0 in FP64) and every product
 // accumulated in FP64.  Writes r rounded to FP32 (the inner solve's right-hand side) and per-block
-// FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0 (r = -K u).
-template <int R, bool DO_F>
-__global__ __launch_bounds__(256, 2) void k_fine_residual64(Lay L, size_t N, int r0, const double* __restrict__ ke,
+// FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0 (r = -K u); DO_K = false
+// gives r = f (u = 0, nothing read from u).
+template <int R, bool DO_F, bool DO_K>
+__global__ __launch_bounds__(kFullThreads, 2) void k_fine_residual64(Lay L, size_t N, int r0, const double* __restrict__ ke,
                                                             const double* __restrict__ kchi,
                                                             const float* __restrict__ xc, double emin,
                                                             const double* __restrict__ u, float* __restrict__ r32,
                                                             double* __restrict__ partial) {
-    __shared__ double sh[4];
+    __shared__ double sh[kFullThreads / 64];
     __shared__ double ke_s[576];
     stage_ke(ke_s, ke);
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t v;
+    const bool live = full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v);
     double nrm[R];
 #pragma unroll
     for (int r = 0; r < R; r++) nrm[r] = 0.0;
-    if (v < N) {
+    if (live) {
         int x, y, z;
         L.pos(v, x, y, z);
         FineCtx c;
         fine_ctx64(c, x, y, z, L, xc, emin);
         double acc[R][3];
-        fine_gather<R, false, double, double>(c, L.n, N, ke_s, u, acc, r0);
+        if (DO_K) {
+            fine_gather<R, kRowAll, double, double>(c, L.n, N, ke_s, u, acc, r0);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) acc[r][i] = 0.0;
+        }
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
@@ -356,19 +416,20 @@ __global__ __launch_bounds__(256, 2) void k_fine_residual64(Lay L, size_t N, int
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const double s = block_sum<256>(nrm[r], sh);
+        const double s = block_sum<kFullThreads>(nrm[r], sh);
         if (threadIdx.x == 0) partial[(r0 + r) * gridDim.x + blockIdx.x] = s;
     }
 }
 
 // |f|^2 per load case of the FP64 macro-strain loads (pinned row excluded).
-__global__ __launch_bounds__(256) void k_rhs_norm64(Lay L, size_t N, const double* __restrict__ kchi,
-                                                    const float* __restrict__ xc, double emin,
-                                                    double* __restrict__ partial) {
-    __shared__ double sh[4];
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kFullThreads) void k_rhs_norm64(Lay L, size_t N, const double* __restrict__ kchi,
+                                                             const float* __restrict__ xc, double emin,
+                                                             double* __restrict__ partial) {
+    __shared__ double sh[kFullThreads / 64];
+    uint32_t v;
+    const bool live = full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v);
     double nrm[6] = {0, 0, 0, 0, 0, 0};
-    if (v < N && v != 0) {
+    if (live && v != 0) {
         int x, y, z;
         L.pos(v, x, y, z);
         FineCtx c;
@@ -385,7 +446,7 @@ __global__ __launch_bounds__(256) void k_rhs_norm64(Lay L, size_t N, const doubl
     }
 #pragma unroll
     for (int s = 0; s < 6; s++) {
-        const double t = block_sum<256>(nrm[s], sh);
+        const double t = block_sum<kFullThreads>(nrm[s], sh);
         if (threadIdx.x == 0) partial[s * gridDim.x + blockIdx.x] = t;
     }
 }
@@ -416,10 +477,10 @@ __global__ void k_zero_pin(size_t N, T* __restrict__ u) {
 // Coarse levels (stored 27-point stencils)
 // ---------------------------------------------------------------------------------------------
 
-template <int R, bool SKIP_SELF>
+template <int R, uint32_t MASK>
 __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L, size_t N, uint32_t v,
                                                const float* __restrict__ st, const float* __restrict__ u,
-                                               float acc[R][3]) {
+                                               float acc[R][3], int r0) {
     const uint32_t X[3] = {L.part(wrap_dec(x, L.n), 0), L.part(x, 0), L.part(wrap_inc(x, L.n), 0)};
     const uint32_t Y[3] = {L.part(wrap_dec(y, L.n), 1), L.part(y, 1), L.part(wrap_inc(y, L.n), 1)};
     const uint32_t Z[3] = {L.part(wrap_dec(z, L.n), 2), L.part(z, 2), L.part(wrap_inc(z, L.n), 2)};
@@ -433,16 +494,16 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L
         for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
 #pragma unroll
     for (int d = 0; d < 27; d++) {
-        if (SKIP_SELF && d == 13) continue;
+        if (!((MASK >> d) & 1u)) continue;
         float S[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) S[k] = bload<float>(rs, vo, (uint32_t)(d * 9 + k) * plane);
         const uint32_t qo = (X[d % 3] + Y[d / 3 % 3] + Z[d / 9]) * 4u;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const float u0 = bload<float>(ru, qo, (uint32_t)(r * 3 + 0) * plane);
-            const float u1 = bload<float>(ru, qo, (uint32_t)(r * 3 + 1) * plane);
-            const float u2 = bload<float>(ru, qo, (uint32_t)(r * 3 + 2) * plane);
+            const float u0 = bload<float>(ru, qo, (uint32_t)((r0 + r) * 3 + 0) * plane);
+            const float u1 = bload<float>(ru, qo, (uint32_t)((r0 + r) * 3 + 1) * plane);
+            const float u2 = bload<float>(ru, qo, (uint32_t)((r0 + r) * 3 + 2) * plane);
             acc[r][0] = fmaf(S[0], u0, fmaf(S[1], u1, fmaf(S[2], u2, acc[r][0])));
             acc[r][1] = fmaf(S[3], u0, fmaf(S[4], u1, fmaf(S[5], u2, acc[r][1])));
             acc[r][2] = fmaf(S[6], u0, fmaf(S[7], u1, fmaf(S[8], u2, acc[r][2])));
@@ -452,36 +513,39 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L
 
 // MODE 0: out = A u; MODE 1: out = f - A u (update_residual_kernel_1, grid.cu:1465-1564)
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_stencil_apply(Lay L, size_t N, const float* __restrict__ st,
-                                                       const float* __restrict__ u, const float* __restrict__ f,
-                                                       float* __restrict__ out) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
+__global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N, const float* __restrict__ st,
+                                                                const float* __restrict__ u,
+                                                                const float* __restrict__ f, float* __restrict__ out) {
+    uint32_t v;
+    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
     int x, y, z;
     L.pos(v, x, y, z);
     float acc[R][3];
-    stencil_gather<R, false>(x, y, z, L, N, v, st, u, acc);
+    const int r0 = blockIdx.y * R;
+    stencil_gather<R, kRowAll>(x, y, z, L, N, v, st, u, acc, r0);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const size_t o = (size_t)(r * 3 + i) * N + v;
+            const size_t o = (size_t)((r0 + r) * 3 + i) * N + v;
             out[o] = (MODE == 1) ? f[o] - acc[r][i] : acc[r][i];
         }
 }
 
 // One colour of the coarse block GS sweep (gs_relaxation_kernel, grid.cu:556-689); no pin on coarse levels.
-template <int R, bool REVERSE>
+// ZC as in k_fine_gs.
+template <int R, bool REVERSE, int ZC>
 __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour, float omega,
                                                     const float* __restrict__ st, float* __restrict__ u,
                                                     const float* __restrict__ f) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.h3) return;
     const uint32_t v = colour_block(colour) * L.h3 + t;
     int x, y, z;
     L.pos(v, x, y, z);
     float acc[R][3];
-    stencil_gather<R, true>(x, y, z, L, N, v, st, u, acc);
+    const int r0 = blockIdx.y * R;
+    stencil_gather<R, gs_mask(ZC)>(x, y, z, L, N, v, st, u, acc, r0);
     float Ks[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) Ks[k] = st[(size_t)(13 * 9 + k) * N + v];
@@ -490,13 +554,26 @@ __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour,
         float uu[3], ff[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            uu[i] = u[(size_t)(r * 3 + i) * N + v];
-            ff[i] = f[(size_t)(r * 3 + i) * N + v];
+            uu[i] = ZC >= 0 ? 0.f : u[(size_t)((r0 + r) * 3 + i) * N + v];
+            ff[i] = f[(size_t)((r0 + r) * 3 + i) * N + v];
         }
         block_update<REVERSE>(uu, ff, acc[r], Ks, omega);
 #pragma unroll
-        for (int i = 0; i < 3; i++) u[(size_t)(r * 3 + i) * N + v] = uu[i];
+        for (int i = 0; i < 3; i++) u[(size_t)((r0 + r) * 3 + i) * N + v] = uu[i];
     }
+}
+
+// Coarse-vertex kernels that read the fine level around 2V take coarse vertices in lexicographic
+// order whatever the coarse storage, so that consecutive lanes read consecutive fine local positions.
+__device__ __forceinline__ bool coarse_lex_vertex(const Lay& Lc, int& X, int& Y, int& Z, uint32_t& V) {
+    const uint32_t i = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)Lc.n;
+    if (i >= n * n * n) return false;
+    X = (int)(i % n);
+    Y = (int)((i / n) % n);
+    Z = (int)(i / (n * n));
+    V = Lc.idx(X, Y, Z);
+    return true;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -507,10 +584,9 @@ template <int R>
 __global__ __launch_bounds__(256) void k_restrict(Lay Lf, Lay Lc, const float* __restrict__ rf,
                                                   float* __restrict__ fc) {
     const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n, Nf = (size_t)Lf.n * Lf.n * Lf.n;
-    const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
-    if (V >= Nc) return;
     int X, Y, Z;
-    Lc.pos(V, X, Y, Z);
+    uint32_t V;
+    if (!coarse_lex_vertex(Lc, X, Y, Z, V)) return;
     const uint32_t xs[3] = {Lf.part(wrap_dec(2 * X, Lf.n), 0), Lf.part(2 * X, 0), Lf.part(2 * X + 1, 0)};
     const uint32_t ys[3] = {Lf.part(wrap_dec(2 * Y, Lf.n), 1), Lf.part(2 * Y, 1), Lf.part(2 * Y + 1, 1)};
     const uint32_t zs[3] = {Lf.part(wrap_dec(2 * Z, Lf.n), 2), Lf.part(2 * Z, 2), Lf.part(2 * Z + 1, 2)};
@@ -519,6 +595,7 @@ __global__ __launch_bounds__(256) void k_restrict(Lay Lf, Lay Lc, const float* _
     for (int r = 0; r < R; r++)
 #pragma unroll
         for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
+    const int r0 = blockIdx.y * R;
 #pragma unroll
     for (int o = 0; o < 27; o++) {
         const int ox = o % 3, oy = o / 3 % 3, oz = o / 9;
@@ -527,20 +604,21 @@ __global__ __launch_bounds__(256) void k_restrict(Lay Lf, Lay Lc, const float* _
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int i = 0; i < 3; i++) acc[r][i] = fmaf(wgt, rf[(size_t)(r * 3 + i) * Nf + q], acc[r][i]);
+            for (int i = 0; i < 3; i++)
+                acc[r][i] = fmaf(wgt, rf[(size_t)((r0 + r) * 3 + i) * Nf + q], acc[r][i]);
     }
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) fc[(size_t)(r * 3 + i) * Nc + V] = acc[r][i];
+        for (int i = 0; i < 3; i++) fc[(size_t)((r0 + r) * 3 + i) * Nc + V] = acc[r][i];
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_prolong(Lay Lf, Lay Lc, const float* __restrict__ uc,
-                                                 float* __restrict__ uf, int pin) {
+__global__ __launch_bounds__(kFullThreads) void k_prolong(Lay Lf, Lay Lc, const float* __restrict__ uc,
+                                                          float* __restrict__ uf, int pin) {
     const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n, Nf = (size_t)Lf.n * Lf.n * Lf.n;
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= Nf) return;
+    uint32_t v;
+    if (!full_vertex(Lf, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
     if (pin && v == 0) return;
     const int nc = Lc.n;
     int x, y, z;
@@ -559,6 +637,7 @@ __global__ __launch_bounds__(256) void k_prolong(Lay Lf, Lay Lc, const float* __
     for (int r = 0; r < R; r++)
 #pragma unroll
         for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
+    const int r0 = blockIdx.y * R;
 #pragma unroll
     for (int p = 0; p < 8; p++) {
         const float wgt = wx[p & 1] * wy[(p >> 1) & 1] * wz[p >> 2];
@@ -567,12 +646,13 @@ __global__ __launch_bounds__(256) void k_prolong(Lay Lf, Lay Lc, const float* __
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int i = 0; i < 3; i++) acc[r][i] = fmaf(wgt, uc[(size_t)(r * 3 + i) * Nc + q], acc[r][i]);
+            for (int i = 0; i < 3; i++)
+                acc[r][i] = fmaf(wgt, uc[(size_t)((r0 + r) * 3 + i) * Nc + q], acc[r][i]);
     }
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) uf[(size_t)(r * 3 + i) * Nf + v] += acc[r][i];
+        for (int i = 0; i < 3; i++) uf[(size_t)((r0 + r) * 3 + i) * Nf + v] += acc[r][i];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -589,10 +669,9 @@ __global__ __launch_bounds__(256) void k_galerkin_fine(Lay Lf, Lay Lc, const flo
                                                        float clamp) {
     const int nf = Lf.n;
     const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n;
-    const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
-    if (V >= Nc) return;
     int X, Y, Z;
-    Lc.pos(V, X, Y, Z);
+    uint32_t V;
+    if (!coarse_lex_vertex(Lc, X, Y, Z, V)) return;
     float wc[64];
 #pragma unroll
     for (int c = 0; c < 64; c++) {
@@ -678,14 +757,14 @@ __global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const f
                                                          float* __restrict__ stc, float clamp) {
     const int nf = Lf.n;
     const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n, Nf = (size_t)nf * nf * nf;
-    const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
-    if (V >= Nc) return;
     int X, Y, Z;
-    Lc.pos(V, X, Y, Z);
+    uint32_t V;
+    if (!coarse_lex_vertex(Lc, X, Y, Z, V)) return;
     const uint32_t xs[3] = {Lf.part(wrap_dec(2 * X, nf), 0), Lf.part(2 * X, 0), Lf.part(2 * X + 1, 0)};
     const uint32_t ys[3] = {Lf.part(wrap_dec(2 * Y, nf), 1), Lf.part(2 * Y, 1), Lf.part(2 * Y + 1, 1)};
     const uint32_t zs[3] = {Lf.part(wrap_dec(2 * Z, nf), 2), Lf.part(2 * Z, 2), Lf.part(2 * Z + 1, 2)};
-    for (int D = 0; D < 27; D++) {
+    {
+        const int D = blockIdx.y;  // one coarse stencil direction per grid row
         const int Dx = D % 3 - 1, Dy = D / 3 % 3 - 1, Dz = D / 9 - 1;
         float S[9];
 #pragma unroll
@@ -739,20 +818,28 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int m, size_t Nc, const do
 // ---------------------------------------------------------------------------------------------
 // FP64 reductions and vector updates (per load case)
 // ---------------------------------------------------------------------------------------------
-// partial[r * nblocks + b] = sum over block b's grid-stride share of a_r . b_r  (3N values per r)
-template <int R>
+// partial[r * gridDim.x + b] = block b's grid-stride share of a_r . b_r, r = blockIdx.y (3N values
+// per load case); products are exact in FP64 and summed in FP64.
 __global__ __launch_bounds__(256) void k_dot(size_t M, const float* __restrict__ a, const float* __restrict__ b,
                                              double* __restrict__ partial) {
     __shared__ double sh[4];
-    for (int r = 0; r < R; r++) {
-        double s = 0.0;
-        const float* ar = a + (size_t)r * M;
-        const float* br = b + (size_t)r * M;
-        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += (size_t)gridDim.x * blockDim.x)
-            s += (double)ar[i] * (double)br[i];
-        s = block_sum<256>(s, sh);
-        if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = s;
+    const int r = blockIdx.y;
+    const float* ar = a + (size_t)r * M;
+    const float* br = b + (size_t)r * M;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    double s = 0.0;
+    if ((M & 3) == 0) {
+        const float4* a4 = reinterpret_cast<const float4*>(ar);
+        const float4* b4 = reinterpret_cast<const float4*>(br);
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += stride) {
+            const float4 x = a4[i], y = b4[i];
+            s += ((double)x.x * y.x + (double)x.y * y.y) + ((double)x.z * y.z + (double)x.w * y.w);
+        }
+    } else {
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride) s += (double)ar[i] * br[i];
     }
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = s;
 }
 
 __global__ void k_sum_partials(int R, int nb, const double* __restrict__ partial, double* __restrict__ out) {
@@ -866,14 +953,18 @@ __device__ __forceinline__ void cell_energies(const Lay& L, size_t N, uint32_t e
 }
 
 // partial[k * nblocks + block] = sum of w_e E_k,e over the block's cells (FP64)
-__global__ __launch_bounds__(256) void k_elastic_matrix(Lay L, size_t N, const float* __restrict__ ke,
-                                                        const float* __restrict__ w, const float* __restrict__ chi,
-                                                        double* __restrict__ partial) {
-    __shared__ double sh[4];
+__global__ __launch_bounds__(kFullThreads) void k_elastic_matrix(Lay L, size_t N, uint32_t nchunks,
+                                                                 const float* __restrict__ ke,
+                                                                 const float* __restrict__ w,
+                                                                 const float* __restrict__ chi,
+                                                                 double* __restrict__ partial) {
+    __shared__ double sh[kFullThreads / 64];
     double acc[21];
 #pragma unroll
     for (int k = 0; k < 21; k++) acc[k] = 0.0;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < N; e += gridDim.x * blockDim.x) {
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        uint32_t e;
+        if (!full_vertex(L, chunk, e)) continue;
         float E[21];
         int x, y, z;
         cell_energies(L, N, e, ke, chi, E, x, y, z);
@@ -882,17 +973,19 @@ __global__ __launch_bounds__(256) void k_elastic_matrix(Lay L, size_t N, const f
         for (int k = 0; k < 21; k++) acc[k] += we * (double)E[k];
     }
     for (int k = 0; k < 21; k++) {
-        const double s = block_sum<256>(acc[k], sh);
+        const double s = block_sum<kFullThreads>(acc[k], sh);
         if (threadIdx.x == 0) partial[k * gridDim.x + blockIdx.x] = s;
     }
 }
 
 // sens[x + (y + z*n)*pitch] = (1/N) sum_{i<=j} E_ij (dC_ij + [i!=j] dC_ji)
-__global__ __launch_bounds__(256) void k_sensitivity(Lay L, size_t N, int pitch, const float* __restrict__ ke,
-                                                     const float* __restrict__ chi, const float* __restrict__ wdc,
-                                                     float* __restrict__ sens) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N) return;
+__global__ __launch_bounds__(kFullThreads) void k_sensitivity(Lay L, size_t N, int pitch,
+                                                              const float* __restrict__ ke,
+                                                              const float* __restrict__ chi,
+                                                              const float* __restrict__ wdc,
+                                                              float* __restrict__ sens) {
+    uint32_t e;
+    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), e)) return;
     float E[21];
     int x, y, z;
     cell_energies(L, N, e, ke, chi, E, x, y, z);
@@ -1018,11 +1111,11 @@ __global__ __launch_bounds__(256) void k_absmax(long N, int n, int pitch, const 
 // pitched lexicographic density -> cell weights w = emin + x and raw x in level storage order
 __global__ __launch_bounds__(256) void k_set_weights(Lay L, int pitch, double emin, const float* __restrict__ x,
                                                      float* __restrict__ w, float* __restrict__ xc) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t N = (uint32_t)L.n * L.n * L.n;
-    if (e >= N) return;
-    int xx, y, z;
-    L.pos(e, xx, y, z);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)L.n;
+    if (i >= n * n * n) return;
+    const int xx = (int)(i % n), y = (int)((i / n) % n), z = (int)(i / (n * n));
+    const uint32_t e = L.idx(xx, y, z);
     const float xv = x[xx + ((size_t)y + (size_t)z * L.n) * pitch];
     xc[e] = xv;
     w[e] = (float)(emin + (double)xv);
@@ -1053,52 +1146,84 @@ static Lay make_lay(int n, int lex) {
     return L;
 }
 
+// workgroups of a full-level kernel (see full_vertex)
+static unsigned full_grid(const Lay& L) {
+    const size_t N = (size_t)L.n * L.n * L.n;
+    return L.lex ? nblk(N, kFullThreads) : nblk(L.h3, 64);
+}
+
 void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const float* u, const float* f,
                        float* out) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
+    const dim3 g(full_grid(L)), b(kFullThreads);
     if (f)
-        hipLaunchKernelGGL((k_fine_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, L, N, ke, w, u, f, out);
+        hipLaunchKernelGGL((k_fine_apply<6, 1>), g, b, 0, s, L, N, ke, w, u, f, out);
     else
-        hipLaunchKernelGGL((k_fine_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, L, N, ke, w, u, f, out);
+        hipLaunchKernelGGL((k_fine_apply<6, 0>), g, b, 0, s, L, N, ke, w, u, f, out);
 }
 
-void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* ke, const float* w,
-                    float* u, const float* f) {
+template <bool REV, int ZC>
+static void fine_gs_colour(hipStream_t s, const Lay& L, size_t N, int colour, float omega, const float* ke,
+                           const float* w, float* u, const float* f) {
+    hipLaunchKernelGGL((k_fine_gs<6, REV, ZC>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, ke, w, u, f);
+}
+
+void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
+                    const float* w, float* u, const float* f) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
-    if (reverse)
-        hipLaunchKernelGGL((k_fine_gs<6, true>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, ke, w, u, f);
-    else
-        hipLaunchKernelGGL((k_fine_gs<6, false>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, ke, w, u, f);
+    if (reverse) {
+        fine_gs_colour<true, -1>(s, L, N, colour, omega, ke, w, u, f);
+        return;
+    }
+    switch (zero_start ? colour : -1) {
+        case 0: fine_gs_colour<false, 0>(s, L, N, colour, omega, ke, w, u, f); break;
+        case 1: fine_gs_colour<false, 1>(s, L, N, colour, omega, ke, w, u, f); break;
+        case 2: fine_gs_colour<false, 2>(s, L, N, colour, omega, ke, w, u, f); break;
+        case 3: fine_gs_colour<false, 3>(s, L, N, colour, omega, ke, w, u, f); break;
+        case 4: fine_gs_colour<false, 4>(s, L, N, colour, omega, ke, w, u, f); break;
+        case 5: fine_gs_colour<false, 5>(s, L, N, colour, omega, ke, w, u, f); break;
+        case 6: fine_gs_colour<false, 6>(s, L, N, colour, omega, ke, w, u, f); break;
+        case 7: fine_gs_colour<false, 7>(s, L, N, colour, omega, ke, w, u, f); break;
+        default: fine_gs_colour<false, -1>(s, L, N, colour, omega, ke, w, u, f); break;
+    }
 }
 
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_macro_rhs, dim3(nblk(N)), dim3(256), 0, s, make_lay(n, 0), N, kchi, w, f);
+    const Lay L = make_lay(n, 0);
+    hipLaunchKernelGGL(k_macro_rhs, dim3(full_grid(L)), dim3(kFullThreads), 0, s, L, N, kchi, w, f);
+}
+
+template <bool DO_F, bool DO_K>
+static void residual64_case(hipStream_t s, const Lay& L, size_t N, int r0, const double* ke, const double* kchi,
+                            const float* xc, double emin, const double* u, float* r32, double* partial) {
+    hipLaunchKernelGGL((k_fine_residual64<1, DO_F, DO_K>), dim3(full_grid(L)), dim3(kFullThreads), 0, s, L, N, r0, ke,
+                       kchi, xc, emin, u, r32, partial);
 }
 
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
-                            const double* u, float* r32, double* partial, double* out6, int do_f) {
+                            const double* u, float* r32, double* partial, double* out6, int do_f, int do_k) {
     const size_t N = (size_t)n * n * n;
-    const unsigned nb = nblk(N);
     const Lay L = make_lay(n, 0);
     for (int r0 = 0; r0 < 6; r0 += 1) {
-        if (do_f)
-            hipLaunchKernelGGL((k_fine_residual64<1, true>), dim3(nb), dim3(256), 0, s, L, N, r0, ke, kchi, xc, emin, u,
-                               r32, partial);
+        if (!do_k)
+            residual64_case<true, false>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
+        else if (do_f)
+            residual64_case<true, true>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
         else
-            hipLaunchKernelGGL((k_fine_residual64<1, false>), dim3(nb), dim3(256), 0, s, L, N, r0, ke, kchi, xc, emin,
-                               u, r32, partial);
+            residual64_case<false, true>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
     }
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)nb, partial, out6);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)full_grid(L), partial, out6);
 }
 
 void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
                        double* out6) {
     const size_t N = (size_t)n * n * n;
-    const unsigned nb = nblk(N);
-    hipLaunchKernelGGL(k_rhs_norm64, dim3(nb), dim3(256), 0, s, make_lay(n, 0), N, kchi, xc, emin, partial);
+    const Lay L = make_lay(n, 0);
+    const unsigned nb = full_grid(L);
+    hipLaunchKernelGGL(k_rhs_norm64, dim3(nb), dim3(kFullThreads), 0, s, L, N, kchi, xc, emin, partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)nb, partial, out6);
 }
 
@@ -1122,36 +1247,79 @@ void launch_zero_pin64(hipStream_t s, size_t N, double* u) {
     hipLaunchKernelGGL((k_zero_pin<double>), dim3(1), dim3(64), 0, s, N, u);
 }
 
+// Levels up to this size run the coarse-stencil kernels with one load case per thread (six grid
+// rows): their stencils sit in cache, and the extra threads hide the gather latency that dominates
+// there.  Larger levels keep all six load cases per thread so the stored stencil is read once.
+constexpr int kSplitLoadCasesMaxN = 32;
+
 void launch_stencil_apply(hipStream_t s, int n, int lex, const float* st, const float* u, const float* f,
                           float* out) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, lex);
-    if (f)
-        hipLaunchKernelGGL((k_stencil_apply<6, 1>), dim3(nblk(N)), dim3(256), 0, s, L, N, st, u, f, out);
-    else
-        hipLaunchKernelGGL((k_stencil_apply<6, 0>), dim3(nblk(N)), dim3(256), 0, s, L, N, st, u, f, out);
+    const bool split = n <= kSplitLoadCasesMaxN;
+    const dim3 g(full_grid(L), split ? 6 : 1), b(kFullThreads);
+    if (split) {
+        if (f)
+            hipLaunchKernelGGL((k_stencil_apply<1, 1>), g, b, 0, s, L, N, st, u, f, out);
+        else
+            hipLaunchKernelGGL((k_stencil_apply<1, 0>), g, b, 0, s, L, N, st, u, f, out);
+    } else {
+        if (f)
+            hipLaunchKernelGGL((k_stencil_apply<6, 1>), g, b, 0, s, L, N, st, u, f, out);
+        else
+            hipLaunchKernelGGL((k_stencil_apply<6, 0>), g, b, 0, s, L, N, st, u, f, out);
+    }
 }
 
-void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, const float* st, float* u,
-                       const float* f) {
+template <int R, bool REV, int ZC>
+static void stencil_gs_colour(hipStream_t s, const Lay& L, size_t N, int colour, float omega, const float* st,
+                              float* u, const float* f) {
+    hipLaunchKernelGGL((k_stencil_gs<R, REV, ZC>), dim3(nblk(L.h3), 6 / R), dim3(256), 0, s, L, N, colour, omega, st,
+                       u, f);
+}
+
+template <int R>
+static void stencil_gs_pick(hipStream_t s, const Lay& L, size_t N, int colour, float omega, int reverse,
+                            int zero_start, const float* st, float* u, const float* f) {
+    if (reverse) {
+        stencil_gs_colour<R, true, -1>(s, L, N, colour, omega, st, u, f);
+        return;
+    }
+    switch (zero_start ? colour : -1) {
+        case 0: stencil_gs_colour<R, false, 0>(s, L, N, colour, omega, st, u, f); break;
+        case 1: stencil_gs_colour<R, false, 1>(s, L, N, colour, omega, st, u, f); break;
+        case 2: stencil_gs_colour<R, false, 2>(s, L, N, colour, omega, st, u, f); break;
+        case 3: stencil_gs_colour<R, false, 3>(s, L, N, colour, omega, st, u, f); break;
+        case 4: stencil_gs_colour<R, false, 4>(s, L, N, colour, omega, st, u, f); break;
+        case 5: stencil_gs_colour<R, false, 5>(s, L, N, colour, omega, st, u, f); break;
+        case 6: stencil_gs_colour<R, false, 6>(s, L, N, colour, omega, st, u, f); break;
+        case 7: stencil_gs_colour<R, false, 7>(s, L, N, colour, omega, st, u, f); break;
+        default: stencil_gs_colour<R, false, -1>(s, L, N, colour, omega, st, u, f); break;
+    }
+}
+
+void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* st,
+                       float* u, const float* f) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
-    if (reverse)
-        hipLaunchKernelGGL((k_stencil_gs<6, true>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, st, u, f);
+    if (n <= kSplitLoadCasesMaxN)
+        stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, st, u, f);
     else
-        hipLaunchKernelGGL((k_stencil_gs<6, false>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, st, u, f);
+        stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, st, u, f);
 }
 
+// Transfers take one load case per thread (six grid rows): the load cases' planes are disjoint, so
+// this costs no extra traffic and shortens each thread's chain of dependent loads.
 void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc) {
     const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
-    hipLaunchKernelGGL((k_restrict<6>), dim3(nblk(Nc)), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc), rf,
-                       fc);
+    hipLaunchKernelGGL((k_restrict<1>), dim3(nblk(Nc), 6), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc),
+                       rf, fc);
 }
 
 void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, float* uf, int pin) {
-    const size_t Nf = (size_t)nf * nf * nf;
-    hipLaunchKernelGGL((k_prolong<6>), dim3(nblk(Nf)), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc), uc, uf,
-                       pin);
+    const Lay Lf = make_lay(nf, 0);
+    hipLaunchKernelGGL((k_prolong<1>), dim3(full_grid(Lf), 6), dim3(kFullThreads), 0, s, Lf, make_lay(nf / 2, lexc),
+                       uc, uf, pin);
 }
 
 void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
@@ -1164,8 +1332,8 @@ void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const
 
 void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const float* stf, float* stc, float clamp) {
     const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
-    hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(Nc)), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc), stf,
-                       stc, clamp);
+    hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(Nc), 27), dim3(256), 0, s, make_lay(nf, 0),
+                       make_lay(nf / 2, lexc), stf, stc, clamp);
 }
 
 void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float* f, float* u) {
@@ -1176,7 +1344,7 @@ void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float*
 
 void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks,
                  double* out) {
-    hipLaunchKernelGGL((k_dot<6>), dim3(nblocks), dim3(256), 0, s, M, a, b, partial);
+    hipLaunchKernelGGL(k_dot, dim3(nblocks, 6), dim3(256), 0, s, M, a, b, partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, nblocks, partial, out);
 }
 
@@ -1199,14 +1367,18 @@ void launch_pcg_update_p(hipStream_t s, size_t M, const double* sc, const float*
 void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,
                            int nblocks, double* out21) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_elastic_matrix, dim3(nblocks), dim3(256), 0, s, make_lay(n, 0), N, ke, w, chi, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 21, nblocks, partial, out21);
+    const Lay L = make_lay(n, 0);
+    const unsigned nchunks = full_grid(L);
+    const int nb = (int)std::min<unsigned>((unsigned)nblocks, nchunks);
+    hipLaunchKernelGGL(k_elastic_matrix, dim3(nb), dim3(kFullThreads), 0, s, L, N, nchunks, ke, w, chi, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 21, nb, partial, out21);
 }
 
 void launch_sensitivity(hipStream_t s, int n, int pitch, const float* ke, const float* chi, const float* wdc,
                         float* sens) {
     const size_t N = (size_t)n * n * n;
-    hipLaunchKernelGGL(k_sensitivity, dim3(nblk(N)), dim3(256), 0, s, make_lay(n, 0), N, pitch, ke, chi, wdc, sens);
+    const Lay L = make_lay(n, 0);
+    hipLaunchKernelGGL(k_sensitivity, dim3(full_grid(L)), dim3(kFullThreads), 0, s, L, N, pitch, ke, chi, wdc, sens);
 }
 
 void launch_symmetrize(hipStream_t s, int n, int pitch, int ng, const int* gperm, const int* gflip, const float* in,
