@@ -296,4 +296,18 @@ private:
     homo3d_mma* h_ = nullptr;
 };
 
+// Optimisation drivers (Framework.cu:137-366, homogenization/testAutoDiff.cu:571-607), implemented in
+// libhomo3d_framework.so.  Each returns the objective history; outputs go to config.outprefix.
+std::vector<double> example_opti_bulk(cfg::HomoConfig config);
+std::vector<double> example_opti_npr(cfg::HomoConfig config);
+std::vector<double> example_opti_shear_isotropy(cfg::HomoConfig config);
+std::vector<double> example_opti_npr_mma(cfg::HomoConfig config);
+void example_yours(cfg::HomoConfig config);  // weak user hook, replaced by a strong definition
+void runCustom(cfg::HomoConfig config);
+std::vector<double> runInstance(cfg::HomoConfig config);
+
 }  // namespace homo
+
+// The reference's main (main.cpp:22-48) as a C entry point: 0 on success, 1 on a flag or I/O error,
+// 2 on a solver error.
+extern "C" int homo3d_framework_main(int argc, char** argv);
