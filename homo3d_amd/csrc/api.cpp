@@ -399,23 +399,27 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         any = any || act[r];
     }
     if (!any) return HOMO3D_OK;
+    if (M % 4) return set_err(h, HOMO3D_NO_SUPPORT, "CG vectors need 4 | 3n^3");
     HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                              h->stream));
     vcycle(h, h->Z, h->Rr, 1, nullptr, true);
     HIPCHK(h, hipMemcpyAsync(h->P, h->Z, vb, hipMemcpyDeviceToDevice, h->stream));
     launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc);  // rz
+    // One CG step: q = K p with p.q; x is advanced by alpha p one step late, fused with the update of
+    // p (and by a last x-only pass on exit), so that p, x, r and q are each streamed once per step.
     double rr[kRhs];
+    bool x_pending = false;
     for (int it = 0; it < maxit; it++) {
         {
             ProfEvent ev;
             prof_begin(h, 1, ev);
-            launch_fine_apply(h->stream, L0.n, h->d_ke, h->d_w, h->P, nullptr, h->Q);
+            launch_fine_apply_dot(h->stream, L0.n, h->d_ke, h->d_w, h->P, h->Q, h->d_partial, kDotBlocks,
+                                  h->d_sc + kRhs);  // pq
             prof_end(h, ev);
         }
-        launch_dot6(h->stream, M, h->P, h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
         launch_pcg_alpha(h->stream, h->d_sc);
-        launch_pcg_update_xr(h->stream, M, h->d_sc, h->P, h->Q, h->X, h->Rr);
-        launch_dot6(h->stream, M, h->Rr, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
+        launch_pcg_update_r(h->stream, M, h->d_sc, h->Q, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
+        x_pending = true;
         HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         bool changed = false, all = true;
@@ -437,8 +441,10 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         vcycle(h, h->Z, h->Rr, 1, nullptr, true);
         launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
         launch_pcg_beta(h->stream, h->d_sc);
-        launch_pcg_update_p(h->stream, M, h->d_sc, h->Z, h->P);
+        launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, h->P, h->X, 1, kDotBlocks);
+        x_pending = false;
     }
+    if (x_pending) launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, h->P, h->X, 0, kDotBlocks);
     return HOMO3D_OK;
 }
 
@@ -534,7 +540,8 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
         return fail(HOMO3D_HIP_ERROR);
     hipMemset(h->X, 0, vb);
     hipMemset(h->F, 0, vb);
-    h->partial_elems = std::max((size_t)21 * kDotBlocks, 6 * ((N0 + 255) / 256));
+    // per-block partial sums (<= 6 per 256 vertices) and per-wave ones (6 per 64 vertices)
+    h->partial_elems = std::max((size_t)21 * kDotBlocks, 6 * ((N0 + 63) / 64));
     if (hipMalloc(&h->U64, sizeof(double) * 3 * kRhs * N0)) return fail(HOMO3D_HIP_ERROR);
     hipMemset(h->U64, 0, sizeof(double) * 3 * kRhs * N0);
     if (hipMalloc(&h->d_partial, sizeof(double) * h->partial_elems) || hipMalloc(&h->d_sc, sizeof(double) * 64) ||

@@ -7,6 +7,10 @@ namespace homo3d {
 
 void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const float* u, const float* f,
                        float* out);
+// out = K u with the per-load-case FP64 dot u . out -> out6.  partial holds 6 * n^3 / 512 doubles (and
+// at least 6 * dot_blocks).
+void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const float* u, float* out,
+                           double* partial, int dot_blocks, double* out6);
 // zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
                     const float* w, float* u, const float* f);
@@ -34,9 +38,12 @@ void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double
 void launch_sum_partials(hipStream_t s, int R, int nblocks, const double* partial, double* out);
 void launch_pcg_alpha(hipStream_t s, double* sc);
 void launch_pcg_beta(hipStream_t s, double* sc);
-void launch_pcg_update_xr(hipStream_t s, size_t M, const double* sc, const float* p, const float* q, float* x,
-                          float* r);
-void launch_pcg_update_p(hipStream_t s, size_t M, const double* sc, const float* z, float* p);
+// CG updates over 6 load cases of M floats each (M % 4 == 0): r -= alpha q with |r|^2 -> out6;
+// x += alpha p and, when do_p, p = z + beta p for the active load cases.
+void launch_pcg_update_r(hipStream_t s, size_t M, const double* sc, const float* q, float* r, double* partial,
+                         int nblocks, double* out6);
+void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const float* z, float* p, float* x, int do_p,
+                          int nblocks);
 void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,
                            int nblocks, double* out21);
 void launch_sensitivity(hipStream_t s, int n, int pitch, const float* ke, const float* chi, const float* wdc,

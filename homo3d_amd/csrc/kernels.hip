@@ -280,11 +280,15 @@ __device__ __forceinline__ uint32_t colour_block(int colour) { return (uint32_t)
 
 // MODE 0: out = K u; MODE 1: out = f - K u.  Pinned origin row is zero.
 // (update_residual_otf_kernel_1 grid.cu:1277-1461; v3_stencilOnLeft_kernel grid.cu:3686-3857)
-template <int R, int MODE>
+// DOT (MODE 0 only): also the per-workgroup FP64 partial sums of u . out per load case (the p . Kp of
+// a CG step) in partial[r * gridDim.x + block].  The launcher uses it only when every lane of the
+// grid holds a vertex (64 | (n/2)^3), so no thread leaves before the workgroup reduction.
+template <int R, int MODE, bool DOT>
 __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
                                                                 const float* __restrict__ w,
                                                                 const float* __restrict__ u,
-                                                                const float* __restrict__ f, float* __restrict__ out) {
+                                                                const float* __restrict__ f, float* __restrict__ out,
+                                                                double* __restrict__ partial) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
     uint32_t v;
@@ -303,6 +307,22 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
             float val = (MODE == 1) ? f[o] - acc[r][i] : acc[r][i];
             out[o] = (v == 0) ? 0.f : val;
         }
+    if constexpr (DOT) {
+        __shared__ double sh[kFullThreads / 64];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            double d = 0.0;
+            if (v != 0) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const size_t o = (size_t)(r * 3 + i) * N + v;
+                    d = fma((double)u[o], (double)acc[r][i], d);
+                }
+            }
+            d = block_sum<kFullThreads>(d, sh);
+            if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = d;
+        }
+    }
 }
 
 // One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
@@ -655,6 +675,105 @@ __global__ __launch_bounds__(kFullThreads) void k_prolong(Lay Lf, Lay Lc, const 
         for (int i = 0; i < 3; i++) uf[(size_t)((r0 + r) * 3 + i) * Nf + v] += acc[r][i];
 }
 
+// Vectorised transfers for nf % 8 == 0 (so nc = nf / 2 and the fine local row length are multiples of
+// 4): each thread handles four consecutive x of one load case (r = blockIdx.y) with float4 / float2
+// accesses.  Same weights and sums as k_restrict / k_prolong.
+//
+// Restriction: coarse X = 4j..4j+3 of row (Y, Z).  Along x the fine taps are 2X (even block, local X),
+// 2X+1 (odd block, local X) and 2X-1 (odd block, local X-1): two float4 rows and one scalar.
+__global__ __launch_bounds__(256) void k_restrict4(Lay Lf, Lay Lc, const float* __restrict__ rf,
+                                                   float* __restrict__ fc) {
+    const int nf = Lf.n, nc = Lc.n, q = nc / 4;
+    const uint32_t i = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)q * nc * nc) return;
+    const int j = (int)(i % q), Y = (int)((i / q) % nc), Z = (int)(i / ((uint32_t)q * nc));
+    const size_t Nf = (size_t)nf * nf * nf, Nc = (size_t)nc * nc * nc;
+    const int r = blockIdx.y;
+    const uint32_t xe = Lf.part(8 * j, 0), xo = Lf.part(8 * j + 1, 0), xp = Lf.part(wrap_dec(8 * j, nf), 0);
+    const uint32_t ys[3] = {Lf.part(wrap_dec(2 * Y, nf), 1), Lf.part(2 * Y, 1), Lf.part(2 * Y + 1, 1)};
+    const uint32_t zs[3] = {Lf.part(wrap_dec(2 * Z, nf), 2), Lf.part(2 * Z, 2), Lf.part(2 * Z + 1, 2)};
+    const float wt[3] = {0.5f, 1.f, 0.5f};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float* pl = rf + (size_t)(r * 3 + c) * Nf;
+        float a[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int oz = 0; oz < 3; oz++)
+#pragma unroll
+            for (int oy = 0; oy < 3; oy++) {
+                const uint32_t base = ys[oy] + zs[oz];
+                const float4 e = *reinterpret_cast<const float4*>(pl + base + xe);
+                const float4 o = *reinterpret_cast<const float4*>(pl + base + xo);
+                const float op = pl[base + xp];
+                const float w = wt[oy] * wt[oz];
+                a[0] = fmaf(w, e.x + 0.5f * (o.x + op), a[0]);
+                a[1] = fmaf(w, e.y + 0.5f * (o.y + o.x), a[1]);
+                a[2] = fmaf(w, e.z + 0.5f * (o.z + o.y), a[2]);
+                a[3] = fmaf(w, e.w + 0.5f * (o.w + o.z), a[3]);
+            }
+        float* out = fc + (size_t)(r * 3 + c) * Nc;
+        if (Lc.lex) {
+            *reinterpret_cast<float4*>(out + Lc.idx(4 * j, Y, Z)) = make_float4(a[0], a[1], a[2], a[3]);
+        } else {
+            const uint32_t yz = Lc.part(Y, 1) + Lc.part(Z, 2);
+            *reinterpret_cast<float2*>(out + yz + Lc.part(4 * j, 0)) = make_float2(a[0], a[2]);
+            *reinterpret_cast<float2*>(out + yz + Lc.part(4 * j + 1, 0)) = make_float2(a[1], a[3]);
+        }
+    }
+}
+
+// Prolongation: the 8 waves of a workgroup take the 8 fine parity blocks; a lane adds P u_c to four
+// consecutive local x (fine x = 2lx + px) of one row.  Parents along x: lx (weight 1) when px = 0;
+// lx and lx + 1 (1/2 each) when px = 1.
+__global__ __launch_bounds__(kFullThreads) void k_prolong4(Lay Lf, Lay Lc, const float* __restrict__ uc,
+                                                           float* __restrict__ uf, int pin) {
+    const uint32_t hf = (uint32_t)Lf.h, q = hf / 4;
+    const uint32_t t4 = xcd_chunk(blockIdx.x, gridDim.x) * 64u + (threadIdx.x & 63u);
+    if (t4 >= Lf.h3 / 4) return;
+    const uint32_t b = threadIdx.x >> 6;
+    const int px = b & 1, py = (b >> 1) & 1, pz = (int)(b >> 2);
+    const uint32_t j = t4 % q, row = t4 / q;
+    const int ly = (int)(row % hf), lz = (int)(row / hf);
+    const int nc = Lc.n;
+    const int Ys[2] = {ly, ly + 1 == nc ? 0 : ly + 1}, Zs[2] = {lz, lz + 1 == nc ? 0 : lz + 1};
+    const float wy[2] = {py ? 0.5f : 1.f, 0.5f}, wz[2] = {pz ? 0.5f : 1.f, 0.5f};
+    const int X4 = (int)(4 * j + 4) == nc ? 0 : (int)(4 * j + 4);
+    const size_t Nf = (size_t)Lf.n * Lf.n * Lf.n, Nc = (size_t)nc * nc * nc;
+    const int r = blockIdx.y;
+    const bool skip0 = pin && b == 0 && row == 0 && j == 0;  // pinned origin
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float* cp = uc + (size_t)(r * 3 + c) * Nc;
+        float a[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int iz = 0; iz <= pz; iz++)
+            for (int iy = 0; iy <= py; iy++) {
+                float cv[5];
+                if (Lc.lex) {
+                    const uint32_t base = (uint32_t)nc * (Ys[iy] + (uint32_t)nc * Zs[iz]);
+                    const float4 v4 = *reinterpret_cast<const float4*>(cp + base + 4 * j);
+                    cv[0] = v4.x; cv[1] = v4.y; cv[2] = v4.z; cv[3] = v4.w;
+                    cv[4] = px ? cp[base + X4] : 0.f;
+                } else {
+                    const uint32_t yz = Lc.part(Ys[iy], 1) + Lc.part(Zs[iz], 2);
+                    const float2 ev = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j, 0));
+                    const float2 od = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j + 1, 0));
+                    cv[0] = ev.x; cv[1] = od.x; cv[2] = ev.y; cv[3] = od.y;
+                    cv[4] = px ? cp[yz + Lc.part(X4, 0)] : 0.f;
+                }
+                const float w = wy[iy] * wz[iz];
+#pragma unroll
+                for (int k = 0; k < 4; k++) a[k] = fmaf(w, px ? 0.5f * (cv[k] + cv[k + 1]) : cv[k], a[k]);
+            }
+        float4* dst = reinterpret_cast<float4*>(uf + (size_t)(r * 3 + c) * Nf + b * Lf.h3 + row * hf + 4 * j);
+        float4 u4 = *dst;
+        u4.x += skip0 ? 0.f : a[0];
+        u4.y += a[1];
+        u4.z += a[2];
+        u4.w += a[3];
+        *dst = u4;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Galerkin coarse operators (restrict_stencil_otf_aos_kernel_1 grid_ex.cu:936-1061;
 // restrict_stencil_aos_kernel_1 grid_ex.cu:1065-1160; pinned-origin rows grid.cpp:1184-1325)
@@ -842,14 +961,15 @@ __global__ __launch_bounds__(256) void k_dot(size_t M, const float* __restrict__
     if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = s;
 }
 
-__global__ void k_sum_partials(int R, int nb, const double* __restrict__ partial, double* __restrict__ out) {
+// out[r] = sum of partial[r * nb .. r * nb + nb), one workgroup per row r = blockIdx.x
+__global__ __launch_bounds__(256) void k_sum_partials(int nb, const double* __restrict__ partial,
+                                                      double* __restrict__ out) {
     __shared__ double sh[4];
-    for (int r = 0; r < R; r++) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[r * nb + i];
-        s = block_sum<256>(s, sh);
-        if (threadIdx.x == 0) out[r] = s;
-    }
+    const int r = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[(size_t)r * nb + i];
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) out[r] = s;
 }
 
 // PCG scalars per load case.  sc layout: [0..R) rz, [R..2R) pq, [2R..3R) alpha, [3R..4R) rz_new,
@@ -869,30 +989,63 @@ __global__ void k_pcg_beta(int R, double* sc) {
     sc[r] = rzn;
 }
 
-// x += alpha p ; r -= alpha q  (per load case alpha)
-__global__ __launch_bounds__(256) void k_pcg_update_xr(int R, size_t M, const double* __restrict__ sc,
-                                                       const float* __restrict__ p, const float* __restrict__ q,
-                                                       float* __restrict__ x, float* __restrict__ r) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    for (int k = 0; k < R; k++) {
-        const float a = (float)sc[2 * R + k];
-        const size_t o = (size_t)k * M + i;
-        x[o] = fmaf(a, p[o], x[o]);
-        r[o] = fmaf(-a, q[o], r[o]);
+// CG vector updates, one grid row per load case k = blockIdx.y, grid-stride over the M values of a
+// load case in float4 steps (M % 4 == 0; the caller checks it).
+//
+// r -= alpha q, with per-block FP64 partial sums of |r|^2 in partial[k * gridDim.x + block].
+__global__ __launch_bounds__(256) void k_pcg_update_r(int R, size_t M, const double* __restrict__ sc,
+                                                      const float* __restrict__ q, float* __restrict__ r,
+                                                      double* __restrict__ partial) {
+    __shared__ double sh[4];
+    const int k = blockIdx.y;
+    const float a = (float)sc[2 * R + k];
+    const float4* q4 = reinterpret_cast<const float4*>(q + (size_t)k * M);
+    float4* r4 = reinterpret_cast<float4*>(r + (size_t)k * M);
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 qv = q4[i];
+        float4 rv = r4[i];
+        rv.x = fmaf(-a, qv.x, rv.x);
+        rv.y = fmaf(-a, qv.y, rv.y);
+        rv.z = fmaf(-a, qv.z, rv.z);
+        rv.w = fmaf(-a, qv.w, rv.w);
+        r4[i] = rv;
+        s += ((double)rv.x * rv.x + (double)rv.y * rv.y) + ((double)rv.z * rv.z + (double)rv.w * rv.w);
     }
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) partial[k * gridDim.x + blockIdx.x] = s;
 }
 
-// p = z + beta p (active load cases only)
-__global__ __launch_bounds__(256) void k_pcg_update_p(int R, size_t M, const double* __restrict__ sc,
-                                                      const float* __restrict__ z, float* __restrict__ p) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    for (int k = 0; k < R; k++) {
-        if (sc[5 * R + k] == 0.0) continue;
-        const float b = (float)sc[4 * R + k];
-        const size_t o = (size_t)k * M + i;
-        p[o] = fmaf(b, p[o], z[o]);
+// x += alpha p (the step of this iteration, all load cases: alpha is zero for finished ones), then,
+// when do_p, p = z + beta p for the active load cases.
+__global__ __launch_bounds__(256) void k_pcg_update_px(int R, size_t M, const double* __restrict__ sc,
+                                                       const float* __restrict__ z, float* __restrict__ p,
+                                                       float* __restrict__ x, int do_p) {
+    const int k = blockIdx.y;
+    const float a = (float)sc[2 * R + k];
+    const float b = (float)sc[4 * R + k];
+    const bool upd = do_p && sc[5 * R + k] != 0.0;
+    const float4* z4 = reinterpret_cast<const float4*>(z + (size_t)k * M);
+    float4* p4 = reinterpret_cast<float4*>(p + (size_t)k * M);
+    float4* x4 = reinterpret_cast<float4*>(x + (size_t)k * M);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
+        float4 pv = p4[i];
+        if (a != 0.f) {
+            float4 xv = x4[i];
+            xv.x = fmaf(a, pv.x, xv.x);
+            xv.y = fmaf(a, pv.y, xv.y);
+            xv.z = fmaf(a, pv.z, xv.z);
+            xv.w = fmaf(a, pv.w, xv.w);
+            x4[i] = xv;
+        }
+        if (upd) {
+            const float4 zv = z4[i];
+            pv.x = fmaf(b, pv.x, zv.x);
+            pv.y = fmaf(b, pv.y, zv.y);
+            pv.z = fmaf(b, pv.z, zv.z);
+            pv.w = fmaf(b, pv.w, zv.w);
+            p4[i] = pv;
+        }
     }
 }
 
@@ -1158,9 +1311,24 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, co
     const Lay L = make_lay(n, 0);
     const dim3 g(full_grid(L)), b(kFullThreads);
     if (f)
-        hipLaunchKernelGGL((k_fine_apply<6, 1>), g, b, 0, s, L, N, ke, w, u, f, out);
+        hipLaunchKernelGGL((k_fine_apply<6, 1, false>), g, b, 0, s, L, N, ke, w, u, f, out, nullptr);
     else
-        hipLaunchKernelGGL((k_fine_apply<6, 0>), g, b, 0, s, L, N, ke, w, u, f, out);
+        hipLaunchKernelGGL((k_fine_apply<6, 0, false>), g, b, 0, s, L, N, ke, w, u, f, out, nullptr);
+}
+
+void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const float* u, float* out,
+                           double* partial, int dot_blocks, double* out6) {
+    const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
+    const unsigned nb = full_grid(L);
+    if (L.h3 % 64) {  // partial waves: separate dot
+        launch_fine_apply(s, n, ke, w, u, nullptr, out);
+        launch_dot6(s, 3 * N, u, out, partial, dot_blocks, out6);
+        return;
+    }
+    hipLaunchKernelGGL((k_fine_apply<6, 0, true>), dim3(nb), dim3(kFullThreads), 0, s, L, N, ke, w, u, nullptr, out,
+                       partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)nb, partial, out6);
 }
 
 template <bool REV, int ZC>
@@ -1215,7 +1383,7 @@ void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double
         else
             residual64_case<false, true>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
     }
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)full_grid(L), partial, out6);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)full_grid(L), partial, out6);
 }
 
 void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
@@ -1224,7 +1392,7 @@ void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc
     const Lay L = make_lay(n, 0);
     const unsigned nb = full_grid(L);
     hipLaunchKernelGGL(k_rhs_norm64, dim3(nb), dim3(kFullThreads), 0, s, L, N, kchi, xc, emin, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, (int)nb, partial, out6);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)nb, partial, out6);
 }
 
 void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u) {
@@ -1312,14 +1480,19 @@ void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int revers
 // this costs no extra traffic and shortens each thread's chain of dependent loads.
 void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc) {
     const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
-    hipLaunchKernelGGL((k_restrict<1>), dim3(nblk(Nc), 6), dim3(256), 0, s, make_lay(nf, 0), make_lay(nf / 2, lexc),
-                       rf, fc);
+    const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
+    if (nf % 8 == 0)
+        hipLaunchKernelGGL(k_restrict4, dim3(nblk(Nc / 4), 6), dim3(256), 0, s, Lf, Lc, rf, fc);
+    else
+        hipLaunchKernelGGL((k_restrict<1>), dim3(nblk(Nc), 6), dim3(256), 0, s, Lf, Lc, rf, fc);
 }
 
 void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, float* uf, int pin) {
-    const Lay Lf = make_lay(nf, 0);
-    hipLaunchKernelGGL((k_prolong<1>), dim3(full_grid(Lf), 6), dim3(kFullThreads), 0, s, Lf, make_lay(nf / 2, lexc),
-                       uc, uf, pin);
+    const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
+    if (nf % 8 == 0)
+        hipLaunchKernelGGL(k_prolong4, dim3(nblk(Lf.h3 / 4, 64), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc, uf, pin);
+    else
+        hipLaunchKernelGGL((k_prolong<1>), dim3(full_grid(Lf), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc, uf, pin);
 }
 
 void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
@@ -1345,23 +1518,25 @@ void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float*
 void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks,
                  double* out) {
     hipLaunchKernelGGL(k_dot, dim3(nblocks, 6), dim3(256), 0, s, M, a, b, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 6, nblocks, partial, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, nblocks, partial, out);
 }
 
 void launch_sum_partials(hipStream_t s, int R, int nblocks, const double* partial, double* out) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, R, nblocks, partial, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(R), dim3(256), 0, s, nblocks, partial, out);
 }
 
 void launch_pcg_alpha(hipStream_t s, double* sc) { hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(64), 0, s, 6, sc); }
 void launch_pcg_beta(hipStream_t s, double* sc) { hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(64), 0, s, 6, sc); }
 
-void launch_pcg_update_xr(hipStream_t s, size_t M, const double* sc, const float* p, const float* q, float* x,
-                          float* r) {
-    hipLaunchKernelGGL(k_pcg_update_xr, dim3(nblk(M)), dim3(256), 0, s, 6, M, sc, p, q, x, r);
+void launch_pcg_update_r(hipStream_t s, size_t M, const double* sc, const float* q, float* r, double* partial,
+                         int nblocks, double* out6) {
+    hipLaunchKernelGGL(k_pcg_update_r, dim3(nblocks, 6), dim3(256), 0, s, 6, M, sc, q, r, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, nblocks, partial, out6);
 }
 
-void launch_pcg_update_p(hipStream_t s, size_t M, const double* sc, const float* z, float* p) {
-    hipLaunchKernelGGL(k_pcg_update_p, dim3(nblk(M)), dim3(256), 0, s, 6, M, sc, z, p);
+void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const float* z, float* p, float* x, int do_p,
+                          int nblocks) {
+    hipLaunchKernelGGL(k_pcg_update_px, dim3(nblocks, 6), dim3(256), 0, s, 6, M, sc, z, p, x, do_p);
 }
 
 void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,
@@ -1371,7 +1546,7 @@ void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w
     const unsigned nchunks = full_grid(L);
     const int nb = (int)std::min<unsigned>((unsigned)nblocks, nchunks);
     hipLaunchKernelGGL(k_elastic_matrix, dim3(nb), dim3(kFullThreads), 0, s, L, N, nchunks, ke, w, chi, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 21, nb, partial, out21);
+    hipLaunchKernelGGL(k_sum_partials, dim3(21), dim3(256), 0, s, nb, partial, out21);
 }
 
 void launch_sensitivity(hipStream_t s, int n, int pitch, const float* ke, const float* chi, const float* wdc,
@@ -1397,7 +1572,7 @@ void launch_oc_trial(hipStream_t s, int n, int pitch, double g, double min_rho, 
                      const float* sens, const float* rho, float* out, double* partial, int nblocks, double* sum) {
     hipLaunchKernelGGL(k_oc_trial, dim3(nblocks), dim3(256), 0, s, n, pitch, g, min_rho, step, damp, sens, rho, out,
                        partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 1, nblocks, partial, sum);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nblocks, partial, sum);
 }
 
 void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* partial, int nblocks) {
@@ -1661,7 +1836,7 @@ void launch_mma_setup(hipStream_t s, int m, long n, int iter, double move, const
                       double* P, double* Q, double* part, int nb, double* out) {
     hipLaunchKernelGGL(k_mma_setup, dim3(nb), dim3(256), 0, s, m, n, iter, move, x, xmin, xmax, xold1, xold2, df0, dfdx,
                        low, upp, alfa, beta, p0, q0, P, Q, part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, m, nb, part, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(m), dim3(256), 0, s, nb, part, out);
 }
 
 void launch_mma_resid(hipStream_t s, int m, long n, const double* lam, double epsi, const double* x,
@@ -1673,7 +1848,7 @@ void launch_mma_resid(hipStream_t s, int m, long n, const double* lam, double ep
     d.epsi = epsi;
     hipLaunchKernelGGL(k_mma_resid, dim3(nb), dim3(256), 0, s, m, n, d, x, xsi, eta, low, upp, alfa, beta, p0, q0, P,
                        Q, part, partmax);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, 1 + m, nb, part, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1 + m), dim3(256), 0, s, nb, part, out);
     hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, s, nb, partmax, out + 1 + m);
 }
 
@@ -1687,7 +1862,7 @@ void launch_mma_newton(hipStream_t s, int m, long n, const double* lam, double e
     const int NS = kMmaMaxM + kMmaMaxM * (kMmaMaxM + 1) / 2;
     hipLaunchKernelGGL(k_mma_newton, dim3(nb), dim3(256), 0, s, m, n, d, x, xsi, eta, low, upp, alfa, beta, p0, q0, P,
                        Q, delx, diagx, part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, NS, nb, part, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(NS), dim3(256), 0, s, nb, part, out);
 }
 
 void launch_mma_dir(hipStream_t s, int m, long n, double epsi, const double* dlam, const double* x, const double* xsi,
