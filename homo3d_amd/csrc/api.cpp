@@ -300,46 +300,51 @@ void prof_end(homo3d_handle* h, ProfEvent& ev) {
 // ---------------------------------------------------------------------------------------------
 // One 8-colour sweep.  zero_start: u is taken as zero on entry whatever it holds (forward sweeps
 // only); every colour pass then writes its block without reading later colours.
-void op_smooth(homo3d_handle* h, int l, float* u, const float* f, float omega, int reverse, bool zero_start = false) {
+// u_half: the level-0 u is packed FP16 (coarse levels are always FP32).
+void op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, int reverse, bool zero_start = false,
+               int u_half = 0) {
     const Level& L = h->lv[l];
-    if (l == 0 && !zero_start) launch_zero_pin(h->stream, L.N, u);
+    if (l == 0 && !zero_start) launch_zero_pin(h->stream, L.N, u, u_half);
     const int zs = zero_start && !reverse ? 1 : 0;
     for (int cc = 0; cc < 8; cc++) {
         const int c = reverse ? 7 - cc : cc;
         if (l == 0) {
             ProfEvent ev;
             prof_begin(h, 0, ev);
-            launch_fine_gs(h->stream, L.n, c, omega, reverse, zs, h->d_ke, h->d_w, u, f);
+            launch_fine_gs(h->stream, L.n, c, omega, reverse, zs, h->d_ke, h->d_w, u, u_half, f);
             prof_end(h, ev);
         } else {
-            launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, u, f);
+            launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, static_cast<float*>(u), f);
         }
     }
 }
 
-void op_residual(homo3d_handle* h, int l, const float* u, const float* f, float* r) {
+void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* r, int u_half = 0) {
     const Level& L = h->lv[l];
     if (l == 0) {
         ProfEvent ev;
         prof_begin(h, 2, ev);
-        launch_fine_apply(h->stream, L.n, h->d_ke, h->d_w, u, f, r);
+        launch_fine_apply(h->stream, L.n, h->d_ke, h->d_w, u, u_half, f, r);
         prof_end(h, ev);
     } else {
-        launch_stencil_apply(h->stream, L.n, L.lex, L.st, u, f, r);
+        launch_stencil_apply(h->stream, L.n, L.lex, L.st, static_cast<const float*>(u), f, r);
     }
 }
 
 // One V(1,1) cycle on the level-0 pair (u0, f0) (MG::v_cycle, MG.cpp:46-70).  When rn2 is given it
 // receives, per load case, the squared norm of the level-0 residual after pre-smoothing.  u0_zero:
 // the cycle starts from u0 = 0 (u0's contents are ignored), as the coarse levels always do.
-void vcycle(homo3d_handle* h, float* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false) {
+// u0_half: u0 is packed FP16 (level 0 only; every other field is FP32).
+void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
+            int u0_half = 0) {
     const float omega = (float)h->cfg.sor_omega;
     for (int l = 0; l < h->nl - 1; l++) {
         Level& L = h->lv[l];
-        float* u = l == 0 ? u0 : L.u;
+        void* u = l == 0 ? u0 : L.u;
         const float* f = l == 0 ? f0 : L.f;
-        op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero);
-        op_residual(h, l, u, f, L.r);
+        const int uh = l == 0 ? u0_half : 0;
+        op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero, uh);
+        op_residual(h, l, u, f, L.r, uh);
         if (l == 0 && rn2) launch_dot6(h->stream, 3 * L.N, L.r, L.r, h->d_partial, kDotBlocks, rn2);
         launch_restrict(h->stream, L.n, h->lv[l + 1].lex, L.r, h->lv[l + 1].f);
     }
@@ -347,10 +352,11 @@ void vcycle(homo3d_handle* h, float* u0, const float* f0, int symmetric, double*
     launch_coarse_solve(h->stream, Lc.n, h->d_Ainv, Lc.f, Lc.u);
     for (int l = h->nl - 2; l >= 0; l--) {
         Level& L = h->lv[l];
-        float* u = l == 0 ? u0 : L.u;
+        void* u = l == 0 ? u0 : L.u;
         const float* f = l == 0 ? f0 : L.f;
-        launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, u, l == 0 ? 1 : 0);
-        op_smooth(h, l, u, f, omega, symmetric ? 1 : 0);
+        const int uh = l == 0 ? u0_half : 0;
+        launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, u, uh, l == 0 ? 1 : 0);
+        op_smooth(h, l, u, f, omega, symmetric ? 1 : 0, false, uh);
     }
 }
 
@@ -402,9 +408,11 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     if (M % 4) return set_err(h, HOMO3D_NO_SUPPORT, "CG vectors need 4 | 3n^3");
     HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                              h->stream));
-    vcycle(h, h->Z, h->Rr, 1, nullptr, true);
-    HIPCHK(h, hipMemcpyAsync(h->P, h->Z, vb, hipMemcpyDeviceToDevice, h->stream));
-    launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc);  // rz
+    // the preconditioner's level-0 iterate z in packed FP16 when enabled and the layout allows it
+    const int zh = (h->cfg.precond_half && L0.n % 8 == 0) ? 1 : 0;
+    vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh);
+    launch_set_p(h->stream, M, h->Z, zh, h->P, kDotBlocks);
+    launch_dot_rz(h->stream, M, h->Rr, h->Z, zh, h->d_partial, kDotBlocks, h->d_sc);  // rz
     // One CG step: q = K p with p.q; x is advanced by alpha p one step late, fused with the update of
     // p (and by a last x-only pass on exit), so that p, x, r and q are each streamed once per step.
     double rr[kRhs];
@@ -438,13 +446,13 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         if (changed)
             HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                                      h->stream));
-        vcycle(h, h->Z, h->Rr, 1, nullptr, true);
-        launch_dot6(h->stream, M, h->Rr, h->Z, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
+        vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh);
+        launch_dot_rz(h->stream, M, h->Rr, h->Z, zh, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
         launch_pcg_beta(h->stream, h->d_sc);
-        launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, h->P, h->X, 1, kDotBlocks);
+        launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, zh, h->P, h->X, 1, kDotBlocks);
         x_pending = false;
     }
-    if (x_pending) launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, h->P, h->X, 0, kDotBlocks);
+    if (x_pending) launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, zh, h->P, h->X, 0, kDotBlocks);
     return HOMO3D_OK;
 }
 
@@ -488,6 +496,7 @@ void homo3d_default_config(homo3d_config* c) {
     c->device = 0;
     c->sor_omega = 1.0;
     c->diag_clamp = 1e-4;
+    c->precond_half = 1;
 }
 
 int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
@@ -920,7 +929,7 @@ int homo3d_residual(homo3d_handle* h, int level) {
     if (!h || level < 0 || level >= h->nl) return HOMO3D_BAD_ARG;
     int rc = require_density(h);
     if (rc) return rc;
-    if (level == 0) launch_zero_pin(h->stream, h->lv[0].N, h->X);
+    if (level == 0) launch_zero_pin(h->stream, h->lv[0].N, h->X, 0);
     op_residual(h, level, field_ptr(h, level, 0), field_ptr(h, level, 1), h->lv[level].r);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
@@ -931,10 +940,10 @@ int homo3d_apply_K(homo3d_handle* h) {
     if (!h) return HOMO3D_BAD_ARG;
     int rc = require_density(h);
     if (rc) return rc;
-    launch_zero_pin(h->stream, h->lv[0].N, h->X);
+    launch_zero_pin(h->stream, h->lv[0].N, h->X, 0);
     ProfEvent ev;
     prof_begin(h, 1, ev);
-    launch_fine_apply(h->stream, h->lv[0].n, h->d_ke, h->d_w, h->X, nullptr, h->lv[0].r);
+    launch_fine_apply(h->stream, h->lv[0].n, h->d_ke, h->d_w, h->X, 0, nullptr, h->lv[0].r);
     prof_end(h, ev);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
@@ -962,7 +971,7 @@ int homo3d_restrict(homo3d_handle* h, int level) {
 int homo3d_prolong(homo3d_handle* h, int level) {
     if (!h || level < 0 || level >= h->nl - 1) return HOMO3D_BAD_ARG;
     if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
-    launch_prolong(h->stream, h->lv[level].n, h->lv[level + 1].lex, field_ptr(h, level + 1, 0), field_ptr(h, level, 0),
+    launch_prolong(h->stream, h->lv[level].n, h->lv[level + 1].lex, field_ptr(h, level + 1, 0), field_ptr(h, level, 0), 0,
                    level == 0 ? 1 : 0);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;

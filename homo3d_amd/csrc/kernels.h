@@ -5,15 +5,18 @@
 
 namespace homo3d {
 
-void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const float* u, const float* f,
-                       float* out);
+// Fine-level kernels take u as FP32 planes ([6][3][n^3]) or, with u_half, as packed FP16 load-case
+// pairs ([3][3][n^3] words of two halves, cases 2p and 2p + 1; only with n % 8 == 0).
+// out = K u (f == nullptr) or f - K u
+void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
+                       const float* f, float* out);
 // out = K u with the per-load-case FP64 dot u . out -> out6.  partial holds 6 * n^3 / 512 doubles (and
 // at least 6 * dot_blocks).
 void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const float* u, float* out,
                            double* partial, int dot_blocks, double* out6);
 // zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
-                    const float* w, float* u, const float* f);
+                    const float* w, void* u, int u_half, const float* f);
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f);
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
                             const double* u, float* r32, double* partial, double* out6, int do_f, int do_k);
@@ -22,14 +25,14 @@ void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc
 void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u);
 void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x);
 void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u);
-void launch_zero_pin(hipStream_t s, size_t N, float* u);
+void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half);
 void launch_zero_pin64(hipStream_t s, size_t N, double* u);
 void launch_stencil_apply(hipStream_t s, int n, int lex, const float* st, const float* u, const float* f,
                           float* out);
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* st,
                        float* u, const float* f);
 void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc);
-void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, float* uf, int pin);
+void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, void* uf, int u_half, int pin);
 void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
                           float clamp);
 void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const float* stf, float* stc, float clamp);
@@ -42,8 +45,13 @@ void launch_pcg_beta(hipStream_t s, double* sc);
 // x += alpha p and, when do_p, p = z + beta p for the active load cases.
 void launch_pcg_update_r(hipStream_t s, size_t M, const double* sc, const float* q, float* r, double* partial,
                          int nblocks, double* out6);
-void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const float* z, float* p, float* x, int do_p,
-                          int nblocks);
+void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const void* z, int z_half, float* p, float* x,
+                          int do_p, int nblocks);
+// p = z (z FP32 or packed FP16)
+void launch_set_p(hipStream_t s, size_t M, const void* z, int z_half, float* p, int nblocks);
+// per-load-case r . z -> out6 (z FP32 or packed FP16)
+void launch_dot_rz(hipStream_t s, size_t M, const float* r, const void* z, int z_half, double* partial, int nblocks,
+                   double* out6);
 void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,
                            int nblocks, double* out21);
 void launch_sensitivity(hipStream_t s, int n, int pitch, const float* ke, const float* chi, const float* wdc,

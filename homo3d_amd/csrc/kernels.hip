@@ -95,6 +95,24 @@ __device__ __forceinline__ double bload<double>(Rsrc r, uint32_t voff, uint32_t 
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
+// Packed FP16 storage of a 6-load-case field: word plane (p * 3 + i) holds components i of load cases
+// 2p (.x) and 2p + 1 (.y), so a neighbour of all six load cases is 9 32-bit words.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+template <>
+__device__ __forceinline__ half2_t bload<half2_t>(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(half2_t, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+// load cases per stored word of a field type
+template <typename A>
+struct CasesPerWord {
+    static constexpr int value = 1;
+};
+template <>
+struct CasesPerWord<half2_t> {
+    static constexpr int value = 2;
+};
+
 // 3x3 block of the assembled fine operator between vertex v and its neighbour v+d:
 // S = sum over the cells containing both of w_e * KE[node(v)][node(v+d)].
 // Cell e (0..7) sits at offset (e&1, e>>1&1, e>>2) - 1 from v; v is its node 7-e.
@@ -193,36 +211,57 @@ constexpr int next_nb(uint32_t mask, int d) {
 // zero-start colour of a GS pass (-1: ordinary pass) -> gathered neighbours
 constexpr uint32_t gs_mask(int zc) { return zc < 0 ? kRowOffDiag : zero_start_mask(zc); }
 
-template <int R, typename A>
-__device__ __forceinline__ void load_nb(A (&dst)[R][3], Rsrc ru, uint32_t qo, uint32_t plane, int r0) {
+// the stored words of one neighbour: W = R / (load cases per word) words per component; r0 counts
+// load cases (a multiple of the cases per word)
+template <int W, typename A>
+__device__ __forceinline__ void load_nb(A (&dst)[W][3], Rsrc ru, uint32_t qo, uint32_t plane, int r0) {
+    constexpr int cpw = CasesPerWord<A>::value;
 #pragma unroll
-    for (int r = 0; r < R; r++)
+    for (int p = 0; p < W; p++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) dst[r][i] = bload<A>(ru, qo, (uint32_t)((r0 + r) * 3 + i) * plane);
+        for (int i = 0; i < 3; i++) dst[p][i] = bload<A>(ru, qo, (uint32_t)((r0 / cpw + p) * 3 + i) * plane);
+}
+
+// acc[r] += S u_r for the load cases of one stored word triple
+template <int W, typename T, typename A>
+__device__ __forceinline__ void block_fma(const T (&S)[9], const A (&cur)[W][3], T (&acc)[W * CasesPerWord<A>::value][3]) {
+#pragma unroll
+    for (int p = 0; p < W; p++) {
+        if constexpr (CasesPerWord<A>::value == 2) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const float u0 = (float)cur[p][0][h], u1 = (float)cur[p][1][h], u2 = (float)cur[p][2][h];
+                T(&a)[3] = acc[2 * p + h];
+                a[0] = fmaf(S[0], u0, fmaf(S[1], u1, fmaf(S[2], u2, a[0])));
+                a[1] = fmaf(S[3], u0, fmaf(S[4], u1, fmaf(S[5], u2, a[1])));
+                a[2] = fmaf(S[6], u0, fmaf(S[7], u1, fmaf(S[8], u2, a[2])));
+            }
+        } else {
+            const T u0 = (T)cur[p][0], u1 = (T)cur[p][1], u2 = (T)cur[p][2];
+            acc[p][0] = fma(S[0], u0, fma(S[1], u1, fma(S[2], u2, acc[p][0])));
+            acc[p][1] = fma(S[3], u0, fma(S[4], u1, fma(S[5], u2, acc[p][1])));
+            acc[p][2] = fma(S[6], u0, fma(S[7], u1, fma(S[8], u2, acc[p][2])));
+        }
+    }
 }
 
 // One neighbour of the row: issue the loads of the next neighbour, build the 3x3 block of this one
 // from the cell weights and KE (in LDS), accumulate, recurse.
 template <int R, uint32_t MASK, int D, typename T, typename A>
 __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke, Rsrc ru, uint32_t plane, int r0,
-                                            const A (&cur)[R][3], T (&acc)[R][3]) {
+                                            const A (&cur)[R / CasesPerWord<A>::value][3], T (&acc)[R][3]) {
+    constexpr int W = R / CasesPerWord<A>::value;
     constexpr int DN = next_nb(MASK, D);
     // keep each neighbour's block build next to its use (bounds register pressure)
     __builtin_amdgcn_sched_barrier(0);
-    A nxt[R][3];
-    if constexpr (DN < 27) load_nb<R, A>(nxt, ru, c.nb(DN) * (uint32_t)sizeof(A), plane, r0);
+    A nxt[W][3];
+    if constexpr (DN < 27) load_nb<W, A>(nxt, ru, c.nb(DN) * (uint32_t)sizeof(A), plane, r0);
     T S[9];
     if constexpr (sizeof(T) == 8)
         fine_block<D>(c.wd, ke, S);
     else
         fine_block<D>(c.wc, ke, S);
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const T u0 = (T)cur[r][0], u1 = (T)cur[r][1], u2 = (T)cur[r][2];
-        acc[r][0] = fma(S[0], u0, fma(S[1], u1, fma(S[2], u2, acc[r][0])));
-        acc[r][1] = fma(S[3], u0, fma(S[4], u1, fma(S[5], u2, acc[r][1])));
-        acc[r][2] = fma(S[6], u0, fma(S[7], u1, fma(S[8], u2, acc[r][2])));
-    }
+    block_fma<W, T, A>(S, cur, acc);
     if constexpr (DN < 27) gather_step<R, MASK, DN, T, A>(c, n, ke, ru, plane, r0, nxt, acc);
 }
 
@@ -232,7 +271,8 @@ __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke
 template <int R, uint32_t MASK, typename T, typename A>
 __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, const T* ke, const A* __restrict__ u,
                                             T (&acc)[R][3], int r0 = 0) {
-    const Rsrc ru = make_rsrc(u, (size_t)18 * N * sizeof(A));
+    constexpr int W = R / CasesPerWord<A>::value;
+    const Rsrc ru = make_rsrc(u, (size_t)(18 / CasesPerWord<A>::value) * N * sizeof(A));
     const uint32_t plane = (uint32_t)(N * sizeof(A));
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -240,8 +280,8 @@ __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, c
         for (int i = 0; i < 3; i++) acc[r][i] = T(0);
     constexpr int D0 = next_nb(MASK, -1);
     if constexpr (D0 < 27) {
-        A first[R][3];
-        load_nb<R, A>(first, ru, c.nb(D0) * (uint32_t)sizeof(A), plane, r0);
+        A first[W][3];
+        load_nb<W, A>(first, ru, c.nb(D0) * (uint32_t)sizeof(A), plane, r0);
         gather_step<R, MASK, D0, T, A>(c, n, ke, ru, plane, r0, first, acc);
     }
 }
@@ -283,12 +323,14 @@ __device__ __forceinline__ uint32_t colour_block(int colour) { return (uint32_t)
 // DOT (MODE 0 only): also the per-workgroup FP64 partial sums of u . out per load case (the p . Kp of
 // a CG step) in partial[r * gridDim.x + block].  The launcher uses it only when every lane of the
 // grid holds a vertex (64 | (n/2)^3), so no thread leaves before the workgroup reduction.
-template <int R, int MODE, bool DOT>
+// A: storage of u (float, or half2_t pairs of load cases; DOT needs float).
+template <int R, int MODE, bool DOT, typename A>
 __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
                                                                 const float* __restrict__ w,
-                                                                const float* __restrict__ u,
+                                                                const A* __restrict__ u,
                                                                 const float* __restrict__ f, float* __restrict__ out,
                                                                 double* __restrict__ partial) {
+    static_assert(!DOT || CasesPerWord<A>::value == 1, "the fused dot reads u as FP32");
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
     uint32_t v;
@@ -298,7 +340,7 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
     FineCtx c;
     fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, kRowAll, float, float>(c, L.n, N, ke_s, u, acc);
+    fine_gather<R, kRowAll, float, A>(c, L.n, N, ke_s, u, acc);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -328,20 +370,23 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
 // One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
 // ZC >= 0: colour ZC of a forward sweep that starts from u = 0 (colours above ZC are never read, so
 // u needs no clearing beforehand; the old value of the vertex itself is zero).
-template <int R, bool REVERSE, int ZC>
+// A: storage of u (float, or half2_t pairs of load cases); f is FP32.
+template <int R, bool REVERSE, int ZC, typename A>
 __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour, float omega,
                                                     const float* __restrict__ ke, const float* __restrict__ w,
-                                                    float* __restrict__ u, const float* __restrict__ f) {
+                                                    A* __restrict__ u, const float* __restrict__ f) {
+    constexpr int cpw = CasesPerWord<A>::value;
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
     const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.h3) return;
     const uint32_t v = colour_block(colour) * L.h3 + t;
+    const int r0 = blockIdx.y * R;  // load cases r0 .. r0 + R - 1
     if (v == 0) {
 #pragma unroll
-        for (int r = 0; r < R; r++)
+        for (int p = 0; p < R / cpw; p++)
 #pragma unroll
-            for (int i = 0; i < 3; i++) u[(size_t)(r * 3 + i) * N] = 0.f;
+            for (int i = 0; i < 3; i++) u[(size_t)((r0 / cpw + p) * 3 + i) * N] = A(0);
         return;
     }
     int x, y, z;
@@ -349,20 +394,35 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
     FineCtx c;
     fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, gs_mask(ZC), float, float>(c, L.n, N, ke_s, u, acc);
+    fine_gather<R, gs_mask(ZC), float, A>(c, L.n, N, ke_s, u, acc, r0);
     float Ks[9];
     fine_block<13, float>(c.wc, ke_s, Ks);
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        float uu[3], ff[3];
+    for (int p = 0; p < R / cpw; p++) {
+        float uu[cpw][3], ff[cpw][3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            uu[i] = ZC >= 0 ? 0.f : u[(size_t)(r * 3 + i) * N + v];
-            ff[i] = f[(size_t)(r * 3 + i) * N + v];
-        }
-        block_update<REVERSE>(uu, ff, acc[r], Ks, omega);
+            const size_t o = (size_t)((r0 / cpw + p) * 3 + i) * N + v;
+            if constexpr (cpw == 2) {
+                const half2_t wv = ZC >= 0 ? half2_t{0, 0} : u[o];
+                uu[0][i] = (float)wv.x;
+                uu[1][i] = (float)wv.y;
+            } else {
+                uu[0][i] = ZC >= 0 ? 0.f : u[o];
+            }
 #pragma unroll
-        for (int i = 0; i < 3; i++) u[(size_t)(r * 3 + i) * N + v] = uu[i];
+            for (int h = 0; h < cpw; h++) ff[h][i] = f[(size_t)((r0 + p * cpw + h) * 3 + i) * N + v];
+        }
+#pragma unroll
+        for (int h = 0; h < cpw; h++) block_update<REVERSE>(uu[h], ff[h], acc[p * cpw + h], Ks, omega);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const size_t o = (size_t)((r0 / cpw + p) * 3 + i) * N + v;
+            if constexpr (cpw == 2)
+                u[o] = half2_t{(_Float16)uu[0][i], (_Float16)uu[1][i]};
+            else
+                u[o] = uu[0][i];
+        }
     }
 }
 
@@ -487,10 +547,10 @@ __global__ __launch_bounds__(256) void k_widen_f64(size_t M, const float* __rest
     if (i < M) u[i] = (double)x[i];
 }
 
-// zero the pinned origin's entries of a 6-load-case field
+// zero the pinned origin's entries of a 6-load-case field of `planes` word planes
 template <typename T>
-__global__ void k_zero_pin(size_t N, T* __restrict__ u) {
-    if (threadIdx.x < 18) u[(size_t)threadIdx.x * N] = T(0);
+__global__ void k_zero_pin(size_t N, int planes, T* __restrict__ u) {
+    if ((int)threadIdx.x < planes) u[(size_t)threadIdx.x * N] = T(0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -774,6 +834,67 @@ __global__ __launch_bounds__(kFullThreads) void k_prolong4(Lay Lf, Lay Lc, const
     }
 }
 
+// k_prolong4 for a fine field in packed FP16 (half2_t word planes): a thread takes the load-case pair
+// blockIdx.y (0..2), both halves of each word.
+__global__ __launch_bounds__(kFullThreads) void k_prolong4h(Lay Lf, Lay Lc, const float* __restrict__ uc,
+                                                            half2_t* __restrict__ uf, int pin) {
+    const uint32_t hf = (uint32_t)Lf.h, q = hf / 4;
+    const uint32_t t4 = xcd_chunk(blockIdx.x, gridDim.x) * 64u + (threadIdx.x & 63u);
+    if (t4 >= Lf.h3 / 4) return;
+    const uint32_t b = threadIdx.x >> 6;
+    const int px = b & 1, py = (b >> 1) & 1, pz = (int)(b >> 2);
+    const uint32_t j = t4 % q, row = t4 / q;
+    const int ly = (int)(row % hf), lz = (int)(row / hf);
+    const int nc = Lc.n;
+    const int Ys[2] = {ly, ly + 1 == nc ? 0 : ly + 1}, Zs[2] = {lz, lz + 1 == nc ? 0 : lz + 1};
+    const float wy[2] = {py ? 0.5f : 1.f, 0.5f}, wz[2] = {pz ? 0.5f : 1.f, 0.5f};
+    const int X4 = (int)(4 * j + 4) == nc ? 0 : (int)(4 * j + 4);
+    const size_t Nf = (size_t)Lf.n * Lf.n * Lf.n, Nc = (size_t)nc * nc * nc;
+    const int pr = blockIdx.y;
+    const bool skip0 = pin && b == 0 && row == 0 && j == 0;  // pinned origin
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float a[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float* cp = uc + (size_t)((2 * pr + h) * 3 + c) * Nc;
+            for (int iz = 0; iz <= pz; iz++)
+                for (int iy = 0; iy <= py; iy++) {
+                    float cv[5];
+                    if (Lc.lex) {
+                        const uint32_t base = (uint32_t)nc * (Ys[iy] + (uint32_t)nc * Zs[iz]);
+                        const float4 v4 = *reinterpret_cast<const float4*>(cp + base + 4 * j);
+                        cv[0] = v4.x; cv[1] = v4.y; cv[2] = v4.z; cv[3] = v4.w;
+                        cv[4] = px ? cp[base + X4] : 0.f;
+                    } else {
+                        const uint32_t yz = Lc.part(Ys[iy], 1) + Lc.part(Zs[iz], 2);
+                        const float2 ev = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j, 0));
+                        const float2 od = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j + 1, 0));
+                        cv[0] = ev.x; cv[1] = od.x; cv[2] = ev.y; cv[3] = od.y;
+                        cv[4] = px ? cp[yz + Lc.part(X4, 0)] : 0.f;
+                    }
+                    const float w = wy[iy] * wz[iz];
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        a[h][k] = fmaf(w, px ? 0.5f * (cv[k] + cv[k + 1]) : cv[k], a[h][k]);
+                }
+        }
+        half2_t* dst = uf + (size_t)(pr * 3 + c) * Nf + b * Lf.h3 + row * hf + 4 * j;
+        uint4 raw = *reinterpret_cast<const uint4*>(dst);
+        uint32_t wv[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            half2_t hv = __builtin_bit_cast(half2_t, wv[k]);
+            if (!(skip0 && k == 0)) {
+                hv.x = (_Float16)((float)hv.x + a[0][k]);
+                hv.y = (_Float16)((float)hv.y + a[1][k]);
+            }
+            wv[k] = __builtin_bit_cast(uint32_t, hv);
+        }
+        *reinterpret_cast<uint4*>(dst) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Galerkin coarse operators (restrict_stencil_otf_aos_kernel_1 grid_ex.cu:936-1061;
 // restrict_stencil_aos_kernel_1 grid_ex.cu:1065-1160; pinned-origin rows grid.cpp:1184-1325)
@@ -1049,6 +1170,77 @@ __global__ __launch_bounds__(256) void k_pcg_update_px(int R, size_t M, const do
     }
 }
 
+// The same CG steps with the preconditioned vector z in packed FP16 (half2_t word planes; load-case
+// pair pr = blockIdx.y holds cases 2pr (.x) and 2pr + 1 (.y); M values per load case, M % 4 == 0).
+//
+// partial[(2pr + h) * gridDim.x + block] = block's share of r_{2pr+h} . z_{2pr+h}
+__global__ __launch_bounds__(256) void k_dot_fh(size_t M, const float* __restrict__ r, const half2_t* __restrict__ z,
+                                                double* __restrict__ partial) {
+    __shared__ double sh[4];
+    const int pr = blockIdx.y;
+    const float4* r0 = reinterpret_cast<const float4*>(r + (size_t)(2 * pr) * M);
+    const float4* r1 = reinterpret_cast<const float4*>(r + (size_t)(2 * pr + 1) * M);
+    const uint4* z4 = reinterpret_cast<const uint4*>(z + (size_t)pr * M);
+    double s0 = 0.0, s1 = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 a = r0[i], b = r1[i];
+        const uint4 zw = z4[i];
+        const half2_t h0 = __builtin_bit_cast(half2_t, zw.x), h1 = __builtin_bit_cast(half2_t, zw.y),
+                      h2 = __builtin_bit_cast(half2_t, zw.z), h3 = __builtin_bit_cast(half2_t, zw.w);
+        s0 += ((double)a.x * (double)h0.x + (double)a.y * (double)h1.x) +
+              ((double)a.z * (double)h2.x + (double)a.w * (double)h3.x);
+        s1 += ((double)b.x * (double)h0.y + (double)b.y * (double)h1.y) +
+              ((double)b.z * (double)h2.y + (double)b.w * (double)h3.y);
+    }
+    s0 = block_sum<256>(s0, sh);
+    if (threadIdx.x == 0) partial[(2 * pr) * gridDim.x + blockIdx.x] = s0;
+    s1 = block_sum<256>(s1, sh);
+    if (threadIdx.x == 0) partial[(2 * pr + 1) * gridDim.x + blockIdx.x] = s1;
+}
+
+// k_pcg_update_px with z in packed FP16; set_p: p = z (start of the iteration, x untouched).
+__global__ __launch_bounds__(256) void k_pcg_update_px_h(int R, size_t M, const double* __restrict__ sc,
+                                                         const half2_t* __restrict__ z, float* __restrict__ p,
+                                                         float* __restrict__ x, int do_p, int set_p) {
+    const int pr = blockIdx.y;
+    float a[2], b[2];
+    bool upd[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int k = 2 * pr + h;
+        a[h] = set_p ? 0.f : (float)sc[2 * R + k];
+        b[h] = set_p ? 0.f : (float)sc[4 * R + k];
+        upd[h] = set_p || (do_p && sc[5 * R + k] != 0.0);
+    }
+    const uint4* z4 = reinterpret_cast<const uint4*>(z + (size_t)pr * M);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
+        const uint4 zw = z4[i];
+        const half2_t zh[4] = {__builtin_bit_cast(half2_t, zw.x), __builtin_bit_cast(half2_t, zw.y),
+                               __builtin_bit_cast(half2_t, zw.z), __builtin_bit_cast(half2_t, zw.w)};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            float4* p4 = reinterpret_cast<float4*>(p + (size_t)(2 * pr + h) * M);
+            float4* x4 = reinterpret_cast<float4*>(x + (size_t)(2 * pr + h) * M);
+            float4 pv = set_p ? make_float4(0.f, 0.f, 0.f, 0.f) : p4[i];
+            if (a[h] != 0.f) {
+                float4 xv = x4[i];
+                xv.x = fmaf(a[h], pv.x, xv.x);
+                xv.y = fmaf(a[h], pv.y, xv.y);
+                xv.z = fmaf(a[h], pv.z, xv.z);
+                xv.w = fmaf(a[h], pv.w, xv.w);
+                x4[i] = xv;
+            }
+            if (upd[h]) {
+                pv.x = fmaf(b[h], pv.x, (float)zh[0][h]);
+                pv.y = fmaf(b[h], pv.y, (float)zh[1][h]);
+                pv.z = fmaf(b[h], pv.z, (float)zh[2][h]);
+                pv.w = fmaf(b[h], pv.w, (float)zh[3][h]);
+                p4[i] = pv;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // C^H and sensitivity (elasticMatrix_kernel_opt homogenization.cu:208-346;
 // Sensitivity_kernel_opt_2 homogenization.cu:697-858).  One thread per cell; the six solution
@@ -1305,15 +1497,24 @@ static unsigned full_grid(const Lay& L) {
     return L.lex ? nblk(N, kFullThreads) : nblk(L.h3, 64);
 }
 
-void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const float* u, const float* f,
-                       float* out) {
+void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
+                       const float* f, float* out) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
     const dim3 g(full_grid(L)), b(kFullThreads);
+    if (u_half) {
+        const half2_t* uh = static_cast<const half2_t*>(u);
+        if (f)
+            hipLaunchKernelGGL((k_fine_apply<6, 1, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr);
+        else
+            hipLaunchKernelGGL((k_fine_apply<6, 0, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr);
+        return;
+    }
+    const float* uf = static_cast<const float*>(u);
     if (f)
-        hipLaunchKernelGGL((k_fine_apply<6, 1, false>), g, b, 0, s, L, N, ke, w, u, f, out, nullptr);
+        hipLaunchKernelGGL((k_fine_apply<6, 1, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr);
     else
-        hipLaunchKernelGGL((k_fine_apply<6, 0, false>), g, b, 0, s, L, N, ke, w, u, f, out, nullptr);
+        hipLaunchKernelGGL((k_fine_apply<6, 0, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr);
 }
 
 void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const float* u, float* out,
@@ -1322,25 +1523,25 @@ void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w
     const Lay L = make_lay(n, 0);
     const unsigned nb = full_grid(L);
     if (L.h3 % 64) {  // partial waves: separate dot
-        launch_fine_apply(s, n, ke, w, u, nullptr, out);
+        launch_fine_apply(s, n, ke, w, u, 0, nullptr, out);
         launch_dot6(s, 3 * N, u, out, partial, dot_blocks, out6);
         return;
     }
-    hipLaunchKernelGGL((k_fine_apply<6, 0, true>), dim3(nb), dim3(kFullThreads), 0, s, L, N, ke, w, u, nullptr, out,
+    hipLaunchKernelGGL((k_fine_apply<6, 0, true, float>), dim3(nb), dim3(kFullThreads), 0, s, L, N, ke, w, u, nullptr, out,
                        partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)nb, partial, out6);
 }
 
-template <bool REV, int ZC>
+template <bool REV, int ZC, typename A>
 static void fine_gs_colour(hipStream_t s, const Lay& L, size_t N, int colour, float omega, const float* ke,
-                           const float* w, float* u, const float* f) {
-    hipLaunchKernelGGL((k_fine_gs<6, REV, ZC>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, ke, w, u, f);
+                           const float* w, A* u, const float* f) {
+    hipLaunchKernelGGL((k_fine_gs<6, REV, ZC, A>), dim3(nblk(L.h3)), dim3(256), 0, s, L, N, colour, omega, ke, w, u,
+                       f);
 }
 
-void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
-                    const float* w, float* u, const float* f) {
-    const size_t N = (size_t)n * n * n;
-    const Lay L = make_lay(n, 0);
+template <typename A>
+static void fine_gs_pick(hipStream_t s, const Lay& L, size_t N, int colour, float omega, int reverse, int zero_start,
+                         const float* ke, const float* w, A* u, const float* f) {
     if (reverse) {
         fine_gs_colour<true, -1>(s, L, N, colour, omega, ke, w, u, f);
         return;
@@ -1356,6 +1557,16 @@ void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, 
         case 7: fine_gs_colour<false, 7>(s, L, N, colour, omega, ke, w, u, f); break;
         default: fine_gs_colour<false, -1>(s, L, N, colour, omega, ke, w, u, f); break;
     }
+}
+
+void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
+                    const float* w, void* u, int u_half, const float* f) {
+    const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
+    if (u_half)
+        fine_gs_pick(s, L, N, colour, omega, reverse, zero_start, ke, w, static_cast<half2_t*>(u), f);
+    else
+        fine_gs_pick(s, L, N, colour, omega, reverse, zero_start, ke, w, static_cast<float*>(u), f);
 }
 
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {
@@ -1407,12 +1618,15 @@ void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u) {
     hipLaunchKernelGGL(k_widen_f64, dim3(nblk(M)), dim3(256), 0, s, M, x, u);
 }
 
-void launch_zero_pin(hipStream_t s, size_t N, float* u) {
-    hipLaunchKernelGGL((k_zero_pin<float>), dim3(1), dim3(64), 0, s, N, u);
+void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half) {
+    if (u_half)
+        hipLaunchKernelGGL((k_zero_pin<uint32_t>), dim3(1), dim3(64), 0, s, N, 9, static_cast<uint32_t*>(u));
+    else
+        hipLaunchKernelGGL((k_zero_pin<float>), dim3(1), dim3(64), 0, s, N, 18, static_cast<float*>(u));
 }
 
 void launch_zero_pin64(hipStream_t s, size_t N, double* u) {
-    hipLaunchKernelGGL((k_zero_pin<double>), dim3(1), dim3(64), 0, s, N, u);
+    hipLaunchKernelGGL((k_zero_pin<double>), dim3(1), dim3(64), 0, s, N, 18, u);
 }
 
 // Levels up to this size run the coarse-stencil kernels with one load case per thread (six grid
@@ -1487,12 +1701,17 @@ void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc
         hipLaunchKernelGGL((k_restrict<1>), dim3(nblk(Nc), 6), dim3(256), 0, s, Lf, Lc, rf, fc);
 }
 
-void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, float* uf, int pin) {
+void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, void* uf, int u_half, int pin) {
     const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
-    if (nf % 8 == 0)
-        hipLaunchKernelGGL(k_prolong4, dim3(nblk(Lf.h3 / 4, 64), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc, uf, pin);
+    if (u_half)  // packed FP16 fine fields are only used with nf % 8 == 0
+        hipLaunchKernelGGL(k_prolong4h, dim3(nblk(Lf.h3 / 4, 64), 3), dim3(kFullThreads), 0, s, Lf, Lc, uc,
+                           static_cast<half2_t*>(uf), pin);
+    else if (nf % 8 == 0)
+        hipLaunchKernelGGL(k_prolong4, dim3(nblk(Lf.h3 / 4, 64), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc,
+                           static_cast<float*>(uf), pin);
     else
-        hipLaunchKernelGGL((k_prolong<1>), dim3(full_grid(Lf), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc, uf, pin);
+        hipLaunchKernelGGL((k_prolong<1>), dim3(full_grid(Lf), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc,
+                           static_cast<float*>(uf), pin);
 }
 
 void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
@@ -1534,9 +1753,32 @@ void launch_pcg_update_r(hipStream_t s, size_t M, const double* sc, const float*
     hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, nblocks, partial, out6);
 }
 
-void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const float* z, float* p, float* x, int do_p,
-                          int nblocks) {
-    hipLaunchKernelGGL(k_pcg_update_px, dim3(nblocks, 6), dim3(256), 0, s, 6, M, sc, z, p, x, do_p);
+void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const void* z, int z_half, float* p, float* x,
+                          int do_p, int nblocks) {
+    if (z_half)
+        hipLaunchKernelGGL(k_pcg_update_px_h, dim3(nblocks, 3), dim3(256), 0, s, 6, M, sc,
+                           static_cast<const half2_t*>(z), p, x, do_p, 0);
+    else
+        hipLaunchKernelGGL(k_pcg_update_px, dim3(nblocks, 6), dim3(256), 0, s, 6, M, sc, static_cast<const float*>(z),
+                           p, x, do_p);
+}
+
+void launch_set_p(hipStream_t s, size_t M, const void* z, int z_half, float* p, int nblocks) {
+    if (z_half)
+        hipLaunchKernelGGL(k_pcg_update_px_h, dim3(nblocks, 3), dim3(256), 0, s, 6, M, nullptr,
+                           static_cast<const half2_t*>(z), p, nullptr, 0, 1);
+    else
+        hipMemcpyAsync(p, z, sizeof(float) * 6 * M, hipMemcpyDeviceToDevice, s);
+}
+
+void launch_dot_rz(hipStream_t s, size_t M, const float* r, const void* z, int z_half, double* partial, int nblocks,
+                   double* out6) {
+    if (!z_half) {
+        launch_dot6(s, M, r, static_cast<const float*>(z), partial, nblocks, out6);
+        return;
+    }
+    hipLaunchKernelGGL(k_dot_fh, dim3(nblocks, 3), dim3(256), 0, s, M, r, static_cast<const half2_t*>(z), partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, nblocks, partial, out6);
 }
 
 void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,

@@ -85,6 +85,8 @@ typedef struct {
     int device;             /* HIP device ordinal */
     double sor_omega;       /* smoother relaxation factor (1.0) */
     double diag_clamp;      /* coarse-stencil diagonal floor (1e-4) */
+    int precond_half;       /* HOMO3D_SOLVER_PCG: keep the preconditioner's finest-level iterate in FP16
+                               (accumulation stays FP32; needs n % 8 == 0, else FP32 is used).  Default 1. */
 } homo3d_config;
 
 typedef struct {
