@@ -23,7 +23,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bench_256cube_pcg_traffic.json")
-PROF_KERNEL = {"fine_gs": "k_fine_gs", "fine_apply": "k_fine_apply<6, 0>", "fine_residual": "k_fine_apply<6, 1>"}
+PROF_KINDS = ("fine_gs", "fine_gs_zero", "fine_apply", "fine_residual")
+# rocprofv3 names of the kernel behind each timed kind (demangled, or mangled for the packed-FP16
+# instantiations), for the traffic lookup
+PROF_KERNEL = {"fine_gs": ("k_fine_gs<6, true, -1", "k_fine_gsILi6ELb1ELin1E"),
+               "fine_apply": ("k_fine_apply<6, 0, true", "k_fine_applyILi6ELi0ELb1E"),
+               "fine_residual": ("k_fine_apply<6, 1, false", "k_fine_applyILi6ELi1ELb0E")}
 METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; stencil GB/s"
 
 
@@ -31,25 +36,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, n, R=6, s_u=4, s_f=4, s_w=4):
-    """Algorithmic HBM bytes per launch of the finest-level kernels (FP32 storage)."""
+def kernel_bytes(name, n, R=6, s_z=2, s_f=4, s_w=4):
+    """Algorithmic HBM bytes per launch of a finest-level kernel.  p, q, r, f are FP32 (s_f); the
+    preconditioner iterate z (u of the V-cycle's finest level) is packed FP16 (s_z = 2) when n % 8 == 0."""
     N = n ** 3
-    if name == "fine_apply":      # y = K u: read u, write y, read w
-        return N * (3 * R * (s_u + s_u) + s_w)
-    if name == "fine_residual":   # r = f - K u: read u, f, write r, read w
-        return N * (3 * R * (s_u + s_f + s_u) + s_w)
-    if name == "fine_gs":         # one colour = 1/8 of an in-place sweep: u read+write, f read, w read
-        return N * (3 * R * (2 * s_u + s_f) + s_w) / 8.0
-    raise KeyError(name)
+    if name == "fine_apply":      # q = K p with p.q: read p, write q, read w
+        return N * (3 * R * (s_f + s_f) + s_w)
+    if name == "fine_residual":   # r = f - K z: read z, f, write r, read w
+        return N * (3 * R * (s_z + s_f + s_f) + s_w)
+    if name == "fine_gs":         # one colour = 1/8 of an in-place sweep: z read+write, f read, w read
+        return N * (3 * R * (2 * s_z + s_f) + s_w) / 8.0
+    return None
 
 
 def measured_traffic(name, n):
     """HBM bytes per launch of `name` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
-    if n != 256 or not os.path.exists(TRAFFIC_JSON):
+    if n != 256 or not os.path.exists(TRAFFIC_JSON) or name not in PROF_KERNEL:
         return None
     data = json.load(open(TRAFFIC_JSON))["kernels"]
-    key = PROF_KERNEL[name]
-    vals = [v["traffic_bytes"] for k, v in data.items() if k.startswith(key)]
+    vals = [v["traffic_bytes"] for k, v in data.items() if any(p in k for p in PROF_KERNEL[name])]
     return sum(vals) / len(vals) if vals else None
 
 
@@ -119,19 +124,23 @@ def main():
         log("step %d: iters %s relres %.2e" % (i, st["iters"], max(st["relres"])))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    prof = {k: hom.profile_get(k) for k in ("fine_gs", "fine_apply", "fine_residual")}
+    prof = {k: hom.profile_get(k) for k in PROF_KINDS}
     hom.profile_enable(False)
+    s_z = 2 if n % 8 == 0 else 4
     for k, (cnt, ms) in prof.items():
         if cnt:
-            log("kernel %-14s launches %6d total %9.2f ms avg %8.4f ms  %.1f GB/s algorithmic" %
-                (k, cnt, ms, ms / cnt, kernel_bytes(k, n) / (ms / cnt * 1e-3) / 1e9))
-    dom = max(prof, key=lambda k: prof[k][1])
+            kb = kernel_bytes(k, n, s_z=s_z)
+            log("kernel %-14s launches %6d total %9.2f ms avg %8.4f ms%s" %
+                (k, cnt, ms, ms / cnt, "  %.1f GB/s algorithmic" % (kb / (ms / cnt * 1e-3) / 1e9) if kb else ""))
+    # dominant kernel: the largest total among those with a fixed per-launch byte count
+    dom = max((k for k in prof if kernel_bytes(k, n) is not None), key=lambda k: prof[k][1])
     cnt, ms = prof[dom]
     avg_s = ms / cnt * 1e-3
-    achieved = kernel_bytes(dom, n) / avg_s / 1e9
+    achieved = kernel_bytes(dom, n, s_z=s_z) / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dom, n), "kernel": dom,
-                "avg_launch_ms": round(ms / cnt, 5), "algorithmic_bytes_per_launch": kernel_bytes(dom, n)}
+                "avg_launch_ms": round(ms / cnt, 5),
+                "algorithmic_bytes_per_launch": kernel_bytes(dom, n, s_z=s_z)}
     out = {
         "metric": METRIC,
         "value": round(args.steps / dt, 5),
@@ -143,7 +152,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32",  # FP32 arithmetic, FP64 refinement residuals and dots; z and coarse stencils stored as FP16
         "data": "synthetic",
         "config": {"workload": "%d^3 periodic homogenization: 6 macro-strain load cases, plain MG-PCG "
                                "(V(1,1) 8-colour block GS, FP64 coarsest) to relres %.0e, seeded randc density "

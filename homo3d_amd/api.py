@@ -25,7 +25,7 @@ class _Config(ctypes.Structure):
     _fields_ = [("reso", ctypes.c_int * 3), ("youngs_modulus", ctypes.c_double), ("poisson_ratio", ctypes.c_double),
                 ("emin", ctypes.c_double), ("fem_rel_thres", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("solver", ctypes.c_int), ("device", ctypes.c_int), ("sor_omega", ctypes.c_double),
-                ("diag_clamp", ctypes.c_double), ("precond_half", ctypes.c_int)]
+                ("diag_clamp", ctypes.c_double), ("precond_half", ctypes.c_int), ("coarse_half", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -115,7 +115,8 @@ class HomoConfig:
     (cmdline.cpp:4-27: E = 10, nu = 0.3, relthres = 1e-2, reso 64)."""
 
     def __init__(self, reso=64, youngsModulu=10.0, poissonRatio=0.3, femRelThres=1e-2, max_iter=200,
-                 solver="mg", emin=1e-9, device=0, sor_omega=1.0, diag_clamp=1e-4, precond_half=True, **kw):
+                 solver="mg", emin=1e-9, device=0, sor_omega=1.0, diag_clamp=1e-4, precond_half=True,
+                 coarse_half=True, **kw):
         self.reso = [reso] * 3 if np.isscalar(reso) else list(reso)
         self.youngsModulu = youngsModulu
         self.poissonRatio = poissonRatio
@@ -127,6 +128,7 @@ class HomoConfig:
         self.sor_omega = sor_omega
         self.diag_clamp = diag_clamp
         self.precond_half = precond_half
+        self.coarse_half = coarse_half
         # optimisation-side fields of cfg::HomoConfig (kept for drivers)
         self.volRatio = kw.get("volRatio", 0.3)
         self.filterRadius = kw.get("filterRadius", 2.0)
@@ -150,6 +152,7 @@ class HomoConfig:
         c.sor_omega = self.sor_omega
         c.diag_clamp = self.diag_clamp
         c.precond_half = 1 if self.precond_half else 0
+        c.coarse_half = 1 if self.coarse_half else 0
         return c
 
 

@@ -31,7 +31,8 @@ struct Level {
     float* u = nullptr;
     float* f = nullptr;
     float* r = nullptr;
-    float* st = nullptr;  // levels >= 1: [243][N]
+    void* st = nullptr;   // levels >= 1: FP32 [243][N], or packed FP16 [122][N] words when st_half
+    int st_half = 0;
 };
 
 struct ProfEvent {
@@ -39,8 +40,11 @@ struct ProfEvent {
     int a = -1, b = -1;  // indices into the handle's event pool
 };
 
-const char* const kProfNames[] = {"fine_gs", "fine_apply", "fine_residual"};
-constexpr int kProfKinds = 3;
+// Timed finest-level kernels: full GS colour passes, zero-start GS colour passes, K p (with p.Kp in
+// the CG), and the V-cycle residual f - K u.
+const char* const kProfNames[] = {"fine_gs", "fine_apply", "fine_residual", "fine_gs_zero"};
+constexpr int kProfKinds = 4;
+enum { kProfGs = 0, kProfApply = 1, kProfResidual = 2, kProfGsZero = 3 };
 constexpr size_t kEventPool = 8192;
 
 }  // namespace
@@ -84,8 +88,8 @@ struct homo3d_handle {
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     std::vector<ProfEvent> pending;
-    long prof_count[kProfKinds] = {0, 0, 0};
-    double prof_ms[kProfKinds] = {0, 0, 0};
+    long prof_count[kProfKinds] = {};
+    double prof_ms[kProfKinds] = {};
 };
 
 namespace {
@@ -262,10 +266,10 @@ void symmetry_group(int sym, std::vector<int>& perm, std::vector<int>& flip) {
 
 void prof_flush(homo3d_handle* h) {
     if (h->pending.empty()) return;
-    hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->stream);
     for (auto& ev : h->pending) {
         float ms = 0;
-        hipEventElapsedTime(&ms, h->pool[ev.a], h->pool[ev.b]);
+        if (hipEventElapsedTime(&ms, h->pool[ev.a], h->pool[ev.b]) != hipSuccess) continue;
         h->prof_count[ev.name] += 1;
         h->prof_ms[ev.name] += ms;
     }
@@ -286,12 +290,12 @@ void prof_begin(homo3d_handle* h, int kind, ProfEvent& ev) {
     ev.name = kind;
     ev.a = (int)h->pool_used++;
     ev.b = (int)h->pool_used++;
-    hipEventRecord(h->pool[ev.a], h->stream);
+    (void)hipEventRecord(h->pool[ev.a], h->stream);
 }
 
 void prof_end(homo3d_handle* h, ProfEvent& ev) {
     if (!h->prof || ev.name < 0) return;
-    hipEventRecord(h->pool[ev.b], h->stream);
+    (void)hipEventRecord(h->pool[ev.b], h->stream);
     h->pending.push_back(ev);
 }
 
@@ -310,11 +314,11 @@ void op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
         const int c = reverse ? 7 - cc : cc;
         if (l == 0) {
             ProfEvent ev;
-            prof_begin(h, 0, ev);
+            prof_begin(h, zs ? kProfGsZero : kProfGs, ev);
             launch_fine_gs(h->stream, L.n, c, omega, reverse, zs, h->d_ke, h->d_w, u, u_half, f);
             prof_end(h, ev);
         } else {
-            launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, static_cast<float*>(u), f);
+            launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, L.st_half, static_cast<float*>(u), f);
         }
     }
 }
@@ -323,11 +327,11 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
     const Level& L = h->lv[l];
     if (l == 0) {
         ProfEvent ev;
-        prof_begin(h, 2, ev);
+        prof_begin(h, kProfResidual, ev);
         launch_fine_apply(h->stream, L.n, h->d_ke, h->d_w, u, u_half, f, r);
         prof_end(h, ev);
     } else {
-        launch_stencil_apply(h->stream, L.n, L.lex, L.st, static_cast<const float*>(u), f, r);
+        launch_stencil_apply(h->stream, L.n, L.lex, L.st, L.st_half, static_cast<const float*>(u), f, r);
     }
 }
 
@@ -362,7 +366,7 @@ void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* 
 
 int ensure_stage(homo3d_handle* h, size_t elems) {
     if (h->stage_elems >= elems) return HOMO3D_OK;
-    if (h->d_stage) hipFree(h->d_stage);
+    if (h->d_stage) (void)hipFree(h->d_stage);
     h->d_stage = nullptr;
     HIPCHK(h, hipMalloc(&h->d_stage, sizeof(float) * elems));
     h->stage_elems = elems;
@@ -408,21 +412,24 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     if (M % 4) return set_err(h, HOMO3D_NO_SUPPORT, "CG vectors need 4 | 3n^3");
     HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                              h->stream));
-    // the preconditioner's level-0 iterate z in packed FP16 when enabled and the layout allows it
+    // the preconditioner's level-0 iterate z in packed FP16 when enabled and the layout allows it (p, q, r
+    // and x stay FP32: an FP16 search direction costs more CG steps than it saves)
     const int zh = (h->cfg.precond_half && L0.n % 8 == 0) ? 1 : 0;
     vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh);
-    launch_set_p(h->stream, M, h->Z, zh, h->P, kDotBlocks);
+    HIPCHK(h, launch_set_p(h->stream, M, h->Z, zh, h->P, kDotBlocks));
     launch_dot_rz(h->stream, M, h->Rr, h->Z, zh, h->d_partial, kDotBlocks, h->d_sc);  // rz
-    // One CG step: q = K p with p.q; x is advanced by alpha p one step late, fused with the update of
-    // p (and by a last x-only pass on exit), so that p, x, r and q are each streamed once per step.
+    // One CG step: q = K p with p.q, formed as K z + beta q_old since p = z + beta p_old; x is advanced by
+    // alpha p one step late, fused with the update of p (and by a last x-only pass on exit), so that p,
+    // x, r and q are each streamed once per step.
     double rr[kRhs];
     bool x_pending = false;
     for (int it = 0; it < maxit; it++) {
         {
             ProfEvent ev;
-            prof_begin(h, 1, ev);
-            launch_fine_apply_dot(h->stream, L0.n, h->d_ke, h->d_w, h->P, h->Q, h->d_partial, kDotBlocks,
-                                  h->d_sc + kRhs);  // pq
+            prof_begin(h, kProfApply, ev);
+            // q = K p as K z (+ beta q_old): the operator gathers the packed z instead of the FP32 p
+            launch_fine_apply_dot(h->stream, L0.n, h->d_ke, h->d_w, h->Z, zh, h->P, it ? h->d_sc + 4 * kRhs : nullptr,
+                                  h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
             prof_end(h, ev);
         }
         launch_pcg_alpha(h->stream, h->d_sc);
@@ -497,6 +504,7 @@ void homo3d_default_config(homo3d_config* c) {
     c->sor_omega = 1.0;
     c->diag_clamp = 1e-4;
     c->precond_half = 1;
+    c->coarse_half = 1;
 }
 
 int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
@@ -536,23 +544,24 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
         L.N = (size_t)L.n * L.n * L.n;
         L.lex = l == nl - 1;
         const size_t bytes = sizeof(float) * 3 * kRhs * L.N;
-        if (hipMalloc(&L.u, bytes) || hipMalloc(&L.f, bytes) || hipMalloc(&L.r, bytes)) return fail(HOMO3D_HIP_ERROR);
-        hipMemset(L.u, 0, bytes);
-        hipMemset(L.f, 0, bytes);
-        hipMemset(L.r, 0, bytes);
-        if (l > 0 && hipMalloc(&L.st, sizeof(float) * 243 * L.N)) return fail(HOMO3D_HIP_ERROR);
+        if (hipMalloc(&L.u, bytes) || hipMalloc(&L.f, bytes) || hipMalloc(&L.r, bytes) || hipMemset(L.u, 0, bytes) ||
+            hipMemset(L.f, 0, bytes) || hipMemset(L.r, 0, bytes))
+            return fail(HOMO3D_HIP_ERROR);
+        if (l > 0) {
+            L.st_half = cfg->coarse_half && l < nl - 1 ? 1 : 0;  // the coarsest stays FP32 for the dense solve
+            if (hipMalloc(&L.st, sizeof(uint32_t) * (L.st_half ? 122 : 243) * L.N)) return fail(HOMO3D_HIP_ERROR);
+        }
     }
     const size_t N0 = h->lv[0].N, vb = sizeof(float) * 3 * kRhs * N0;
     if (hipMalloc(&h->X, vb) || hipMalloc(&h->F, vb) || hipMalloc(&h->P, vb) || hipMalloc(&h->Q, vb) ||
         hipMalloc(&h->Z, vb) || hipMalloc(&h->Rr, vb) || hipMalloc(&h->d_w, sizeof(float) * N0) ||
         hipMalloc(&h->d_x, sizeof(float) * N0))
         return fail(HOMO3D_HIP_ERROR);
-    hipMemset(h->X, 0, vb);
-    hipMemset(h->F, 0, vb);
+    if (hipMemset(h->X, 0, vb) || hipMemset(h->F, 0, vb)) return fail(HOMO3D_HIP_ERROR);
     // per-block partial sums (<= 6 per 256 vertices) and per-wave ones (6 per 64 vertices)
     h->partial_elems = std::max((size_t)21 * kDotBlocks, 6 * ((N0 + 63) / 64));
-    if (hipMalloc(&h->U64, sizeof(double) * 3 * kRhs * N0)) return fail(HOMO3D_HIP_ERROR);
-    hipMemset(h->U64, 0, sizeof(double) * 3 * kRhs * N0);
+    if (hipMalloc(&h->U64, sizeof(double) * 3 * kRhs * N0) || hipMemset(h->U64, 0, sizeof(double) * 3 * kRhs * N0))
+        return fail(HOMO3D_HIP_ERROR);
     if (hipMalloc(&h->d_partial, sizeof(double) * h->partial_elems) || hipMalloc(&h->d_sc, sizeof(double) * 64) ||
         hipMalloc(&h->d_tmp, sizeof(double) * 64) || hipMalloc(&h->d_ftmp, sizeof(float) * 4096))
         return fail(HOMO3D_HIP_ERROR);
@@ -597,16 +606,18 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
                 for (int c = 0; c < 3; c++) t += h->ke[r_ * 24 + 3 * b + c] * macro_disp(b, s_, c);
             kchi64[i] = t;
         }
-        hipMemcpy(h->d_ke64, h->ke, sizeof(double) * 576, hipMemcpyHostToDevice);
-        hipMemcpy(h->d_kchi64, kchi64.data(), sizeof(double) * 144, hipMemcpyHostToDevice);
+        if (hipMemcpy(h->d_ke64, h->ke, sizeof(double) * 576, hipMemcpyHostToDevice) ||
+            hipMemcpy(h->d_kchi64, kchi64.data(), sizeof(double) * 144, hipMemcpyHostToDevice))
+            return fail(HOMO3D_HIP_ERROR);
     }
     if (hipMalloc(&h->d_ke, sizeof(float) * 576) || hipMalloc(&h->d_kchi, sizeof(float) * 144) ||
         hipMalloc(&h->d_G, sizeof(float) * G.size()) || hipMalloc(&h->d_gperm, sizeof(int) * 48) ||
         hipMalloc(&h->d_gflip, sizeof(int) * 48))
         return fail(HOMO3D_HIP_ERROR);
-    hipMemcpy(h->d_ke, kef.data(), sizeof(float) * 576, hipMemcpyHostToDevice);
-    hipMemcpy(h->d_kchi, kchi.data(), sizeof(float) * 144, hipMemcpyHostToDevice);
-    hipMemcpy(h->d_G, G.data(), sizeof(float) * G.size(), hipMemcpyHostToDevice);
+    if (hipMemcpy(h->d_ke, kef.data(), sizeof(float) * 576, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->d_kchi, kchi.data(), sizeof(float) * 144, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->d_G, G.data(), sizeof(float) * G.size(), hipMemcpyHostToDevice))
+        return fail(HOMO3D_HIP_ERROR);
     const int nc = sizes[nl - 1];
     h->mc = 3 * nc * nc * nc;
     if (hipMalloc(&h->d_Ainv, sizeof(double) * h->mc * h->mc)) return fail(HOMO3D_HIP_ERROR);
@@ -617,19 +628,20 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
 
 int homo3d_destroy(homo3d_handle* h) {
     if (!h) return HOMO3D_OK;
-    if (h->stream) hipStreamSynchronize(h->stream);
-    for (hipEvent_t e : h->pool) hipEventDestroy(e);
+    // teardown: errors are not actionable here, so they are ignored
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
     for (int l = 0; l < kMaxLevels; l++) {
-        hipFree(h->lv[l].u);
-        hipFree(h->lv[l].f);
-        hipFree(h->lv[l].r);
-        hipFree(h->lv[l].st);
+        (void)hipFree(h->lv[l].u);
+        (void)hipFree(h->lv[l].f);
+        (void)hipFree(h->lv[l].r);
+        (void)hipFree(h->lv[l].st);
     }
     void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->X, h->F, h->P, h->Q, h->Z,
                     h->Rr, h->d_partial, h->d_sc, h->d_tmp, h->d_ftmp, h->d_gperm, h->d_gflip};
     for (void* p : ptrs)
-        if (p) hipFree(p);
-    if (h->stream) hipStreamDestroy(h->stream);
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HOMO3D_OK;
 }
@@ -657,9 +669,10 @@ int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_de
     if (rc) return rc;
     launch_set_weights(h->stream, n, pitch, h->cfg.emin, dx, h->d_w, h->d_x);
     const float clamp = (float)h->cfg.diag_clamp;
-    launch_galerkin_fine(h->stream, n, h->lv[1].lex, h->d_G, h->d_ke, h->d_w, h->lv[1].st, clamp);
+    launch_galerkin_fine(h->stream, n, h->lv[1].lex, h->d_G, h->d_ke, h->d_w, h->lv[1].st, h->lv[1].st_half, clamp);
     for (int l = 2; l < h->nl; l++)
-        launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l].lex, h->lv[l - 1].st, h->lv[l].st, clamp);
+        launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l].lex, h->lv[l - 1].st, h->lv[l - 1].st_half,
+                               h->lv[l].st, h->lv[l].st_half, clamp);
     // coarsest dense system
     const Level& Lc = h->lv[h->nl - 1];
     std::vector<float> st(243 * Lc.N);
@@ -942,7 +955,7 @@ int homo3d_apply_K(homo3d_handle* h) {
     if (rc) return rc;
     launch_zero_pin(h->stream, h->lv[0].N, h->X, 0);
     ProfEvent ev;
-    prof_begin(h, 1, ev);
+    prof_begin(h, kProfApply, ev);
     launch_fine_apply(h->stream, h->lv[0].n, h->d_ke, h->d_w, h->X, 0, nullptr, h->lv[0].r);
     prof_end(h, ev);
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1001,14 +1014,27 @@ int homo3d_get_stencil(homo3d_handle* h, int level, float* host) {
     if (!h || !host || level < 1 || level >= h->nl) return HOMO3D_BAD_ARG;
     if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
     const Level& L = h->lv[level];
-    std::vector<float> soa(243 * L.N);
-    HIPCHK(h, hipMemcpyAsync(soa.data(), L.st, sizeof(float) * soa.size(), hipMemcpyDeviceToHost, h->stream));
+    const int words = L.st_half ? 122 : 243;
+    std::vector<uint32_t> soa((size_t)words * L.N);
+    HIPCHK(h, hipMemcpyAsync(soa.data(), L.st, sizeof(uint32_t) * soa.size(), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    auto entry = [&](int k, size_t sv) -> float {
+        const uint32_t wv = soa[(size_t)(L.st_half ? k >> 1 : k) * L.N + sv];
+        if (!L.st_half) {
+            float f;
+            std::memcpy(&f, &wv, sizeof(f));
+            return f;
+        }
+        const uint16_t bits = (uint16_t)((k & 1) ? wv >> 16 : wv & 0xFFFFu);
+        _Float16 hv;
+        std::memcpy(&hv, &bits, sizeof(hv));
+        return (float)hv;
+    };
     for (int z = 0; z < L.n; z++)
         for (int y = 0; y < L.n; y++)
             for (int x = 0; x < L.n; x++) {
                 const size_t v = x + (size_t)L.n * (y + (size_t)L.n * z), sv = store_index(L, x, y, z);
-                for (int k = 0; k < 243; k++) host[v * 243 + k] = soa[(size_t)k * L.N + sv];
+                for (int k = 0; k < 243; k++) host[v * 243 + k] = entry(k, sv);
             }
     return HOMO3D_OK;
 }

@@ -10,10 +10,12 @@ namespace homo3d {
 // out = K u (f == nullptr) or f - K u
 void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
                        const float* f, float* out);
-// out = K u with the per-load-case FP64 dot u . out -> out6.  partial holds 6 * n^3 / 512 doubles (and
-// at least 6 * dot_blocks).
-void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const float* u, float* out,
-                           double* partial, int dot_blocks, double* out6);
+// CG step q = K p with the per-load-case FP64 dot p . q -> out6.  With beta (device, per load case) the
+// operator is applied to u = z and q = K z + beta q_old (p = z + beta p_old); without, u must be p.
+// u is FP32 or packed FP16 (u_half).  partial holds 6 * n^3 / 512 doubles (and at least 6 * dot_blocks).
+void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
+                           const float* p, const double* beta, float* out, double* partial, int dot_blocks,
+                           double* out6);
 // zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
                     const float* w, void* u, int u_half, const float* f);
@@ -27,15 +29,17 @@ void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x);
 void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u);
 void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half);
 void launch_zero_pin64(hipStream_t s, size_t N, double* u);
-void launch_stencil_apply(hipStream_t s, int n, int lex, const float* st, const float* u, const float* f,
+// Coarse stencils: FP32 [243][n^3] or, with *_half, packed FP16 [122][n^3] words (entries 2w, 2w + 1).
+void launch_stencil_apply(hipStream_t s, int n, int lex, const void* st, int st_half, const float* u, const float* f,
                           float* out);
-void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* st,
-                       float* u, const float* f);
+void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const void* st,
+                       int st_half, float* u, const float* f);
 void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc);
 void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, void* uf, int u_half, int pin);
-void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
-                          float clamp);
-void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const float* stf, float* stc, float clamp);
+void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, void* stc,
+                          int out_half, float clamp);
+void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const void* stf, int in_half, void* stc, int out_half,
+                            float clamp);
 void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float* f, float* u);
 void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks, double* out);
 void launch_sum_partials(hipStream_t s, int R, int nblocks, const double* partial, double* out);
@@ -45,10 +49,11 @@ void launch_pcg_beta(hipStream_t s, double* sc);
 // x += alpha p and, when do_p, p = z + beta p for the active load cases.
 void launch_pcg_update_r(hipStream_t s, size_t M, const double* sc, const float* q, float* r, double* partial,
                          int nblocks, double* out6);
+// z FP32 or packed FP16 (z_half); p and x FP32
 void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const void* z, int z_half, float* p, float* x,
                           int do_p, int nblocks);
-// p = z (z FP32 or packed FP16)
-void launch_set_p(hipStream_t s, size_t M, const void* z, int z_half, float* p, int nblocks);
+// p = z
+hipError_t launch_set_p(hipStream_t s, size_t M, const void* z, int z_half, float* p, int nblocks);
 // per-load-case r . z -> out6 (z FP32 or packed FP16)
 void launch_dot_rz(hipStream_t s, size_t M, const float* r, const void* z, int z_half, double* partial, int nblocks,
                    double* out6);

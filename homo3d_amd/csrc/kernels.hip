@@ -295,17 +295,18 @@ __device__ __forceinline__ void stage_ke(T* ke_s, const T* __restrict__ ke) {
 
 // Block Gauss-Seidel / SOR update of one vertex: x, y, z in turn (z, y, x when REVERSE)
 // (gs_relaxation_otf_kernel update formula, grid.cu:530-541).
+// inv holds 1 / Ks[0], 1 / Ks[4], 1 / Ks[8] (shared by all load cases of the vertex).
 template <bool REVERSE>
 __device__ __forceinline__ void block_update(float u[3], const float f[3], const float KeU[3], const float Ks[9],
-                                             float omega) {
+                                             const float inv[3], float omega) {
     if (!REVERSE) {
-        u[0] = omega * (f[0] - KeU[0] - Ks[1] * u[1] - Ks[2] * u[2]) / Ks[0] + (1.f - omega) * u[0];
-        u[1] = omega * (f[1] - KeU[1] - Ks[3] * u[0] - Ks[5] * u[2]) / Ks[4] + (1.f - omega) * u[1];
-        u[2] = omega * (f[2] - KeU[2] - Ks[6] * u[0] - Ks[7] * u[1]) / Ks[8] + (1.f - omega) * u[2];
+        u[0] = omega * (f[0] - KeU[0] - Ks[1] * u[1] - Ks[2] * u[2]) * inv[0] + (1.f - omega) * u[0];
+        u[1] = omega * (f[1] - KeU[1] - Ks[3] * u[0] - Ks[5] * u[2]) * inv[1] + (1.f - omega) * u[1];
+        u[2] = omega * (f[2] - KeU[2] - Ks[6] * u[0] - Ks[7] * u[1]) * inv[2] + (1.f - omega) * u[2];
     } else {
-        u[2] = omega * (f[2] - KeU[2] - Ks[6] * u[0] - Ks[7] * u[1]) / Ks[8] + (1.f - omega) * u[2];
-        u[1] = omega * (f[1] - KeU[1] - Ks[3] * u[0] - Ks[5] * u[2]) / Ks[4] + (1.f - omega) * u[1];
-        u[0] = omega * (f[0] - KeU[0] - Ks[1] * u[1] - Ks[2] * u[2]) / Ks[0] + (1.f - omega) * u[0];
+        u[2] = omega * (f[2] - KeU[2] - Ks[6] * u[0] - Ks[7] * u[1]) * inv[2] + (1.f - omega) * u[2];
+        u[1] = omega * (f[1] - KeU[1] - Ks[3] * u[0] - Ks[5] * u[2]) * inv[1] + (1.f - omega) * u[1];
+        u[0] = omega * (f[0] - KeU[0] - Ks[1] * u[1] - Ks[2] * u[2]) * inv[0] + (1.f - omega) * u[0];
     }
 }
 
@@ -320,17 +321,21 @@ __device__ __forceinline__ uint32_t colour_block(int colour) { return (uint32_t)
 
 // MODE 0: out = K u; MODE 1: out = f - K u.  Pinned origin row is zero.
 // (update_residual_otf_kernel_1 grid.cu:1277-1461; v3_stencilOnLeft_kernel grid.cu:3686-3857)
-// DOT (MODE 0 only): also the per-workgroup FP64 partial sums of u . out per load case (the p . Kp of
-// a CG step) in partial[r * gridDim.x + block].  The launcher uses it only when every lane of the
-// grid holds a vertex (64 | (n/2)^3), so no thread leaves before the workgroup reduction.
-// A: storage of u (float, or half2_t pairs of load cases; DOT needs float).
-template <int R, int MODE, bool DOT, typename A>
+// MODE 0: out = K u, or with beta (per load case) out = K u + beta out_old, the CG recurrence
+// K p = K z + beta K p_old that applies K to the (cheaper to gather) preconditioned vector z.
+// DOT (MODE 0): also the per-workgroup FP64 partial sums of dotv . out per load case (the p . Kp of a
+// CG step) in partial[r * gridDim.x + block].  The launcher uses it only when every lane of the grid
+// holds a vertex (64 | (n/2)^3), so no thread leaves before the workgroup reduction.
+// A: storage of u (float, or half2_t pairs of load cases).  BETA selects the recurrence form at compile
+// time (a run-time test in the epilogue makes the compiler spill the gather pipeline).
+template <int R, int MODE, bool DOT, typename A, bool BETA = false>
 __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
                                                                 const float* __restrict__ w,
                                                                 const A* __restrict__ u,
                                                                 const float* __restrict__ f, float* __restrict__ out,
-                                                                double* __restrict__ partial) {
-    static_assert(!DOT || CasesPerWord<A>::value == 1, "the fused dot reads u as FP32");
+                                                                double* __restrict__ partial,
+                                                                const float* __restrict__ dotv,
+                                                                const double* __restrict__ beta) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
     uint32_t v;
@@ -342,26 +347,31 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
     float acc[R][3];
     fine_gather<R, kRowAll, float, A>(c, L.n, N, ke_s, u, acc);
 #pragma unroll
-    for (int r = 0; r < R; r++)
+    for (int r = 0; r < R; r++) {
+        float b = 0.f;
+        if constexpr (BETA) b = (float)beta[r];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const size_t o = (size_t)(r * 3 + i) * N + v;
             float val = (MODE == 1) ? f[o] - acc[r][i] : acc[r][i];
-            out[o] = (v == 0) ? 0.f : val;
+            if constexpr (BETA) val = fmaf(b, out[o], val);
+            acc[r][i] = (v == 0) ? 0.f : val;
+            out[o] = acc[r][i];
         }
+    }
     if constexpr (DOT) {
         __shared__ double sh[kFullThreads / 64];
+        double dot[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            double d = 0.0;
-            if (v != 0) {
+            dot[r] = 0.0;
 #pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const size_t o = (size_t)(r * 3 + i) * N + v;
-                    d = fma((double)u[o], (double)acc[r][i], d);
-                }
-            }
-            d = block_sum<kFullThreads>(d, sh);
+            for (int i = 0; i < 3; i++)
+                dot[r] = fma((double)dotv[(size_t)(r * 3 + i) * N + v], (double)acc[r][i], dot[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double d = block_sum<kFullThreads>(dot[r], sh);
             if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = d;
         }
     }
@@ -397,6 +407,7 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
     fine_gather<R, gs_mask(ZC), float, A>(c, L.n, N, ke_s, u, acc, r0);
     float Ks[9];
     fine_block<13, float>(c.wc, ke_s, Ks);
+    const float inv[3] = {1.f / Ks[0], 1.f / Ks[4], 1.f / Ks[8]};
 #pragma unroll
     for (int p = 0; p < R / cpw; p++) {
         float uu[cpw][3], ff[cpw][3];
@@ -414,7 +425,7 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
             for (int h = 0; h < cpw; h++) ff[h][i] = f[(size_t)((r0 + p * cpw + h) * 3 + i) * N + v];
         }
 #pragma unroll
-        for (int h = 0; h < cpw; h++) block_update<REVERSE>(uu[h], ff[h], acc[p * cpw + h], Ks, omega);
+        for (int h = 0; h < cpw; h++) block_update<REVERSE>(uu[h], ff[h], acc[p * cpw + h], Ks, inv, omega);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const size_t o = (size_t)((r0 / cpw + p) * 3 + i) * N + v;
@@ -557,14 +568,38 @@ __global__ void k_zero_pin(size_t N, int planes, T* __restrict__ u) {
 // Coarse levels (stored 27-point stencils)
 // ---------------------------------------------------------------------------------------------
 
-template <int R, uint32_t MASK>
+// A stored coarse stencil: FP32 entries [243][N] (entry e = 9d + 3i + j), or packed FP16 [122][N]
+// words, word w holding entries 2w (.x) and 2w + 1 (.y).  Both have 4-byte words per vertex.
+template <typename SA>
+struct StencilIO;
+template <>
+struct StencilIO<float> {
+    static constexpr int words = 243;
+    __device__ static float get(Rsrc rs, uint32_t vo, uint32_t plane, int e) {
+        return bload<float>(rs, vo, (uint32_t)e * plane);
+    }
+    __device__ static void put(float* st, size_t N, uint32_t v, int e, float val) { st[(size_t)e * N + v] = val; }
+};
+template <>
+struct StencilIO<half2_t> {
+    static constexpr int words = 122;
+    __device__ static float get(Rsrc rs, uint32_t vo, uint32_t plane, int e) {
+        return (float)bload<half2_t>(rs, vo, (uint32_t)(e >> 1) * plane)[e & 1];
+    }
+    // one 16-bit store per entry: the two halves of a word may come from different threads
+    __device__ static void put(half2_t* st, size_t N, uint32_t v, int e, float val) {
+        reinterpret_cast<_Float16*>(st + (size_t)(e >> 1) * N + v)[e & 1] = (_Float16)val;
+    }
+};
+
+template <int R, uint32_t MASK, typename SA>
 __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L, size_t N, uint32_t v,
-                                               const float* __restrict__ st, const float* __restrict__ u,
+                                               const SA* __restrict__ st, const float* __restrict__ u,
                                                float acc[R][3], int r0) {
     const uint32_t X[3] = {L.part(wrap_dec(x, L.n), 0), L.part(x, 0), L.part(wrap_inc(x, L.n), 0)};
     const uint32_t Y[3] = {L.part(wrap_dec(y, L.n), 1), L.part(y, 1), L.part(wrap_inc(y, L.n), 1)};
     const uint32_t Z[3] = {L.part(wrap_dec(z, L.n), 2), L.part(z, 2), L.part(wrap_inc(z, L.n), 2)};
-    const Rsrc rs = make_rsrc(st, (size_t)243 * N * sizeof(float));
+    const Rsrc rs = make_rsrc(st, (size_t)StencilIO<SA>::words * N * 4);
     const Rsrc ru = make_rsrc(u, (size_t)18 * N * sizeof(float));
     const uint32_t plane = (uint32_t)(N * sizeof(float));
     const uint32_t vo = v * 4u;
@@ -577,7 +612,7 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L
         if (!((MASK >> d) & 1u)) continue;
         float S[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) S[k] = bload<float>(rs, vo, (uint32_t)(d * 9 + k) * plane);
+        for (int k = 0; k < 9; k++) S[k] = StencilIO<SA>::get(rs, vo, plane, d * 9 + k);
         const uint32_t qo = (X[d % 3] + Y[d / 3 % 3] + Z[d / 9]) * 4u;
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -592,8 +627,8 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L
 }
 
 // MODE 0: out = A u; MODE 1: out = f - A u (update_residual_kernel_1, grid.cu:1465-1564)
-template <int R, int MODE>
-__global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N, const float* __restrict__ st,
+template <int R, int MODE, typename SA>
+__global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N, const SA* __restrict__ st,
                                                                 const float* __restrict__ u,
                                                                 const float* __restrict__ f, float* __restrict__ out) {
     uint32_t v;
@@ -602,7 +637,7 @@ __global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N,
     L.pos(v, x, y, z);
     float acc[R][3];
     const int r0 = blockIdx.y * R;
-    stencil_gather<R, kRowAll>(x, y, z, L, N, v, st, u, acc, r0);
+    stencil_gather<R, kRowAll, SA>(x, y, z, L, N, v, st, u, acc, r0);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -614,9 +649,9 @@ __global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N,
 
 // One colour of the coarse block GS sweep (gs_relaxation_kernel, grid.cu:556-689); no pin on coarse levels.
 // ZC as in k_fine_gs.
-template <int R, bool REVERSE, int ZC>
+template <int R, bool REVERSE, int ZC, typename SA>
 __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour, float omega,
-                                                    const float* __restrict__ st, float* __restrict__ u,
+                                                    const SA* __restrict__ st, float* __restrict__ u,
                                                     const float* __restrict__ f) {
     const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.h3) return;
@@ -625,10 +660,12 @@ __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour,
     L.pos(v, x, y, z);
     float acc[R][3];
     const int r0 = blockIdx.y * R;
-    stencil_gather<R, gs_mask(ZC)>(x, y, z, L, N, v, st, u, acc, r0);
+    stencil_gather<R, gs_mask(ZC), SA>(x, y, z, L, N, v, st, u, acc, r0);
     float Ks[9];
+    const Rsrc rs = make_rsrc(st, (size_t)StencilIO<SA>::words * N * 4);
 #pragma unroll
-    for (int k = 0; k < 9; k++) Ks[k] = st[(size_t)(13 * 9 + k) * N + v];
+    for (int k = 0; k < 9; k++) Ks[k] = StencilIO<SA>::get(rs, v * 4u, (uint32_t)(N * 4), 13 * 9 + k);
+    const float inv[3] = {1.f / Ks[0], 1.f / Ks[4], 1.f / Ks[8]};
 #pragma unroll
     for (int r = 0; r < R; r++) {
         float uu[3], ff[3];
@@ -637,7 +674,7 @@ __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour,
             uu[i] = ZC >= 0 ? 0.f : u[(size_t)((r0 + r) * 3 + i) * N + v];
             ff[i] = f[(size_t)((r0 + r) * 3 + i) * N + v];
         }
-        block_update<REVERSE>(uu, ff, acc[r], Ks, omega);
+        block_update<REVERSE>(uu, ff, acc[r], Ks, inv, omega);
 #pragma unroll
         for (int i = 0; i < 3; i++) u[(size_t)((r0 + r) * 3 + i) * N + v] = uu[i];
     }
@@ -904,8 +941,9 @@ __device__ __forceinline__ float hat(int o) { return o == 0 ? 1.f : ((o == 1 || 
 
 // Level-1 stencil away from the pinned origin: st[V][D] = sum_{c} w_{2V+c} G[c][D], c in {-2..1}^3.
 // G (64 x 27 x 9) is the density-independent table of P-weighted KE blocks.
+template <typename SO>
 __global__ __launch_bounds__(256) void k_galerkin_fine(Lay Lf, Lay Lc, const float* __restrict__ G,
-                                                       const float* __restrict__ w, float* __restrict__ stc,
+                                                       const float* __restrict__ w, SO* __restrict__ stc,
                                                        float clamp) {
     const int nf = Lf.n;
     const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n;
@@ -937,14 +975,15 @@ __global__ __launch_bounds__(256) void k_galerkin_fine(Lay Lf, Lay Lc, const flo
             if (fabsf(S[8]) < clamp) S[8] = clamp;
         }
 #pragma unroll
-        for (int k = 0; k < 9; k++) stc[(size_t)(D * 9 + k) * Nc + V] = S[k];
+        for (int k = 0; k < 9; k++) StencilIO<SO>::put(stc, Nc, V, D * 9 + k, S[k]);
     }
 }
 
 // Level-1 stencil of the 27 coarse vertices around the pinned origin: node pairs touching the
 // origin contribute identity when they are the same node and nothing otherwise.
+template <typename SO>
 __global__ void k_galerkin_fine_pin(Lay Lf, Lay Lc, const float* __restrict__ ke, const float* __restrict__ w,
-                                    float* __restrict__ stc, float clamp) {
+                                    SO* __restrict__ stc, float clamp) {
     const int nf = Lf.n, nc = Lc.n;
     const size_t Nc = (size_t)nc * nc * nc;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (t%3-1, t/3%3-1, t/9-1) around the origin
@@ -989,12 +1028,13 @@ __global__ void k_galerkin_fine_pin(Lay Lf, Lay Lc, const float* __restrict__ ke
         if (fabsf(S[8]) < clamp) S[8] = clamp;
     }
     const uint32_t V = Lc.idx((X + nc) % nc, (Y + nc) % nc, (Z + nc) % nc);
-    for (int k = 0; k < 9; k++) stc[(size_t)(D * 9 + k) * Nc + V] = S[k];
+    for (int k = 0; k < 9; k++) StencilIO<SO>::put(stc, Nc, V, D * 9 + k, S[k]);
 }
 
 // Coarser stencil from a stored one: st_c[V][D] = sum_{o,d} hat(o) A_f[2V+o][d] hat3(o+d-2D).
-__global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const float* __restrict__ stf,
-                                                         float* __restrict__ stc, float clamp) {
+template <typename SI, typename SO>
+__global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const SI* __restrict__ stf,
+                                                         SO* __restrict__ stc, float clamp) {
     const int nf = Lf.n;
     const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n, Nf = (size_t)nf * nf * nf;
     int X, Y, Z;
@@ -1003,6 +1043,7 @@ __global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const f
     const uint32_t xs[3] = {Lf.part(wrap_dec(2 * X, nf), 0), Lf.part(2 * X, 0), Lf.part(2 * X + 1, 0)};
     const uint32_t ys[3] = {Lf.part(wrap_dec(2 * Y, nf), 1), Lf.part(2 * Y, 1), Lf.part(2 * Y + 1, 1)};
     const uint32_t zs[3] = {Lf.part(wrap_dec(2 * Z, nf), 2), Lf.part(2 * Z, 2), Lf.part(2 * Z + 1, 2)};
+    const Rsrc rf = make_rsrc(stf, (size_t)StencilIO<SI>::words * Nf * 4);
     {
         const int D = blockIdx.y;  // one coarse stencil direction per grid row
         const int Dx = D % 3 - 1, Dy = D / 3 % 3 - 1, Dz = D / 9 - 1;
@@ -1019,7 +1060,8 @@ __global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const f
                 if (wd == 0.f) continue;
                 const float f = wo * wd;
 #pragma unroll
-                for (int k = 0; k < 9; k++) S[k] = fmaf(f, stf[(size_t)(d * 9 + k) * Nf + vf], S[k]);
+                for (int k = 0; k < 9; k++)
+                    S[k] = fmaf(f, StencilIO<SI>::get(rf, (uint32_t)vf * 4u, (uint32_t)(Nf * 4), d * 9 + k), S[k]);
             }
         }
         if (D == 13) {
@@ -1028,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const f
             if (fabsf(S[8]) < clamp) S[8] = clamp;
         }
 #pragma unroll
-        for (int k = 0; k < 9; k++) stc[(size_t)(D * 9 + k) * Nc + V] = S[k];
+        for (int k = 0; k < 9; k++) StencilIO<SO>::put(stc, Nc, V, D * 9 + k, S[k]);
     }
 }
 
@@ -1505,30 +1547,48 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, co
     if (u_half) {
         const half2_t* uh = static_cast<const half2_t*>(u);
         if (f)
-            hipLaunchKernelGGL((k_fine_apply<6, 1, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr);
+            hipLaunchKernelGGL((k_fine_apply<6, 1, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr,
+                               nullptr, nullptr);
         else
-            hipLaunchKernelGGL((k_fine_apply<6, 0, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr);
+            hipLaunchKernelGGL((k_fine_apply<6, 0, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr,
+                               nullptr, nullptr);
         return;
     }
     const float* uf = static_cast<const float*>(u);
     if (f)
-        hipLaunchKernelGGL((k_fine_apply<6, 1, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr);
+        hipLaunchKernelGGL((k_fine_apply<6, 1, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr, nullptr,
+                           nullptr);
     else
-        hipLaunchKernelGGL((k_fine_apply<6, 0, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr);
+        hipLaunchKernelGGL((k_fine_apply<6, 0, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr, nullptr,
+                           nullptr);
 }
 
-void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const float* u, float* out,
-                           double* partial, int dot_blocks, double* out6) {
+void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
+                           const float* p, const double* beta, float* out, double* partial, int dot_blocks,
+                           double* out6) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
     const unsigned nb = full_grid(L);
-    if (L.h3 % 64) {  // partial waves: separate dot
-        launch_fine_apply(s, n, ke, w, u, 0, nullptr, out);
-        launch_dot6(s, 3 * N, u, out, partial, dot_blocks, out6);
+    if (L.h3 % 64) {  // partial waves (FP32 fields only: packed FP16 needs n % 8 == 0): K p, separate dot
+        launch_fine_apply(s, n, ke, w, p, 0, nullptr, out);
+        launch_dot6(s, 3 * N, p, out, partial, dot_blocks, out6);
         return;
     }
-    hipLaunchKernelGGL((k_fine_apply<6, 0, true, float>), dim3(nb), dim3(kFullThreads), 0, s, L, N, ke, w, u, nullptr, out,
-                       partial);
+    const dim3 g(nb), b(kFullThreads);
+    const half2_t* uh = static_cast<const half2_t*>(u);
+    const float* uf = static_cast<const float*>(u);
+    if (u_half && beta)
+        hipLaunchKernelGGL((k_fine_apply<6, 0, true, half2_t, true>), g, b, 0, s, L, N, ke, w, uh, nullptr, out,
+                           partial, p, beta);
+    else if (u_half)
+        hipLaunchKernelGGL((k_fine_apply<6, 0, true, half2_t, false>), g, b, 0, s, L, N, ke, w, uh, nullptr, out,
+                           partial, p, beta);
+    else if (beta)
+        hipLaunchKernelGGL((k_fine_apply<6, 0, true, float, true>), g, b, 0, s, L, N, ke, w, uf, nullptr, out,
+                           partial, p, beta);
+    else
+        hipLaunchKernelGGL((k_fine_apply<6, 0, true, float, false>), g, b, 0, s, L, N, ke, w, uf, nullptr, out,
+                           partial, p, beta);
     hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)nb, partial, out6);
 }
 
@@ -1575,18 +1635,22 @@ void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, f
     hipLaunchKernelGGL(k_macro_rhs, dim3(full_grid(L)), dim3(kFullThreads), 0, s, L, N, kchi, w, f);
 }
 
+// One load case per FP64 residual launch: with two, the FP64 gather pipeline no longer fits the
+// register file.
+constexpr int kResid64Cases = 1;
+
 template <bool DO_F, bool DO_K>
 static void residual64_case(hipStream_t s, const Lay& L, size_t N, int r0, const double* ke, const double* kchi,
                             const float* xc, double emin, const double* u, float* r32, double* partial) {
-    hipLaunchKernelGGL((k_fine_residual64<1, DO_F, DO_K>), dim3(full_grid(L)), dim3(kFullThreads), 0, s, L, N, r0, ke,
-                       kchi, xc, emin, u, r32, partial);
+    hipLaunchKernelGGL((k_fine_residual64<kResid64Cases, DO_F, DO_K>), dim3(full_grid(L)), dim3(kFullThreads), 0, s, L,
+                       N, r0, ke, kchi, xc, emin, u, r32, partial);
 }
 
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
                             const double* u, float* r32, double* partial, double* out6, int do_f, int do_k) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
-    for (int r0 = 0; r0 < 6; r0 += 1) {
+    for (int r0 = 0; r0 < 6; r0 += kResid64Cases) {
         if (!do_k)
             residual64_case<true, false>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
         else if (do_f)
@@ -1634,35 +1698,46 @@ void launch_zero_pin64(hipStream_t s, size_t N, double* u) {
 // there.  Larger levels keep all six load cases per thread so the stored stencil is read once.
 constexpr int kSplitLoadCasesMaxN = 32;
 
-void launch_stencil_apply(hipStream_t s, int n, int lex, const float* st, const float* u, const float* f,
+template <int R, typename SA>
+static void stencil_apply_case(hipStream_t s, const Lay& L, size_t N, const SA* st, const float* u, const float* f,
+                               float* out) {
+    const dim3 g(full_grid(L), 6 / R), b(kFullThreads);
+    if (f)
+        hipLaunchKernelGGL((k_stencil_apply<R, 1, SA>), g, b, 0, s, L, N, st, u, f, out);
+    else
+        hipLaunchKernelGGL((k_stencil_apply<R, 0, SA>), g, b, 0, s, L, N, st, u, f, out);
+}
+
+void launch_stencil_apply(hipStream_t s, int n, int lex, const void* st, int st_half, const float* u, const float* f,
                           float* out) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, lex);
     const bool split = n <= kSplitLoadCasesMaxN;
-    const dim3 g(full_grid(L), split ? 6 : 1), b(kFullThreads);
-    if (split) {
-        if (f)
-            hipLaunchKernelGGL((k_stencil_apply<1, 1>), g, b, 0, s, L, N, st, u, f, out);
+    if (st_half) {
+        const half2_t* sh = static_cast<const half2_t*>(st);
+        if (split)
+            stencil_apply_case<1>(s, L, N, sh, u, f, out);
         else
-            hipLaunchKernelGGL((k_stencil_apply<1, 0>), g, b, 0, s, L, N, st, u, f, out);
+            stencil_apply_case<6>(s, L, N, sh, u, f, out);
     } else {
-        if (f)
-            hipLaunchKernelGGL((k_stencil_apply<6, 1>), g, b, 0, s, L, N, st, u, f, out);
+        const float* sf = static_cast<const float*>(st);
+        if (split)
+            stencil_apply_case<1>(s, L, N, sf, u, f, out);
         else
-            hipLaunchKernelGGL((k_stencil_apply<6, 0>), g, b, 0, s, L, N, st, u, f, out);
+            stencil_apply_case<6>(s, L, N, sf, u, f, out);
     }
 }
 
-template <int R, bool REV, int ZC>
-static void stencil_gs_colour(hipStream_t s, const Lay& L, size_t N, int colour, float omega, const float* st,
-                              float* u, const float* f) {
-    hipLaunchKernelGGL((k_stencil_gs<R, REV, ZC>), dim3(nblk(L.h3), 6 / R), dim3(256), 0, s, L, N, colour, omega, st,
-                       u, f);
+template <int R, bool REV, int ZC, typename SA>
+static void stencil_gs_colour(hipStream_t s, const Lay& L, size_t N, int colour, float omega, const SA* st, float* u,
+                              const float* f) {
+    hipLaunchKernelGGL((k_stencil_gs<R, REV, ZC, SA>), dim3(nblk(L.h3), 6 / R), dim3(256), 0, s, L, N, colour, omega,
+                       st, u, f);
 }
 
-template <int R>
+template <int R, typename SA>
 static void stencil_gs_pick(hipStream_t s, const Lay& L, size_t N, int colour, float omega, int reverse,
-                            int zero_start, const float* st, float* u, const float* f) {
+                            int zero_start, const SA* st, float* u, const float* f) {
     if (reverse) {
         stencil_gs_colour<R, true, -1>(s, L, N, colour, omega, st, u, f);
         return;
@@ -1680,14 +1755,24 @@ static void stencil_gs_pick(hipStream_t s, const Lay& L, size_t N, int colour, f
     }
 }
 
-void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* st,
-                       float* u, const float* f) {
+void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const void* st,
+                       int st_half, float* u, const float* f) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
-    if (n <= kSplitLoadCasesMaxN)
-        stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, st, u, f);
-    else
-        stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, st, u, f);
+    const bool split = n <= kSplitLoadCasesMaxN;
+    if (st_half) {
+        const half2_t* sh = static_cast<const half2_t*>(st);
+        if (split)
+            stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, sh, u, f);
+        else
+            stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, sh, u, f);
+    } else {
+        const float* sf = static_cast<const float*>(st);
+        if (split)
+            stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, sf, u, f);
+        else
+            stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, sf, u, f);
+    }
 }
 
 // Transfers take one load case per thread (six grid rows): the load cases' planes are disjoint, so
@@ -1714,18 +1799,39 @@ void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, void* uf, 
                            static_cast<float*>(uf), pin);
 }
 
-void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, float* stc,
-                          float clamp) {
+void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, void* stc,
+                          int out_half, float clamp) {
     const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
     const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
-    hipLaunchKernelGGL(k_galerkin_fine, dim3(nblk(Nc)), dim3(256), 0, s, Lf, Lc, G, w, stc, clamp);
-    hipLaunchKernelGGL(k_galerkin_fine_pin, dim3(nblk(27 * 27)), dim3(256), 0, s, Lf, Lc, ke, w, stc, clamp);
+    if (out_half) {
+        half2_t* o = static_cast<half2_t*>(stc);
+        hipLaunchKernelGGL(k_galerkin_fine<half2_t>, dim3(nblk(Nc)), dim3(256), 0, s, Lf, Lc, G, w, o, clamp);
+        hipLaunchKernelGGL(k_galerkin_fine_pin<half2_t>, dim3(nblk(27 * 27)), dim3(256), 0, s, Lf, Lc, ke, w, o, clamp);
+    } else {
+        float* o = static_cast<float*>(stc);
+        hipLaunchKernelGGL(k_galerkin_fine<float>, dim3(nblk(Nc)), dim3(256), 0, s, Lf, Lc, G, w, o, clamp);
+        hipLaunchKernelGGL(k_galerkin_fine_pin<float>, dim3(nblk(27 * 27)), dim3(256), 0, s, Lf, Lc, ke, w, o, clamp);
+    }
 }
 
-void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const float* stf, float* stc, float clamp) {
-    const size_t Nc = (size_t)(nf / 2) * (nf / 2) * (nf / 2);
-    hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(Nc), 27), dim3(256), 0, s, make_lay(nf, 0),
-                       make_lay(nf / 2, lexc), stf, stc, clamp);
+template <typename SI, typename SO>
+static void galerkin_coarse_case(hipStream_t s, const Lay& Lf, const Lay& Lc, const void* stf, void* stc, float clamp) {
+    const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n;
+    hipLaunchKernelGGL((k_galerkin_coarse<SI, SO>), dim3(nblk(Nc), 27), dim3(256), 0, s, Lf, Lc,
+                       static_cast<const SI*>(stf), static_cast<SO*>(stc), clamp);
+}
+
+void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const void* stf, int in_half, void* stc, int out_half,
+                            float clamp) {
+    const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
+    if (in_half && out_half)
+        galerkin_coarse_case<half2_t, half2_t>(s, Lf, Lc, stf, stc, clamp);
+    else if (in_half)
+        galerkin_coarse_case<half2_t, float>(s, Lf, Lc, stf, stc, clamp);
+    else if (out_half)
+        galerkin_coarse_case<float, half2_t>(s, Lf, Lc, stf, stc, clamp);
+    else
+        galerkin_coarse_case<float, float>(s, Lf, Lc, stf, stc, clamp);
 }
 
 void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float* f, float* u) {
@@ -1763,12 +1869,11 @@ void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const void*
                            p, x, do_p);
 }
 
-void launch_set_p(hipStream_t s, size_t M, const void* z, int z_half, float* p, int nblocks) {
-    if (z_half)
-        hipLaunchKernelGGL(k_pcg_update_px_h, dim3(nblocks, 3), dim3(256), 0, s, 6, M, nullptr,
-                           static_cast<const half2_t*>(z), p, nullptr, 0, 1);
-    else
-        hipMemcpyAsync(p, z, sizeof(float) * 6 * M, hipMemcpyDeviceToDevice, s);
+hipError_t launch_set_p(hipStream_t s, size_t M, const void* z, int z_half, float* p, int nblocks) {
+    if (!z_half) return hipMemcpyAsync(p, z, sizeof(float) * 6 * M, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL(k_pcg_update_px_h, dim3(nblocks, 3), dim3(256), 0, s, 6, M, nullptr,
+                       static_cast<const half2_t*>(z), p, nullptr, 0, 1);
+    return hipGetLastError();
 }
 
 void launch_dot_rz(hipStream_t s, size_t M, const float* r, const void* z, int z_half, double* partial, int nblocks,

@@ -102,26 +102,26 @@ int homo3d_mma_create(int m, long n, double a0, double a, double c, double d, in
         hipMalloc(&h->partmax, sizeof(double) * h->nb) || hipMalloc(&h->out, sizeof(double) * 64) ||
         hipMalloc(&h->dlam, sizeof(double) * kMmaMaxM) || hipMalloc(&h->stage, sizeof(float) * n))
         return fail(HOMO3D_HIP_ERROR);
-    hipMemset(h->xmin, 0, sizeof(double) * n);
-    hipMemset(h->xold1, 0, sizeof(double) * n);
-    hipMemset(h->xold2, 0, sizeof(double) * n);
-    hipMemset(h->low, 0, sizeof(double) * n);
-    hipMemset(h->upp, 0, sizeof(double) * n);
     std::vector<double> ones(n, 1.0);
-    hipMemcpy(h->xmax, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    if (hipMemset(h->xmin, 0, sizeof(double) * n) || hipMemset(h->xold1, 0, sizeof(double) * n) ||
+        hipMemset(h->xold2, 0, sizeof(double) * n) || hipMemset(h->low, 0, sizeof(double) * n) ||
+        hipMemset(h->upp, 0, sizeof(double) * n) ||
+        hipMemcpy(h->xmax, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice))
+        return fail(HOMO3D_HIP_ERROR);
     *out = h;
     return HOMO3D_OK;
 }
 
 int homo3d_mma_destroy(homo3d_mma* h) {
     if (!h) return HOMO3D_OK;
-    if (h->stream) hipStreamSynchronize(h->stream);
+    // teardown: errors are not actionable here, so they are ignored
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* ptrs[] = {h->x, h->xmin, h->xmax, h->xold1, h->xold2, h->low, h->upp, h->df0, h->dfdx, h->alfa, h->beta,
                     h->p0, h->q0, h->P, h->Q, h->xs, h->xsi, h->eta, h->xso, h->xsio, h->etao, h->delx, h->diagx,
                     h->dx, h->dxsi, h->deta, h->part, h->partmax, h->out, h->dlam, h->stage};
     for (void* p : ptrs)
-        if (p) hipFree(p);
-    if (h->stream) hipStreamDestroy(h->stream);
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HOMO3D_OK;
 }

@@ -87,6 +87,8 @@ typedef struct {
     double diag_clamp;      /* coarse-stencil diagonal floor (1e-4) */
     int precond_half;       /* HOMO3D_SOLVER_PCG: keep the preconditioner's finest-level iterate in FP16
                                (accumulation stays FP32; needs n % 8 == 0, else FP32 is used).  Default 1. */
+    int coarse_half;        /* store the Galerkin stencils of the intermediate levels in FP16, as the reference
+                               does (grid_ex.cu:936-1160); the coarsest level stays FP32.  Default 1. */
 } homo3d_config;
 
 typedef struct {
@@ -147,7 +149,9 @@ int homo3d_mma_state(homo3d_mma* h, double* x, double* low, double* upp, int* ne
 /* Seeded randc density (project generator), written to host memory, n^3 floats, pitch n. */
 int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, float* out);
 
-/* Kernel timing: enable event pairs around the named kernel's launches, then read totals. */
+/* Kernel timing: enable event pairs around the finest-level kernel launches, then read totals per
+ * kernel: "fine_gs" (full GS colour passes), "fine_gs_zero" (colour passes of sweeps that start from
+ * zero), "fine_apply" (K p) and "fine_residual" (f - K u inside the V-cycle). */
 int homo3d_profile_enable(homo3d_handle* h, int enable);
 int homo3d_profile_get(homo3d_handle* h, const char* kernel, long* launches, double* total_ms);
 

@@ -33,8 +33,11 @@ def rel(a, b):
                  max(np.abs(np.asarray(b, np.float64)).max(), 1e-300))
 
 
-def make(h3d, n, solver="mg", seed=0, vol=0.3, sym="reflect6"):
-    hom = h3d.Homogenization(h3d.HomoConfig(reso=n, solver=solver))
+def make(h3d, n, solver="mg", seed=0, vol=0.3, sym="reflect6", coarse_half=False):
+    """Handle on a seeded randc density.  The operator-level parity tests use FP32 coarse stencils
+    (coarse_half=False), the exact counterpart of the oracle; the FP16 stencils of the default
+    configuration have their own test."""
+    hom = h3d.Homogenization(h3d.HomoConfig(reso=n, solver=solver, coarse_half=coarse_half))
     rho = h3d.randc_density(n, seed, vol, sym)
     x = (rho.astype(np.float64) ** 3).astype(np.float32)
     hom.update(x)
@@ -100,6 +103,23 @@ def test_galerkin_stencils(h3d, orc):
         got = hom.stencil(level)
         ref = mg.stencil(level)
         assert rel(got, ref) < 1e-5, level
+
+
+def test_galerkin_stencils_fp16(h3d, orc):
+    """FP16 storage of the intermediate levels' stencils (the default, as in the reference): each
+    level agrees with the FP64 Galerkin product to FP16 rounding compounded over the levels above it
+    (2^-11 relative per level, scale = the level's largest entry); the coarsest stays FP32."""
+    say("galerkin fp16: n=32")
+    hom, x = make(h3d, 32, seed=4, coarse_half=True)
+    mg = orc.MG(32, E, NU)
+    mg.set_density(x.astype(np.float64))
+    nl = len(hom.sizes)
+    for level in range(1, nl):
+        got = hom.stencil(level)
+        ref = mg.stencil(level)
+        assert rel(got, ref) < 2.0 ** -11 * (level + 1), level
+        if level < nl - 1:  # values are FP16-representable
+            assert np.array_equal(got.astype(np.float16).astype(np.float32), got), level
 
 
 def test_transfers_and_coarse_solve(case16, orc):
@@ -179,9 +199,15 @@ def test_mg_solve_and_elastic_matrix_32(h3d, orc):
     assert rel(C, C_same) < 1e-5
 
 
-def test_pcg_tight_solve_matches_oracle(h3d, orc):
-    say("pcg: n=32 relres 1e-5")
-    hom, x = make(h3d, 32, solver="pcg", seed=2, sym="rotate3", vol=0.2)
+@pytest.mark.parametrize("mixed", [True, False])
+def test_pcg_tight_solve_matches_oracle(h3d, orc, mixed):
+    """mixed: the production configuration (FP16 z, p and intermediate-level stencils); otherwise all
+    inner storage FP32.  Either way the FP64-refined relres meets the tolerance."""
+    say("pcg: n=32 relres 1e-5 mixed=%s" % mixed)
+    hom = h3d.Homogenization(h3d.HomoConfig(reso=32, solver="pcg", coarse_half=mixed, precond_half=mixed))
+    rho = h3d.randc_density(32, 2, 0.2, "rotate3")
+    x = (rho.astype(np.float64) ** 3).astype(np.float32)
+    hom.update(x)
     C = hom.elasticMatrix(tol=1e-5, warm=False)
     st = hom.last_stats
     assert st["status"] == 0 and max(st["relres"]) <= 1e-5
