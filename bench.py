@@ -26,7 +26,8 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bench_256cube_pcg_traffic.json")
 PROF_KINDS = ("fine_gs", "fine_gs_zero", "fine_apply", "fine_residual")
 # rocprofv3 names of the kernel behind each timed kind (demangled, or mangled for the packed-FP16
 # instantiations), for the traffic lookup
-PROF_KERNEL = {"fine_gs": ("k_fine_gs<6, true, -1", "k_fine_gsILi6ELb1ELin1E"),
+PROF_KERNEL = {"fine_gs": ("k_fine_gs2<6, true, -1", "k_fine_gs2ILi6ELb1ELin1E", "k_fine_gs<6, true, -1",
+                           "k_fine_gsILi6ELb1ELin1E"),
                "fine_apply": ("k_fine_apply<6, 0, true", "k_fine_applyILi6ELi0ELb1E"),
                "fine_residual": ("k_fine_apply<6, 1, false", "k_fine_applyILi6ELi1ELb0E")}
 METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; stencil GB/s"
@@ -34,6 +35,13 @@ METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; ste
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def gs_colours_per_launch(n):
+    """Colours per finest-level GS launch: two when whole grid rows of a parity block fit a 256-thread
+    workgroup (the library fuses colour pairs then), else one."""
+    h = n // 2
+    return 2 if (n % 2 == 0 and h <= 256 and 256 % h == 0 and h ** 3 % 256 == 0) else 1
 
 
 def kernel_bytes(name, n, R=6, s_z=2, s_f=4, s_w=4):
@@ -44,8 +52,8 @@ def kernel_bytes(name, n, R=6, s_z=2, s_f=4, s_w=4):
         return N * (3 * R * (s_f + s_f) + s_w)
     if name == "fine_residual":   # r = f - K z: read z, f, write r, read w
         return N * (3 * R * (s_z + s_f + s_f) + s_w)
-    if name == "fine_gs":         # one colour = 1/8 of an in-place sweep: z read+write, f read, w read
-        return N * (3 * R * (2 * s_z + s_f) + s_w) / 8.0
+    if name == "fine_gs":         # c colours = c/8 of an in-place sweep: z read+write, f read, w read
+        return N * (3 * R * (2 * s_z + s_f) + s_w) * gs_colours_per_launch(n) / 8.0
     return None
 
 

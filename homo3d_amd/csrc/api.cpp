@@ -305,11 +305,27 @@ void prof_end(homo3d_handle* h, ProfEvent& ev) {
 // One 8-colour sweep.  zero_start: u is taken as zero on entry whatever it holds (forward sweeps
 // only); every colour pass then writes its block without reading later colours.
 // u_half: the level-0 u is packed FP16 (coarse levels are always FP32).
-void op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, int reverse, bool zero_start = false,
-               int u_half = 0) {
+// dot_out (level 0, reverse sweeps of a packed-FP16 u with fused colour pairs; returns false otherwise):
+// also the per-load-case f . u after the sweep, accumulated by the pair kernels as each vertex gets its
+// final value.
+bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, int reverse, bool zero_start = false,
+               int u_half = 0, double* dot_out = nullptr) {
     const Level& L = h->lv[l];
     if (l == 0 && !zero_start) launch_zero_pin(h->stream, L.N, u, u_half);
     const int zs = zero_start && !reverse ? 1 : 0;
+    if (l == 0 && fine_gs_pairs_ok(L.n)) {  // colours two at a time (same result as one by one)
+        const bool dot = dot_out && reverse && u_half;
+        const int G = (int)(L.N / 8 / 256);  // workgroups per pair launch
+        for (int cc = 0; cc < 8; cc += 2) {
+            ProfEvent ev;
+            prof_begin(h, zs ? kProfGsZero : kProfGs, ev);
+            launch_fine_gs2(h->stream, L.n, reverse ? 7 - cc : cc, omega, reverse, zs, h->d_ke, h->d_w, u, u_half, f,
+                            dot ? h->d_partial + (size_t)(cc / 2) * G : nullptr, 4 * G);
+            prof_end(h, ev);
+        }
+        if (dot) launch_sum_partials(h->stream, kRhs, 4 * G, h->d_partial, dot_out);
+        return dot;
+    }
     for (int cc = 0; cc < 8; cc++) {
         const int c = reverse ? 7 - cc : cc;
         if (l == 0) {
@@ -321,6 +337,7 @@ void op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
             launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, L.st_half, static_cast<float*>(u), f);
         }
     }
+    return false;
 }
 
 void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* r, int u_half = 0) {
@@ -338,9 +355,10 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
 // One V(1,1) cycle on the level-0 pair (u0, f0) (MG::v_cycle, MG.cpp:46-70).  When rn2 is given it
 // receives, per load case, the squared norm of the level-0 residual after pre-smoothing.  u0_zero:
 // the cycle starts from u0 = 0 (u0's contents are ignored), as the coarse levels always do.
-// u0_half: u0 is packed FP16 (level 0 only; every other field is FP32).
+// u0_half: u0 is packed FP16 (level 0 only; every other field is FP32).  dot_out: per-load-case
+// f0 . u0 of the result (the r . z of a CG step).
 void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
-            int u0_half = 0) {
+            int u0_half = 0, double* dot_out = nullptr) {
     const float omega = (float)h->cfg.sor_omega;
     for (int l = 0; l < h->nl - 1; l++) {
         Level& L = h->lv[l];
@@ -360,7 +378,8 @@ void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* 
         const float* f = l == 0 ? f0 : L.f;
         const int uh = l == 0 ? u0_half : 0;
         launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, u, uh, l == 0 ? 1 : 0);
-        op_smooth(h, l, u, f, omega, symmetric ? 1 : 0, false, uh);
+        const bool fused = op_smooth(h, l, u, f, omega, symmetric ? 1 : 0, false, uh, l == 0 ? dot_out : nullptr);
+        if (l == 0 && dot_out && !fused) launch_dot_rz(h->stream, 3 * L.N, f0, u0, uh, h->d_partial, kDotBlocks, dot_out);
     }
 }
 
@@ -390,6 +409,8 @@ int stage_in(homo3d_handle* h, const float* a, int pitch, int on_device, const f
 }
 
 constexpr int kMaxOuter = 12;
+// Inner solve target per refinement round (measured at 256^3: 1e-4 and 1e-6 both cost more CG steps
+// than they save in FP64 residual rounds).
 constexpr double kInnerTol = 1e-3;
 
 // MG-preconditioned CG on K x = r for the load cases not marked done: x (h->X) starts at zero, r
@@ -415,9 +436,8 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     // the preconditioner's level-0 iterate z in packed FP16 when enabled and the layout allows it (p, q, r
     // and x stay FP32: an FP16 search direction costs more CG steps than it saves)
     const int zh = (h->cfg.precond_half && L0.n % 8 == 0) ? 1 : 0;
-    vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh);
+    vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc);  // with rz
     HIPCHK(h, launch_set_p(h->stream, M, h->Z, zh, h->P, kDotBlocks));
-    launch_dot_rz(h->stream, M, h->Rr, h->Z, zh, h->d_partial, kDotBlocks, h->d_sc);  // rz
     // One CG step: q = K p with p.q, formed as K z + beta q_old since p = z + beta p_old; x is advanced by
     // alpha p one step late, fused with the update of p (and by a last x-only pass on exit), so that p,
     // x, r and q are each streamed once per step.
@@ -453,8 +473,7 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         if (changed)
             HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                                      h->stream));
-        vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh);
-        launch_dot_rz(h->stream, M, h->Rr, h->Z, zh, h->d_partial, kDotBlocks, h->d_sc + 3 * kRhs);  // rz_new
+        vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc + 3 * kRhs);  // with rz_new
         launch_pcg_beta(h->stream, h->d_sc);
         launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, zh, h->P, h->X, 1, kDotBlocks);
         x_pending = false;

@@ -19,6 +19,14 @@ void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w
 // zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
                     const float* w, void* u, int u_half, const float* f);
+// Colours cfirst and cfirst + 1 (cfirst even; REVERSE: cfirst and cfirst - 1, cfirst odd) in one launch,
+// identical to two launch_fine_gs calls; only when fine_gs_pairs_ok(n).  With partial (reverse sweeps of
+// a packed-FP16 u only), each of the (n/2)^3 / 256 workgroups also writes its FP64 sums of f . u_new per
+// load case to partial[r * pstride + workgroup].
+int fine_gs_pairs_ok(int n);
+void launch_fine_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse, int zero_start, const float* ke,
+                     const float* w, void* u, int u_half, const float* f, double* partial = nullptr,
+                     int pstride = 0);
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f);
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
                             const double* u, float* r32, double* partial, double* out6, int do_f, int do_k);

@@ -222,6 +222,32 @@ __device__ __forceinline__ void load_nb(A (&dst)[W][3], Rsrc ru, uint32_t qo, ui
         for (int i = 0; i < 3; i++) dst[p][i] = bload<A>(ru, qo, (uint32_t)((r0 / cpw + p) * 3 + i) * plane);
 }
 
+// The two x-neighbours (d = 12, 14) of a vertex taken from LDS instead of memory: `base` holds 3W word
+// planes of 256 slots (one per thread of the workgroup), slot[0] / slot[1] the x-1 / x+1 neighbour.
+template <typename A>
+struct XNbLds {
+    const A* base = nullptr;
+    int slot[2] = {0, 0};
+};
+
+template <int W, typename A>
+__device__ __forceinline__ void load_nb_lds(A (&dst)[W][3], const XNbLds<A>& xl, int k) {
+#pragma unroll
+    for (int p = 0; p < W; p++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) dst[p][i] = xl.base[(p * 3 + i) * 256 + xl.slot[k]];
+}
+
+// the stored words of neighbour d: from memory, or from LDS for the x-neighbours when XL
+template <int W, typename A, bool XL, int D>
+__device__ __forceinline__ void load_nb_at(A (&dst)[W][3], const FineCtx& c, Rsrc ru, uint32_t plane, int r0,
+                                           const XNbLds<A>& xl) {
+    if constexpr (XL && (D == 12 || D == 14))
+        load_nb_lds<W, A>(dst, xl, D == 12 ? 0 : 1);
+    else
+        load_nb<W, A>(dst, ru, c.nb(D) * (uint32_t)sizeof(A), plane, r0);
+}
+
 // acc[r] += S u_r for the load cases of one stored word triple
 template <int W, typename T, typename A>
 __device__ __forceinline__ void block_fma(const T (&S)[9], const A (&cur)[W][3], T (&acc)[W * CasesPerWord<A>::value][3]) {
@@ -247,30 +273,32 @@ __device__ __forceinline__ void block_fma(const T (&S)[9], const A (&cur)[W][3],
 
 // One neighbour of the row: issue the loads of the next neighbour, build the 3x3 block of this one
 // from the cell weights and KE (in LDS), accumulate, recurse.
-template <int R, uint32_t MASK, int D, typename T, typename A>
+template <int R, uint32_t MASK, int D, typename T, typename A, bool XL>
 __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke, Rsrc ru, uint32_t plane, int r0,
-                                            const A (&cur)[R / CasesPerWord<A>::value][3], T (&acc)[R][3]) {
+                                            const A (&cur)[R / CasesPerWord<A>::value][3], T (&acc)[R][3],
+                                            const XNbLds<A>& xl) {
     constexpr int W = R / CasesPerWord<A>::value;
     constexpr int DN = next_nb(MASK, D);
     // keep each neighbour's block build next to its use (bounds register pressure)
     __builtin_amdgcn_sched_barrier(0);
     A nxt[W][3];
-    if constexpr (DN < 27) load_nb<W, A>(nxt, ru, c.nb(DN) * (uint32_t)sizeof(A), plane, r0);
+    if constexpr (DN < 27) load_nb_at<W, A, XL, DN>(nxt, c, ru, plane, r0, xl);
     T S[9];
     if constexpr (sizeof(T) == 8)
         fine_block<D>(c.wd, ke, S);
     else
         fine_block<D>(c.wc, ke, S);
     block_fma<W, T, A>(S, cur, acc);
-    if constexpr (DN < 27) gather_step<R, MASK, DN, T, A>(c, n, ke, ru, plane, r0, nxt, acc);
+    if constexpr (DN < 27) gather_step<R, MASK, DN, T, A, XL>(c, n, ke, ru, plane, r0, nxt, acc, xl);
 }
 
 // acc[r][i] = sum_d S_d u_{v+d} over the neighbours d selected by MASK, load cases r0..r0+R-1.
 // ke points to the element matrix staged in LDS.  The pinned origin's entries of u are zero by
-// construction (every entry point zeroes them before the fine operator reads u).
-template <int R, uint32_t MASK, typename T, typename A>
+// construction (every entry point zeroes them before the fine operator reads u).  XL: the two
+// x-neighbours come from LDS (xl).
+template <int R, uint32_t MASK, typename T, typename A, bool XL = false>
 __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, const T* ke, const A* __restrict__ u,
-                                            T (&acc)[R][3], int r0 = 0) {
+                                            T (&acc)[R][3], int r0 = 0, const XNbLds<A>& xl = XNbLds<A>()) {
     constexpr int W = R / CasesPerWord<A>::value;
     const Rsrc ru = make_rsrc(u, (size_t)(18 / CasesPerWord<A>::value) * N * sizeof(A));
     const uint32_t plane = (uint32_t)(N * sizeof(A));
@@ -281,8 +309,8 @@ __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, c
     constexpr int D0 = next_nb(MASK, -1);
     if constexpr (D0 < 27) {
         A first[W][3];
-        load_nb<W, A>(first, ru, c.nb(D0) * (uint32_t)sizeof(A), plane, r0);
-        gather_step<R, MASK, D0, T, A>(c, n, ke, ru, plane, r0, first, acc);
+        load_nb_at<W, A, XL, D0>(first, c, ru, plane, r0, xl);
+        gather_step<R, MASK, D0, T, A, XL>(c, n, ke, ru, plane, r0, first, acc, xl);
     }
 }
 
@@ -377,37 +405,29 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
     }
 }
 
-// One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
-// ZC >= 0: colour ZC of a forward sweep that starts from u = 0 (colours above ZC are never read, so
-// u needs no clearing beforehand; the old value of the vertex itself is zero).
-// A: storage of u (float, or half2_t pairs of load cases); f is FP32.
-template <int R, bool REVERSE, int ZC, typename A>
-__global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour, float omega,
-                                                    const float* __restrict__ ke, const float* __restrict__ w,
-                                                    A* __restrict__ u, const float* __restrict__ f) {
+// GS / SOR update of the colour vertex v (load cases r0 .. r0 + R - 1), in place.  A: storage of u
+// (float, or half2_t pairs of load cases); f is FP32.  ZC >= 0: colour ZC of a forward sweep that
+// starts from u = 0 (colours above ZC are never read, so u needs no clearing beforehand; the old value
+// of the vertex itself is zero).  XL: the x-neighbours come from LDS; TO_LDS: the new words are also
+// stored to slot threadIdx.x of out_lds.  DOT: dot[r] += f . u_new over the vertex (stored values;
+// FP32 over the vertex's 3 components, FP64 across vertices).
+// The pinned origin keeps u = 0.
+template <int R, bool REVERSE, int ZC, typename A, bool XL, bool TO_LDS, bool DOT = false>
+__device__ __forceinline__ void gs_vertex(const Lay& L, size_t N, uint32_t v, float omega, const float* ke_s,
+                                          const float* __restrict__ w, A* __restrict__ u,
+                                          const float* __restrict__ f, int r0, const XNbLds<A>& xl, A* out_lds,
+                                          float* dot = nullptr) {
     constexpr int cpw = CasesPerWord<A>::value;
-    __shared__ float ke_s[576];
-    stage_ke(ke_s, ke);
-    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= L.h3) return;
-    const uint32_t v = colour_block(colour) * L.h3 + t;
-    const int r0 = blockIdx.y * R;  // load cases r0 .. r0 + R - 1
-    if (v == 0) {
-#pragma unroll
-        for (int p = 0; p < R / cpw; p++)
-#pragma unroll
-            for (int i = 0; i < 3; i++) u[(size_t)((r0 / cpw + p) * 3 + i) * N] = A(0);
-        return;
-    }
     int x, y, z;
     L.pos(v, x, y, z);
     FineCtx c;
     fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, gs_mask(ZC), float, A>(c, L.n, N, ke_s, u, acc, r0);
+    fine_gather<R, gs_mask(ZC), float, A, XL>(c, L.n, N, ke_s, u, acc, r0, xl);
     float Ks[9];
     fine_block<13, float>(c.wc, ke_s, Ks);
     const float inv[3] = {1.f / Ks[0], 1.f / Ks[4], 1.f / Ks[8]};
+    const bool pin = v == 0;
 #pragma unroll
     for (int p = 0; p < R / cpw; p++) {
         float uu[cpw][3], ff[cpw][3];
@@ -429,10 +449,79 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const size_t o = (size_t)((r0 / cpw + p) * 3 + i) * N + v;
+            A word;
             if constexpr (cpw == 2)
-                u[o] = half2_t{(_Float16)uu[0][i], (_Float16)uu[1][i]};
+                word = half2_t{(_Float16)uu[0][i], (_Float16)uu[1][i]};
             else
-                u[o] = uu[0][i];
+                word = uu[0][i];
+            if (pin) word = A(0);
+            u[o] = word;
+            if constexpr (TO_LDS) out_lds[(p * 3 + i) * 256 + threadIdx.x] = word;
+            if constexpr (DOT) {
+                if constexpr (cpw == 2) {
+                    dot[p * 2] = fmaf(ff[0][i], (float)word.x, dot[p * 2]);
+                    dot[p * 2 + 1] = fmaf(ff[1][i], (float)word.y, dot[p * 2 + 1]);
+                } else {
+                    dot[p] = fmaf(ff[0][i], word, dot[p]);
+                }
+            }
+        }
+    }
+}
+
+// One colour of the fine block GS / SOR sweep, in place (gs_relaxation_otf_kernel, grid.cu:323-551).
+template <int R, bool REVERSE, int ZC, typename A>
+__global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour, float omega,
+                                                    const float* __restrict__ ke, const float* __restrict__ w,
+                                                    A* __restrict__ u, const float* __restrict__ f) {
+    __shared__ float ke_s[576];
+    stage_ke(ke_s, ke);
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= L.h3) return;
+    gs_vertex<R, REVERSE, ZC, A, false, false>(L, N, colour_block(colour) * L.h3 + t, omega, ke_s, w, u, f,
+                                              blockIdx.y * R, XNbLds<A>(), nullptr);
+}
+
+// Two consecutive colours of a sweep in one launch: cfirst, then cfirst + 1 (forward) or cfirst - 1
+// (REVERSE), cfirst even resp. odd.  DOT: per-workgroup FP64 sums of f . u_new over the pair's vertices
+// per load case to partial[r * pstride + blockIdx.x] (over the four pair launches of a sweep that is
+// the last to write u, this is the r . z of the CG step).  The two colours differ only in x-parity, and a vertex of the
+// second depends on the first only through its x-neighbours, which lie in the same grid row.  Each
+// workgroup owns whole rows (256 % (n/2) == 0, (n/2)^3 % 256 == 0, checked by the launcher): it
+// updates the first colour of its rows, keeps the new values in LDS, and updates the second colour
+// reading those x-neighbours from LDS.  Other workgroups never read the rows' first-colour values
+// before they are final, so the result equals the two colour passes in sequence.
+template <int R, bool REVERSE, int ZC, typename A, bool DOT = false>
+__global__ __launch_bounds__(256, 4) void k_fine_gs2(Lay L, size_t N, int cfirst, float omega,
+                                                     const float* __restrict__ ke, const float* __restrict__ w,
+                                                     A* __restrict__ u, const float* __restrict__ f,
+                                                     double* __restrict__ partial, int pstride) {
+    constexpr int W = R / CasesPerWord<A>::value;
+    __shared__ float ke_s[576];
+    __shared__ A xs[3 * W * 256];
+    stage_ke(ke_s, ke);
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+    const int c2 = REVERSE ? cfirst - 1 : cfirst + 1;
+    float dot[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) dot[r] = 0.f;
+    gs_vertex<R, REVERSE, ZC, A, false, true, DOT>(L, N, colour_block(cfirst) * L.h3 + t, omega, ke_s, w, u, f, 0,
+                                                  XNbLds<A>(), xs, dot);
+    __syncthreads();
+    XNbLds<A> xl;
+    xl.base = xs;
+    const int h = L.h, lx = (int)(t % (uint32_t)h), row0 = (int)threadIdx.x - lx;
+    const bool odd = (colour_block(c2) & 1u) != 0;  // x-parity of the second colour
+    xl.slot[0] = row0 + (odd ? lx : (lx == 0 ? h - 1 : lx - 1));
+    xl.slot[1] = row0 + (odd ? (lx + 1 == h ? 0 : lx + 1) : lx);
+    gs_vertex<R, REVERSE, (ZC < 0 ? -1 : ZC + 1), A, true, false, DOT>(L, N, colour_block(c2) * L.h3 + t, omega, ke_s,
+                                                                       w, u, f, 0, xl, nullptr, dot);
+    if constexpr (DOT) {
+        __shared__ double sh[4];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double sdot = block_sum<256>((double)dot[r], sh);
+            if (threadIdx.x == 0) partial[(size_t)r * pstride + blockIdx.x] = sdot;
         }
     }
 }
@@ -1627,6 +1716,51 @@ void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, 
         fine_gs_pick(s, L, N, colour, omega, reverse, zero_start, ke, w, static_cast<half2_t*>(u), f);
     else
         fine_gs_pick(s, L, N, colour, omega, reverse, zero_start, ke, w, static_cast<float*>(u), f);
+}
+
+// fused colour pairs need every workgroup to own whole rows of one parity block
+int fine_gs_pairs_ok(int n) {
+    const int h = n / 2;
+    const long h3 = (long)h * h * h;
+    return n % 2 == 0 && h <= 256 && 256 % h == 0 && h3 % 256 == 0;
+}
+
+template <bool REV, int ZC, typename A, bool DOT = false>
+static void fine_gs2_pair(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, const float* ke,
+                          const float* w, A* u, const float* f, double* partial = nullptr, int pstride = 0) {
+    hipLaunchKernelGGL((k_fine_gs2<6, REV, ZC, A, DOT>), dim3(L.h3 / 256), dim3(256), 0, s, L, N, cfirst, omega, ke, w,
+                       u, f, partial, pstride);
+}
+
+template <typename A>
+static void fine_gs2_pick(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, int reverse,
+                          int zero_start, const float* ke, const float* w, A* u, const float* f) {
+    if (reverse) {
+        fine_gs2_pair<true, -1>(s, L, N, cfirst, omega, ke, w, u, f);
+        return;
+    }
+    switch (zero_start ? cfirst : -1) {
+        case 0: fine_gs2_pair<false, 0>(s, L, N, cfirst, omega, ke, w, u, f); break;
+        case 2: fine_gs2_pair<false, 2>(s, L, N, cfirst, omega, ke, w, u, f); break;
+        case 4: fine_gs2_pair<false, 4>(s, L, N, cfirst, omega, ke, w, u, f); break;
+        case 6: fine_gs2_pair<false, 6>(s, L, N, cfirst, omega, ke, w, u, f); break;
+        default: fine_gs2_pair<false, -1>(s, L, N, cfirst, omega, ke, w, u, f); break;
+    }
+}
+
+void launch_fine_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse, int zero_start, const float* ke,
+                     const float* w, void* u, int u_half, const float* f, double* partial, int pstride) {
+    const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
+    if (partial && reverse && u_half) {  // f . u_new per workgroup (the r . z of a CG step)
+        fine_gs2_pair<true, -1, half2_t, true>(s, L, N, cfirst, omega, ke, w, static_cast<half2_t*>(u), f, partial,
+                                               pstride);
+        return;
+    }
+    if (u_half)
+        fine_gs2_pick(s, L, N, cfirst, omega, reverse, zero_start, ke, w, static_cast<half2_t*>(u), f);
+    else
+        fine_gs2_pick(s, L, N, cfirst, omega, reverse, zero_start, ke, w, static_cast<float*>(u), f);
 }
 
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {
