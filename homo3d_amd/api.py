@@ -25,7 +25,8 @@ class _Config(ctypes.Structure):
     _fields_ = [("reso", ctypes.c_int * 3), ("youngs_modulus", ctypes.c_double), ("poisson_ratio", ctypes.c_double),
                 ("emin", ctypes.c_double), ("fem_rel_thres", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("solver", ctypes.c_int), ("device", ctypes.c_int), ("sor_omega", ctypes.c_double),
-                ("diag_clamp", ctypes.c_double), ("precond_half", ctypes.c_int), ("coarse_half", ctypes.c_int)]
+                ("diag_clamp", ctypes.c_double), ("precond_half", ctypes.c_int), ("coarse_half", ctypes.c_int),
+                ("fine_mfma", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -51,6 +52,8 @@ EXPORTS = {
     "homo3d_macro_rhs": (ctypes.c_int, [ctypes.c_void_p]),
     "homo3d_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_apply_K": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_apply_K16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_smooth16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
     "homo3d_smooth": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "homo3d_restrict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_prolong": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -116,7 +119,7 @@ class HomoConfig:
 
     def __init__(self, reso=64, youngsModulu=10.0, poissonRatio=0.3, femRelThres=1e-2, max_iter=200,
                  solver="mg", emin=1e-9, device=0, sor_omega=1.0, diag_clamp=1e-4, precond_half=True,
-                 coarse_half=True, **kw):
+                 coarse_half=True, fine_mfma=False, **kw):
         self.reso = [reso] * 3 if np.isscalar(reso) else list(reso)
         self.youngsModulu = youngsModulu
         self.poissonRatio = poissonRatio
@@ -129,6 +132,7 @@ class HomoConfig:
         self.diag_clamp = diag_clamp
         self.precond_half = precond_half
         self.coarse_half = coarse_half
+        self.fine_mfma = fine_mfma
         # optimisation-side fields of cfg::HomoConfig (kept for drivers)
         self.volRatio = kw.get("volRatio", 0.3)
         self.filterRadius = kw.get("filterRadius", 2.0)
@@ -153,6 +157,7 @@ class HomoConfig:
         c.diag_clamp = self.diag_clamp
         c.precond_half = 1 if self.precond_half else 0
         c.coarse_half = 1 if self.coarse_half else 0
+        c.fine_mfma = 1 if self.fine_mfma else 0
         return c
 
 
@@ -267,6 +272,14 @@ class Homogenization:
 
     def apply_K(self):
         self._chk(lib().homo3d_apply_K(self._h), "apply_K")
+
+    def apply_K16(self, residual=False):
+        """r[0] = K fp16(u[0]) (or f - K fp16(u[0])) with the packed-FP16 finest-level operator."""
+        self._chk(lib().homo3d_apply_K16(self._h, int(residual)), "apply_K16")
+
+    def smooth16(self, omega=1.0, reverse=False):
+        """One finest-level sweep on fp16(u[0]) with FP16 storage; u[0] receives the result."""
+        self._chk(lib().homo3d_smooth16(self._h, float(omega), int(reverse)), "smooth16")
 
     def smooth(self, level=0, omega=1.0, reverse=False):
         self._chk(lib().homo3d_smooth(self._h, level, float(omega), int(reverse)), "smooth")

@@ -55,6 +55,9 @@ struct homo3d_handle {
     Level lv[kMaxLevels];
     hipStream_t stream = nullptr;
     float* d_ke = nullptr;
+    void* d_ke16 = nullptr;     // element matrix as matrix-core operands (ke_mfma_table), or null
+    FineMfma mfma{};            // its launch parameters
+    const FineMfma* mf() const { return d_ke16 ? &mfma : nullptr; }
     float* d_kchi = nullptr;
     float* d_G = nullptr;
     float* d_w = nullptr;
@@ -113,8 +116,7 @@ int set_err(homo3d_handle* h, int code, const char* fmt, ...) {
 // ---------------------------------------------------------------------------------------------
 // Element matrix (templateMatrix.cpp:63-166): 2x2x2 Gauss quadrature of B^T C B on the unit hex.
 // ---------------------------------------------------------------------------------------------
-void element_matrix(double E, double nu, double* ke) {
-    const double lam = nu * E / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+void element_matrix_lame(double lam, double mu, double* ke) {
     double C[6][6] = {};
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) C[i][j] = lam;
@@ -149,6 +151,31 @@ void element_matrix(double E, double nu, double* ke) {
     }
     for (int i = 0; i < 24; i++)
         for (int j = 0; j < 24; j++) ke[i * 24 + j] = 0.5 * (K[i * 24 + j] + K[j * 24 + i]);
+}
+
+void lame_params(double E, double nu, double& lam, double& mu) {
+    lam = nu * E / ((1 + nu) * (1 - 2 * nu));
+    mu = E / (2 * (1 + nu));
+}
+
+void element_matrix(double E, double nu, double* ke) {
+    double lam, mu;
+    lame_params(E, nu, lam, mu);
+    element_matrix_lame(lam, mu, ke);
+}
+
+// The Lame tables 72 KE(lambda = 1, mu = 0) and 72 KE(0, 1) (templateMatrix.cpp:8-61): small integers,
+// recovered from the quadrature above.  False if an entry is not within 1e-9 of an integer.
+bool lame_tables(double* kl, double* km) {
+    element_matrix_lame(72.0, 0.0, kl);
+    element_matrix_lame(0.0, 72.0, km);
+    for (int i = 0; i < 576; i++) {
+        const double a = std::nearbyint(kl[i]), b = std::nearbyint(km[i]);
+        if (std::fabs(a - kl[i]) > 1e-9 || std::fabs(b - km[i]) > 1e-9) return false;
+        kl[i] = a;
+        km[i] = b;
+    }
+    return true;
 }
 
 double macro_disp(int node, int s, int c) {
@@ -345,7 +372,7 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
     if (l == 0) {
         ProfEvent ev;
         prof_begin(h, kProfResidual, ev);
-        launch_fine_apply(h->stream, L.n, h->d_ke, h->d_w, u, u_half, f, r);
+        launch_fine_apply(h->stream, L.n, h->d_ke, h->mf(), h->d_w, u, u_half, f, r);
         prof_end(h, ev);
     } else {
         launch_stencil_apply(h->stream, L.n, L.lex, L.st, L.st_half, static_cast<const float*>(u), f, r);
@@ -448,8 +475,9 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
             ProfEvent ev;
             prof_begin(h, kProfApply, ev);
             // q = K p as K z (+ beta q_old): the operator gathers the packed z instead of the FP32 p
-            launch_fine_apply_dot(h->stream, L0.n, h->d_ke, h->d_w, h->Z, zh, h->P, it ? h->d_sc + 4 * kRhs : nullptr,
-                                  h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
+            launch_fine_apply_dot(h->stream, L0.n, h->d_ke, h->mf(), h->d_w, h->Z, zh, h->P,
+                                  it ? h->d_sc + 4 * kRhs : nullptr, h->Q, h->d_partial, kDotBlocks,
+                                  h->d_sc + kRhs);  // pq
             prof_end(h, ev);
         }
         launch_pcg_alpha(h->stream, h->d_sc);
@@ -524,6 +552,7 @@ void homo3d_default_config(homo3d_config* c) {
     c->diag_clamp = 1e-4;
     c->precond_half = 1;
     c->coarse_half = 1;
+    c->fine_mfma = 0;
 }
 
 int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
@@ -633,6 +662,17 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
         hipMalloc(&h->d_G, sizeof(float) * G.size()) || hipMalloc(&h->d_gperm, sizeof(int) * 48) ||
         hipMalloc(&h->d_gflip, sizeof(int) * 48))
         return fail(HOMO3D_HIP_ERROR);
+    double kl[576], km[576];
+    if (cfg->fine_mfma && lame_tables(kl, km)) {
+        std::vector<unsigned char> t16(kKeMfmaBytes);
+        double lam, mu;
+        lame_params(cfg->youngs_modulus, cfg->poisson_ratio, lam, mu);
+        h->mfma.nt = ke_mfma_table(kl, km, lam, mu, t16.data(), &h->mfma.c0, &h->mfma.c1);
+        if (hipMalloc(&h->d_ke16, kKeMfmaBytes) ||
+            hipMemcpy(h->d_ke16, t16.data(), kKeMfmaBytes, hipMemcpyHostToDevice))
+            return fail(HOMO3D_HIP_ERROR);
+        h->mfma.table = h->d_ke16;
+    }
     if (hipMemcpy(h->d_ke, kef.data(), sizeof(float) * 576, hipMemcpyHostToDevice) ||
         hipMemcpy(h->d_kchi, kchi.data(), sizeof(float) * 144, hipMemcpyHostToDevice) ||
         hipMemcpy(h->d_G, G.data(), sizeof(float) * G.size(), hipMemcpyHostToDevice))
@@ -656,7 +696,7 @@ int homo3d_destroy(homo3d_handle* h) {
         (void)hipFree(h->lv[l].r);
         (void)hipFree(h->lv[l].st);
     }
-    void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->X, h->F, h->P, h->Q, h->Z,
+    void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_ke16, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->X, h->F, h->P, h->Q, h->Z,
                     h->Rr, h->d_partial, h->d_sc, h->d_tmp, h->d_ftmp, h->d_gperm, h->d_gflip};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -975,8 +1015,46 @@ int homo3d_apply_K(homo3d_handle* h) {
     launch_zero_pin(h->stream, h->lv[0].N, h->X, 0);
     ProfEvent ev;
     prof_begin(h, kProfApply, ev);
-    launch_fine_apply(h->stream, h->lv[0].n, h->d_ke, h->d_w, h->X, 0, nullptr, h->lv[0].r);
+    launch_fine_apply(h->stream, h->lv[0].n, h->d_ke, nullptr, h->d_w, h->X, 0, nullptr, h->lv[0].r);
     prof_end(h, ev);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+// Finest-level operator and smoother on packed-FP16 storage (the form the PCG preconditioner runs),
+// on an FP16-rounded copy of the level-0 u held in the handle's z buffer.
+static int fp16_ready(homo3d_handle* h) {
+    if (!h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    if (h->lv[0].n % 8) return set_err(h, HOMO3D_NO_SUPPORT, "packed FP16 fields need n %% 8 == 0 (n = %d)", h->lv[0].n);
+    return HOMO3D_OK;
+}
+
+int homo3d_apply_K16(homo3d_handle* h, int residual) {
+    int rc = fp16_ready(h);
+    if (rc) return rc;
+    const Level& L = h->lv[0];
+    launch_pack_half(h->stream, L.N, h->X, h->Z);
+    launch_zero_pin(h->stream, L.N, h->Z, 1);
+    ProfEvent ev;
+    prof_begin(h, residual ? kProfResidual : kProfApply, ev);
+    launch_fine_apply(h->stream, L.n, h->d_ke, h->mf(), h->d_w, h->Z, 1, residual ? h->F : nullptr,
+                      L.r);
+    prof_end(h, ev);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_smooth16(homo3d_handle* h, double omega, int reverse) {
+    int rc = fp16_ready(h);
+    if (rc) return rc;
+    const Level& L = h->lv[0];
+    launch_pack_half(h->stream, L.N, h->X, h->Z);
+    op_smooth(h, 0, h->Z, h->F, (float)omega, reverse, false, 1);
+    launch_unpack_half(h->stream, L.N, h->Z, h->X);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
     return HOMO3D_OK;

@@ -7,14 +7,28 @@ namespace homo3d {
 
 // Fine-level kernels take u as FP32 planes ([6][3][n^3]) or, with u_half, as packed FP16 load-case
 // pairs ([3][3][n^3] words of two halves, cases 2p and 2p + 1; only with n % 8 == 0).
-// out = K u (f == nullptr) or f - K u
-void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
-                       const float* f, float* out);
+// Element matrix for the matrix-core form of the fine operator, built from the integer Lame tables
+// Kl = 72 KE(1, 0), Km = 72 KE(0, 1): either one FP16-exact table T = (lambda / mu) Kl + Km with
+// KE = c0 T (nt = 1), or Kl and Km with KE = c0 Kl + c1 Km (nt = 2), laid out as the A operands of
+// 16x16x32 tiles (kKeMfmaBytes).  ke_mfma_table returns nt.
+constexpr int kKeMfmaBytes = 3 * 64 * 8 * 2;
+int ke_mfma_table(const double* kl, const double* km, double lam, double mu, void* table, float* c0, float* c1);
+struct FineMfma {
+    const void* table;  // device copy of the ke_mfma_table output
+    float c0, c1;
+    int nt;
+};
+// n for which packed-FP16 fine operators run in element form on the matrix cores
+int fine_apply_mfma_ok(int n);
+// out = K u (f == nullptr) or f - K u.  With mf, packed-FP16 u and fine_apply_mfma_ok(n) the operator
+// runs in element form on the matrix cores.
+void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
+                       int u_half, const float* f, float* out);
 // CG step q = K p with the per-load-case FP64 dot p . q -> out6.  With beta (device, per load case) the
 // operator is applied to u = z and q = K z + beta q_old (p = z + beta p_old); without, u must be p.
 // u is FP32 or packed FP16 (u_half).  partial holds 6 * n^3 / 512 doubles (and at least 6 * dot_blocks).
-void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
-                           const float* p, const double* beta, float* out, double* partial, int dot_blocks,
+void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
+                           int u_half, const float* p, const double* beta, float* out, double* partial, int dot_blocks,
                            double* out6);
 // zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
@@ -36,6 +50,9 @@ void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u);
 void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x);
 void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u);
 void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half);
+// FP32 field [6][3][N] <-> packed FP16 field [3][3][N] words (round to nearest)
+void launch_pack_half(hipStream_t s, size_t N, const float* u, void* z);
+void launch_unpack_half(hipStream_t s, size_t N, const void* z, float* u);
 void launch_zero_pin64(hipStream_t s, size_t N, double* u);
 // Coarse stencils: FP32 [243][n^3] or, with *_half, packed FP16 [122][n^3] words (entries 2w, 2w + 1).
 void launch_stencil_apply(hipStream_t s, int n, int lex, const void* st, int st_half, const float* u, const float* f,

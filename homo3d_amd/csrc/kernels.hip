@@ -13,6 +13,8 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "kernels.h"
 
@@ -405,6 +407,244 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Finest-level operator in element form on the matrix cores (packed-FP16 u, n % 32 == 0)
+// ---------------------------------------------------------------------------------------------
+// K u = sum over cells e of w_e KE u_e (the per-cell form of grid.cu:1277-1461).  A workgroup owns a
+// tile of 32 x 4 x 4 vertices.  For one load-case pair at a time it stages the 34 x 6 x 6 node box
+// of u in LDS, forms KE u_e for the 33 x 5 x 5 cells that touch the tile with
+// v_mfma_f32_16x16x32_f16 and adds w_e KE u_e into the tile's vertices in LDS.  KE enters through its
+// Lame tables, KE = lambda/72 Kl + mu/72 Km with small-integer Kl, Km (templateMatrix.cpp:8-61),
+// which FP16 holds exactly: every product is exact, the sums are FP32, and a constant (rigid
+// translation) field gives exactly zero, as it must; CG on this operator relies on that.
+//
+// Lanes: column c = lane % 16 of a wave is one cell; lane group g = lane / 16 carries the cell's two
+// nodes (ox, g & 1, g >> 1), ox = 0, 1.  B holds their 6 words (3 components x 2 nodes) as K entries
+// 8g .. 8g + 5, and the three 16-row output tiles return to the lane the same 6 rows of Kl u_e and of
+// Km u_e (table layout: ke_mfma_table).  The 16 cells of a group are consecutive cells of one grid
+// row, so the face shared by cells c and c + 1 is summed across lanes (DPP row shift).
+//
+// Sums without atomics: a vertex row (y, z) receives from the four cell rows (ey, ez) in
+// {y-1, y} x {z-1, z}, one of each (ey, ez) parity class.  Each class writes its own LDS plane, and
+// every vertex of a plane is written exactly once per load case: plain stores, no clearing, no
+// ordering between waves.  The few contributions that meet on a plane at the x-seams of the row
+// groups (x = 0, 16: first lane of a group; x = 0 also from the cells at x = -1, which form
+// separate groups) go to small per-class arrays of their own.  The epilogue adds it all up.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kEfTX = 16, kEfTY = 2, kEfTZ = 2;                                     // parity positions
+constexpr int kEfVX = 2 * kEfTX, kEfVY = 2 * kEfTY, kEfVZ = 2 * kEfTZ;              // owned vertices
+constexpr int kEfVerts = kEfVX * kEfVY * kEfVZ;
+constexpr int kEfBX = kEfVX + 2, kEfBY = kEfVY + 2, kEfBZ = kEfVZ + 2;              // node box
+constexpr int kEfNodes = kEfBX * kEfBY * kEfBZ;
+constexpr int kEfCX = kEfVX + 1, kEfCY = kEfVY + 1, kEfCZ = kEfVZ + 1;              // cells, from -1
+constexpr int kEfCells = kEfCX * kEfCY * kEfCZ;
+constexpr int kEfRows = kEfCY * kEfCZ;
+constexpr int kEfRowGroups = kEfVX / 16;                 // groups of cells x = 0 .. VX-1 per row
+constexpr int kEfFull = kEfRows * kEfRowGroups;
+constexpr int kEfGroups = kEfFull + (kEfRows + 15) / 16;  // + groups of the x = -1 cells
+constexpr int kEfYZ = kEfVY * kEfVZ;                      // vertex rows of the tile
+constexpr int kEfThreads = 512;
+constexpr int kEfStage = (3 * kEfNodes + kEfThreads - 1) / kEfThreads;  // staged words per thread and pair
+static_assert(kEfVerts == kEfThreads, "epilogue maps one vertex per thread and plane");
+static_assert(kEfRowGroups == 2, "x-seams at x = 0 and x = 16");
+
+// row shift of a 16-lane row: lane c reads lane c + 1, lane 15 reads 0
+__device__ __forceinline__ float dpp_next(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x101, 0xf, 0xf, true));
+}
+
+// One group of 16 cells and the two load cases of a packed word pair: B operands from the node box,
+// NT + 1 output tiles per case (NT = 1: one exact table T; NT = 2: Kl and Km), and the lane's 6 rows
+// of w_e KE u_e per case in val[case][k] (k = 3 ox + component).
+template <int NT>
+__device__ __forceinline__ void ef_cells(const uint32_t* nodes, int nb0, half8_t a0, half8_t a1, half8_t a2, float w0,
+                                         float w1, float (&val)[2][6]) {
+    half2_t wv[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) wv[k] = __builtin_bit_cast(half2_t, nodes[(k % 3) * kEfNodes + nb0 + k / 3]);
+    half8_t b[2];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        b[0][k] = wv[k].x;
+        b[1][k] = wv[k].y;
+    }
+    b[0][6] = b[0][7] = b[1][6] = b[1][7] = (_Float16)0.f;
+    const float4_t zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++) {
+        const float4_t d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b[hh], zero, 0, 0, 0);
+        const float4_t d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b[hh], zero, 0, 0, 0);
+        if constexpr (NT == 1) {
+            // rows k = 0..3 in tile 0, k = 4, 5 in tile 1
+            val[hh][0] = w0 * d0[0];
+            val[hh][1] = w0 * d0[1];
+            val[hh][2] = w0 * d0[2];
+            val[hh][3] = w0 * d0[3];
+            val[hh][4] = w0 * d1[0];
+            val[hh][5] = w0 * d1[1];
+        } else {
+            // Kl rows in tiles 0/1, Km rows in tiles 1/2
+            const float4_t d2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2, b[hh], zero, 0, 0, 0);
+            val[hh][0] = fmaf(w0, d0[0], w1 * d1[2]);
+            val[hh][1] = fmaf(w0, d0[1], w1 * d1[3]);
+            val[hh][2] = fmaf(w0, d0[2], w1 * d2[0]);
+            val[hh][3] = fmaf(w0, d0[3], w1 * d2[1]);
+            val[hh][4] = fmaf(w0, d1[0], w1 * d2[2]);
+            val[hh][5] = fmaf(w0, d1[1], w1 * d2[3]);
+        }
+    }
+}
+
+// MODE 0: out = K u (BETA: + beta out_old); MODE 1: out = f - K u.  DOT (MODE 0): per-workgroup FP64
+// partial sums of dotv . out per load case in partial[r * gridDim.x + block].  Pinned origin row zero.
+// KE = c0 T (NT = 1) or c0 Kl + c1 Km (NT = 2).  Needs 72 n^3 < 2^32 (32-bit buffer offsets).
+template <int MODE, bool DOT, bool BETA, int NT>
+__global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, const half8_t* __restrict__ amat,
+                                                              float c0, float c1, const float* __restrict__ w,
+                                                              const uint32_t* __restrict__ u,
+                                                              const float* __restrict__ f, float* __restrict__ out,
+                                                              double* __restrict__ partial,
+                                                              const float* __restrict__ dotv,
+                                                              const double* __restrict__ beta) {
+    __shared__ uint32_t nodes[3 * kEfNodes];       // packed words of the node box, one load-case pair
+    __shared__ float part[4][6][kEfVerts];         // per cell-row class, (case of the pair, component)
+    __shared__ float seam[4][6][2][kEfYZ];         // x = 0 / 16 contributions of the first lanes
+    __shared__ float halo[4][6][kEfYZ];            // x = 0 contributions of the cells at x = -1
+    __shared__ float wcell[kEfCells];              // w_e of the cells from -1
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, col = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: group loops stay scalar
+    const int gy = g & 1, gz = g >> 1;
+    const uint32_t wg = xcd_chunk(blockIdx.x, gridDim.x);
+    const uint32_t ntx = (uint32_t)(L.h / kEfTX), nty = (uint32_t)(L.h / kEfTY);
+    const int x0 = kEfVX * (int)(wg % ntx), y0 = kEfVY * (int)((wg / ntx) % nty), z0 = kEfVZ * (int)(wg / (ntx * nty));
+    const half8_t a0 = amat[lane], a1 = amat[64 + lane], a2 = NT == 2 ? amat[128 + lane] : a1;
+    const Rsrc ru = make_rsrc(u, 9 * N * sizeof(uint32_t));
+    const Rsrc ro = make_rsrc(out, 18 * N * sizeof(float));
+    const Rsrc rf = make_rsrc(f, f ? 18 * N * sizeof(float) : 0);
+    const Rsrc rd = make_rsrc(dotv, dotv ? 18 * N * sizeof(float) : 0);
+    for (int c = tid; c < kEfCells; c += kEfThreads) {
+        const int ex = c % kEfCX - 1, ey = (c / kEfCX) % kEfCY - 1, ez = c / (kEfCX * kEfCY) - 1;
+        wcell[c] = w[L.idx(L.wrap(x0 + ex), L.wrap(y0 + ey), L.wrap(z0 + ez))];
+    }
+    // node-box words of this thread: byte offsets within load-case pair 0 (~0u: none).  The next
+    // pair's words are fetched into registers before the current pair is computed.
+    uint32_t soff[kEfStage], pre[kEfStage];
+#pragma unroll
+    for (int k = 0; k < kEfStage; k++) {
+        const int s = tid + k * kEfThreads, i = s / kEfNodes, b = s - i * kEfNodes;
+        const int bx = b % kEfBX, by = (b / kEfBX) % kEfBY, bz = b / (kEfBX * kEfBY);
+        soff[k] = s < 3 * kEfNodes
+                      ? 4u * ((uint32_t)i * (uint32_t)N + L.idx(L.wrap(x0 - 1 + bx), L.wrap(y0 - 1 + by), L.wrap(z0 - 1 + bz)))
+                      : ~0u;
+    }
+    const uint32_t pair_bytes = (uint32_t)(3 * N * sizeof(uint32_t));
+    auto fetch = [&](int p) {
+#pragma unroll
+        for (int k = 0; k < kEfStage; k++)
+            if (soff[k] != ~0u)
+                pre[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(ru, (int)soff[k], (int)(p * pair_bytes), 0);
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int k = 0; k < kEfStage; k++)
+            if (soff[k] != ~0u) nodes[tid + k * kEfThreads] = pre[k];
+    };
+    fetch(0);
+    put();
+    __syncthreads();
+    // epilogue vertex of this thread
+    const int vx = tid % kEfVX, vyz = tid / kEfVX;
+    const uint32_t vg = L.idx(x0 + vx, y0 + vyz % kEfVY, z0 + vyz / kEfVY), vb = 4u * vg;
+    const uint32_t plane_bytes = (uint32_t)(N * sizeof(float));
+    // lane offsets of the full-row groups: node (0, gy, gz) of cell (col, 0, 0) in the node box; vertex
+    // (col + 1, gy, gz) in the tile
+    const int lane_nb = (gz * kEfBY + gy) * kEfBX + col, lane_v = (gz * kEfVY + gy) * kEfVX + col;
+    double dot[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        if (p < 2) fetch(p + 1);
+        // epilogue inputs of this pair, in flight during the cell products
+        float fin[6], oin[6], din[6];
+#pragma unroll
+        for (int pl = 0; pl < 6; pl++) {
+            const uint32_t so = (uint32_t)(6 * p + pl) * plane_bytes;
+            if constexpr (MODE == 1) fin[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rf, (int)vb, (int)so, 0));
+            if constexpr (BETA) oin[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ro, (int)vb, (int)so, 0));
+            if constexpr (DOT) din[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rd, (int)vb, (int)so, 0));
+        }
+        // cells x = 0 .. VX-1 of every cell row, 16 per group
+        for (int gi = wave; gi < kEfFull; gi += kEfThreads / 64) {
+            const int row = gi >> 1, half = gi & 1;
+            const int ey = row % kEfCY - 1, ez = row / kEfCY - 1;
+            const int cls = ((ey + 1) & 1) | (((ez + 1) & 1) << 1);
+            const float wc = wcell[((ez + 1) * kEfCY + (ey + 1)) * kEfCX + 16 * half + 1 + col];
+            float val[2][6];
+            ef_cells<NT>(nodes, ((ez + 1) * kEfBY + (ey + 1)) * kEfBX + 16 * half + 1 + lane_nb, a0, a1, a2, wc * c0,
+                         wc * c1, val);
+            const int vy = ey + gy, vz = ez + gz;
+            const bool own = vy >= 0 && vy < kEfVY && vz >= 0 && vz < kEfVZ;
+            const int v1 = (ez * kEfVY + ey) * kEfVX + 16 * half + 1 + lane_v;  // vertex x = 16 half + col + 1
+            const int yz = vz * kEfVY + vy;
+#pragma unroll
+            for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    // node ox = 1 of cell c is node ox = 0 of cell c + 1
+                    const float s1 = val[hh][3 + i] + dpp_next(val[hh][i]);
+                    if (own && (half == 0 || col < 15)) part[cls][hh * 3 + i][v1] = s1;
+                    if (own && col == 0) seam[cls][hh * 3 + i][half][yz] = val[hh][i];
+                }
+        }
+        // cells x = -1 of the cell rows, 16 rows per group
+        for (int gi = kEfFull + wave; gi < kEfGroups; gi += kEfThreads / 64) {
+            const int k = 16 * (gi - kEfFull) + col;
+            const bool valid = k < kEfRows;
+            const int kk = valid ? k : 0, ey = kk % kEfCY - 1, ez = kk / kEfCY - 1;
+            const int cls = ((ey + 1) & 1) | (((ez + 1) & 1) << 1);
+            const float wc = wcell[((ez + 1) * kEfCY + (ey + 1)) * kEfCX];
+            float val[2][6];
+            ef_cells<NT>(nodes, ((ez + 1 + gz) * kEfBY + (ey + 1 + gy)) * kEfBX, a0, a1, a2, wc * c0, wc * c1, val);
+            const int vy = ey + gy, vz = ez + gz;
+            if (valid && vy >= 0 && vy < kEfVY && vz >= 0 && vz < kEfVZ) {
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) halo[cls][hh * 3 + i][vz * kEfVY + vy] = val[hh][3 + i];  // node x = 0
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pl = 0; pl < 6; pl++) {
+            const int r = 2 * p + pl / 3;
+            float val = 0.f;
+            if (vx != 0) val = (part[0][pl][tid] + part[1][pl][tid]) + (part[2][pl][tid] + part[3][pl][tid]);
+            if (vx == 0 || vx == 16) {
+                const int sx = vx >> 4;
+                val += (seam[0][pl][sx][vyz] + seam[1][pl][sx][vyz]) + (seam[2][pl][sx][vyz] + seam[3][pl][sx][vyz]);
+            }
+            if (vx == 0) val += (halo[0][pl][vyz] + halo[1][pl][vyz]) + (halo[2][pl][vyz] + halo[3][pl][vyz]);
+            if constexpr (MODE == 1) val = fin[pl] - val;
+            if constexpr (BETA) val = fmaf((float)beta[r], oin[pl], val);
+            if (vg == 0) val = 0.f;
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val), ro, (int)vb,
+                                                  (int)((uint32_t)(6 * p + pl) * plane_bytes), 0);
+            if constexpr (DOT) dot[r] = fma((double)din[pl], (double)val, dot[r]);
+        }
+        if (p < 2) put();
+        __syncthreads();
+    }
+    if constexpr (DOT) {
+        __shared__ double sh[kEfThreads / 64];
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            const double t = block_sum<kEfThreads>(dot[r], sh);
+            if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = t;
+        }
+    }
+}
+
 // GS / SOR update of the colour vertex v (load cases r0 .. r0 + R - 1), in place.  A: storage of u
 // (float, or half2_t pairs of load cases); f is FP32.  ZC >= 0: colour ZC of a forward sweep that
 // starts from u = 0 (colours above ZC are never read, so u needs no clearing beforehand; the old value
@@ -648,6 +888,25 @@ __global__ __launch_bounds__(256) void k_widen_f64(size_t M, const float* __rest
 }
 
 // zero the pinned origin's entries of a 6-load-case field of `planes` word planes
+// FP32 planes [6][3][N] <-> packed FP16 words [3][3][N] (word plane p * 3 + i: cases 2p, 2p + 1)
+__global__ __launch_bounds__(256) void k_pack_half(size_t N, const float* __restrict__ u, half2_t* __restrict__ z) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= 9 * N) return;
+    const size_t q = t / N, v = t - q * N, p = q / 3, i = q - 3 * p;
+    half2_t h;
+    h.x = (_Float16)u[((2 * p) * 3 + i) * N + v];
+    h.y = (_Float16)u[((2 * p + 1) * 3 + i) * N + v];
+    z[t] = h;
+}
+__global__ __launch_bounds__(256) void k_unpack_half(size_t N, const half2_t* __restrict__ z, float* __restrict__ u) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= 9 * N) return;
+    const size_t q = t / N, v = t - q * N, p = q / 3, i = q - 3 * p;
+    const half2_t h = z[t];
+    u[((2 * p) * 3 + i) * N + v] = (float)h.x;
+    u[((2 * p + 1) * 3 + i) * N + v] = (float)h.y;
+}
+
 template <typename T>
 __global__ void k_zero_pin(size_t N, int planes, T* __restrict__ u) {
     if ((int)threadIdx.x < planes) u[(size_t)threadIdx.x * N] = T(0);
@@ -1628,11 +1887,71 @@ static unsigned full_grid(const Lay& L) {
     return L.lex ? nblk(N, kFullThreads) : nblk(L.h3, 64);
 }
 
-void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
-                       const float* f, float* out) {
+int ke_mfma_table(const double* kl, const double* km, double lam, double mu, void* table, float* c0, float* c1) {
+    // One exact table T = r Kl + Km (KE = mu/72 T) when r = lambda / mu is a short binary fraction
+    // (nu = 0.3 gives r = 3/2) and every entry of T is an FP16 value; else Kl and Km separately.
+    std::vector<double> T(576);
+    const double r = std::nearbyint(lam / mu * 256.0) / 256.0;
+    bool one = std::fabs(r - lam / mu) <= 1e-12 * std::fabs(lam / mu);
+    for (int e = 0; one && e < 576; e++) {
+        T[e] = r * kl[e] + km[e];
+        one = (double)(_Float16)T[e] == T[e];
+    }
+    // lane l of tile t holds row 16t + l % 16, K entries 8(l / 16) .. + 7.  Row 16t + 4g + j is slot
+    // 4t + j of lane group g: one table: slots 0..5 its rows k = 0..5 of the group; two tables: slots
+    // 0..5 the Kl rows, 6..11 the Km rows.  Row k of group g is component k % 3 of node
+    // (k / 3, g & 1, g >> 1).  K entry 8g' + e (e < 6) is component e % 3 of node (e / 3, g' & 1, g' >> 1);
+    // entries 6 and 7 are zero.
+    _Float16* A = static_cast<_Float16*>(table);
+    for (int t = 0; t < 3; t++)
+        for (int l = 0; l < 64; l++) {
+            const int m = l % 16, gp = l / 16, g = m / 4, slot = 4 * t + m % 4;
+            const double* src = one ? (slot < 6 ? T.data() : nullptr) : (slot < 6 ? kl : km);
+            const int k = slot % 6, o = k / 3 + 2 * (g & 1) + 4 * (g >> 1), i = k % 3;
+            for (int e = 0; e < 8; e++) {
+                const int op = e / 3 + 2 * (gp & 1) + 4 * (gp >> 1), ip = e % 3;
+                A[(t * 64 + l) * 8 + e] = (_Float16)(src && e < 6 ? src[(3 * o + i) * 24 + 3 * op + ip] : 0.0);
+            }
+        }
+    *c0 = (float)((one ? mu : lam) / 72.0);
+    *c1 = one ? 0.f : (float)(mu / 72.0);
+    return one ? 1 : 2;
+}
+
+int fine_apply_mfma_ok(int n) { return n % 32 == 0 && 72.0 * n * n * n < 4294967296.0; }
+
+static unsigned ef_grid(int n) {
+    const unsigned h = (unsigned)n / 2;
+    return (h / kEfTX) * (h / kEfTY) * (h / kEfTZ);
+}
+
+template <int MODE, bool DOT, bool BETA>
+static void fine_apply_ef(hipStream_t s, int n, const FineMfma* mf, const float* w, const void* u, const float* f,
+                          float* out, double* partial, const float* dotv, const double* beta) {
+    const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
+    const half8_t* am = static_cast<const half8_t*>(mf->table);
+    const uint32_t* uw = static_cast<const uint32_t*>(u);
+    if (mf->nt == 1)
+        hipLaunchKernelGGL((k_fine_apply_ef<MODE, DOT, BETA, 1>), dim3(ef_grid(n)), dim3(kEfThreads), 0, s, L, N, am,
+                           mf->c0, mf->c1, w, uw, f, out, partial, dotv, beta);
+    else
+        hipLaunchKernelGGL((k_fine_apply_ef<MODE, DOT, BETA, 2>), dim3(ef_grid(n)), dim3(kEfThreads), 0, s, L, N, am,
+                           mf->c0, mf->c1, w, uw, f, out, partial, dotv, beta);
+}
+
+void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
+                       int u_half, const float* f, float* out) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
     const dim3 g(full_grid(L)), b(kFullThreads);
+    if (u_half && mf && fine_apply_mfma_ok(n)) {
+        if (f)
+            fine_apply_ef<1, false, false>(s, n, mf, w, u, f, out, nullptr, nullptr, nullptr);
+        else
+            fine_apply_ef<0, false, false>(s, n, mf, w, u, f, out, nullptr, nullptr, nullptr);
+        return;
+    }
     if (u_half) {
         const half2_t* uh = static_cast<const half2_t*>(u);
         if (f)
@@ -1652,14 +1971,22 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const float* w, co
                            nullptr);
 }
 
-void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
-                           const float* p, const double* beta, float* out, double* partial, int dot_blocks,
+void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
+                           int u_half, const float* p, const double* beta, float* out, double* partial, int dot_blocks,
                            double* out6) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
+    if (u_half && mf && fine_apply_mfma_ok(n)) {
+        if (beta)
+            fine_apply_ef<0, true, true>(s, n, mf, w, u, nullptr, out, partial, p, beta);
+        else
+            fine_apply_ef<0, true, false>(s, n, mf, w, u, nullptr, out, partial, p, beta);
+        hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)ef_grid(n), partial, out6);
+        return;
+    }
     const unsigned nb = full_grid(L);
     if (L.h3 % 64) {  // partial waves (FP32 fields only: packed FP16 needs n % 8 == 0): K p, separate dot
-        launch_fine_apply(s, n, ke, w, p, 0, nullptr, out);
+        launch_fine_apply(s, n, ke, nullptr, w, p, 0, nullptr, out);
         launch_dot6(s, 3 * N, p, out, partial, dot_blocks, out6);
         return;
     }
@@ -1814,6 +2141,14 @@ void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x) {
 
 void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u) {
     hipLaunchKernelGGL(k_widen_f64, dim3(nblk(M)), dim3(256), 0, s, M, x, u);
+}
+
+void launch_pack_half(hipStream_t s, size_t N, const float* u, void* z) {
+    hipLaunchKernelGGL(k_pack_half, dim3(nblk(9 * N)), dim3(256), 0, s, N, u, static_cast<half2_t*>(z));
+}
+
+void launch_unpack_half(hipStream_t s, size_t N, const void* z, float* u) {
+    hipLaunchKernelGGL(k_unpack_half, dim3(nblk(9 * N)), dim3(256), 0, s, N, static_cast<const half2_t*>(z), u);
 }
 
 void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half) {

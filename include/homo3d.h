@@ -89,6 +89,9 @@ typedef struct {
                                (accumulation stays FP32; needs n % 8 == 0, else FP32 is used).  Default 1. */
     int coarse_half;        /* store the Galerkin stencils of the intermediate levels in FP16, as the reference
                                does (grid_ex.cu:936-1160); the coarsest level stays FP32.  Default 1. */
+    int fine_mfma;          /* apply the finest-level operator to FP16 fields in element form on the matrix cores
+                               (integer Lame tables, exact in FP16; FP32 sums; needs n % 32 == 0 and
+                               72 n^3 < 2^32).  Default 0. */
 } homo3d_config;
 
 typedef struct {
@@ -119,6 +122,12 @@ int homo3d_macro_rhs(homo3d_handle* h);
 int homo3d_residual(homo3d_handle* h, int level);
 int homo3d_apply_K(homo3d_handle* h);              /* r = K u on the finest level (pinned) */
 int homo3d_smooth(homo3d_handle* h, int level, double omega, int reverse);
+/* The same finest-level operations on packed-FP16 storage, the form the PCG preconditioner runs
+   (no reference counterpart: the reference smooths FP32 fields).  u16 = fp16(u[0]) with the pin zeroed;
+   apply_K16: r[0] = K u16, or f - K u16 when residual; smooth16: one 8-colour sweep on u16 held in
+   FP16, then u[0] = u16.  Need n % 8 == 0 (HOMO3D_NO_SUPPORT otherwise). */
+int homo3d_apply_K16(homo3d_handle* h, int residual);
+int homo3d_smooth16(homo3d_handle* h, double omega, int reverse);
 int homo3d_restrict(homo3d_handle* h, int level);  /* f[level] = P^T r[level-1] */
 int homo3d_prolong(homo3d_handle* h, int level);   /* u[level] += P u[level+1] */
 int homo3d_coarse_solve(homo3d_handle* h);

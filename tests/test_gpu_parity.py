@@ -94,6 +94,64 @@ def test_fine_gauss_seidel_sweep(case16, orc, omega, reverse):
         assert rel(got[s], ref) < 2e-5
 
 
+def fp16_case(h3d, orc, n, seed, mfma=True, nu=NU):
+    hom = h3d.Homogenization(h3d.HomoConfig(reso=n, solver="pcg", coarse_half=False, fine_mfma=mfma,
+                                            poissonRatio=nu))
+    rho = h3d.randc_density(n, seed, 0.3, "reflect6")
+    x = (rho.astype(np.float64) ** 3).astype(np.float32)
+    hom.update(x)
+    mg = orc.MG(n, E, nu)
+    mg.set_density(x.astype(np.float64))
+    return hom, mg
+
+
+@pytest.mark.parametrize("n,mfma,nu", [(32, True, NU), (64, True, NU), (32, True, 0.35), (32, False, NU)])
+def test_fine_operator_fp16(h3d, orc, n, mfma, nu):
+    """Packed-FP16 finest-level operator: element form on the matrix cores (integer Lame tables, exact in
+    FP16: one combined table when lambda/mu is a short binary fraction, as for nu = 0.3, two tables
+    otherwise, as for nu = 0.35; exact products, FP32 sums) or the FP32 row form reading FP16 u.  Both
+    against the FP64 oracle on the same FP16-rounded u, to the FP32-field tolerance."""
+    say("fp16 operator: n=%d mfma=%s nu=%g" % (n, mfma, nu))
+    hom, mg = fp16_case(h3d, orc, n, 6, mfma, nu)
+    rng = np.random.default_rng(n)
+    u = rng.standard_normal(hom.level_shape(0)).astype(np.float32)
+    f = rng.standard_normal(hom.level_shape(0)).astype(np.float32)
+    u16 = u.astype(np.float16).astype(np.float64)
+    hom.set_field(0, 0, u)
+    hom.set_field(0, 1, f)
+    hom.apply_K16(residual=False)
+    y = hom.get_field(0, 2)
+    hom.apply_K16(residual=True)
+    r = hom.get_field(0, 2)
+    for s in range(6):
+        ky = orc.apply_K(n, mg.KE, mg.w, u16[s], pin=True)
+        rr = orc.residual(n, mg.KE, mg.w, u16[s], f[s].astype(np.float64))
+        say("  case %d: K u %.2e, f - K u %.2e" % (s, rel(y[s], ky), rel(r[s], rr)))
+        assert rel(y[s], ky) < 2e-6
+        assert rel(r[s], rr) < 2e-6
+
+
+@pytest.mark.parametrize("n,reverse", [(32, False), (32, True), (24, True)])
+def test_fine_gauss_seidel_fp16(h3d, orc, n, reverse):
+    """One sweep with FP16 storage of u (colour pairs at n = 32, one colour per launch at n = 24)
+    against the FP64 sweep from the same FP16-rounded start.  Every colour's new values are rounded to
+    FP16 before later colours read them, so the tolerance is FP16's (2^-11) with some propagation."""
+    say("fp16 sweep: n=%d reverse=%s" % (n, reverse))
+    hom, mg = fp16_case(h3d, orc, n, 7)
+    rng = np.random.default_rng(n + 1)
+    u = (rng.standard_normal(hom.level_shape(0)) * 0.1).astype(np.float32)
+    f = rng.standard_normal(hom.level_shape(0)).astype(np.float32)
+    u16 = u.astype(np.float16).astype(np.float64)
+    hom.set_field(0, 0, u)
+    hom.set_field(0, 1, f)
+    hom.smooth16(1.0, reverse)
+    got = hom.get_field(0, 0)
+    for s in range(6):
+        ref = orc.gs_fine(n, mg.KE, mg.w, u16[s], f[s].astype(np.float64), 1.0, reverse)
+        say("  case %d: %.2e" % (s, rel(got[s], ref)))
+        assert rel(got[s], ref) < 3e-3
+
+
 def test_galerkin_stencils(h3d, orc):
     say("galerkin: n=32")
     hom, x = make(h3d, 32, seed=4)
