@@ -23,12 +23,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bench_256cube_pcg_traffic.json")
-PROF_KINDS = ("fine_gs", "fine_gs_zero", "fine_apply", "fine_residual")
+PROF_KINDS = ("fine_gs", "fine_gs_zero", "fine_apply", "fine_apply_first", "fine_residual")
 # rocprofv3 names of the kernel behind each timed kind (demangled, or mangled for the packed-FP16
 # instantiations), for the traffic lookup
 PROF_KERNEL = {"fine_gs": ("k_fine_gs2<6, true, -1", "k_fine_gs2ILi6ELb1ELin1E", "k_fine_gs<6, true, -1",
                            "k_fine_gsILi6ELb1ELin1E"),
-               "fine_apply": ("k_fine_apply<6, 0, true", "k_fine_applyILi6ELi0ELb1E"),
+               "fine_apply": ("k_fine_applyILi6ELi0ELb1EDv2_DF16_Lb1ELi2E", "k_fine_apply_efILi0ELb1ELb1ELi1ELi2E",
+                              "k_fine_apply_efILi0ELb1ELb1ELi2ELi2E", "k_fine_apply<6, 0, true, half2, true, 2>"),
                "fine_residual": ("k_fine_apply<6, 1, false", "k_fine_applyILi6ELi1ELb0E")}
 METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; stencil GB/s"
 
@@ -45,11 +46,13 @@ def gs_colours_per_launch(n):
 
 
 def kernel_bytes(name, n, R=6, s_z=2, s_f=4, s_w=4):
-    """Algorithmic HBM bytes per launch of a finest-level kernel.  p, q, r, f are FP32 (s_f); the
+    """Algorithmic HBM bytes per launch of a finest-level kernel.  p, q, r, f, x are FP32 (s_f); the
     preconditioner iterate z (u of the V-cycle's finest level) is packed FP16 (s_z = 2) when n % 8 == 0."""
     N = n ** 3
-    if name == "fine_apply":      # q = K p with p.q: read p, write q, read w
-        return N * (3 * R * (s_f + s_f) + s_w)
+    if name == "fine_apply":        # CG pass: read z, q_old, p_old, x, w; write q = K z + beta q_old, p, x
+        return N * (3 * R * (s_z + 6 * s_f) + s_w)
+    if name == "fine_apply_first":  # first CG pass of an inner solve: read z, w; write q = K z, p = z
+        return N * (3 * R * (s_z + 2 * s_f) + s_w)
     if name == "fine_residual":   # r = f - K z: read z, f, write r, read w
         return N * (3 * R * (s_z + s_f + s_f) + s_w)
     if name == "fine_gs":         # c colours = c/8 of an in-place sweep: z read+write, f read, w read
@@ -97,6 +100,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mfma", type=int, default=None,
+                    help="finest-level operator on FP16 fields in element form on the matrix cores (1) or row form (0); "
+                         "default: the library default")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -111,7 +117,8 @@ def main():
     rho = homo3d_amd.randc_density(n, 0, 0.2, "rotate3")
     x = torch.from_numpy((rho.astype(np.float64) ** 3).astype(np.float32)).to("cuda:0")
     torch.cuda.synchronize()
-    hom = homo3d_amd.Homogenization(homo3d_amd.HomoConfig(reso=n, solver="pcg", max_iter=400))
+    kw = {} if args.mfma is None else {"fine_mfma": bool(args.mfma)}
+    hom = homo3d_amd.Homogenization(homo3d_amd.HomoConfig(reso=n, solver="pcg", max_iter=400, **kw))
 
     def step():
         hom.update(x.data_ptr(), n, on_device=True)

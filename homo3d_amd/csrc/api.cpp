@@ -40,11 +40,12 @@ struct ProfEvent {
     int a = -1, b = -1;  // indices into the handle's event pool
 };
 
-// Timed finest-level kernels: full GS colour passes, zero-start GS colour passes, K p (with p.Kp in
-// the CG), and the V-cycle residual f - K u.
-const char* const kProfNames[] = {"fine_gs", "fine_apply", "fine_residual", "fine_gs_zero"};
-constexpr int kProfKinds = 4;
-enum { kProfGs = 0, kProfApply = 1, kProfResidual = 2, kProfGsZero = 3 };
+// Timed finest-level kernels: full GS colour passes, zero-start GS colour passes, the CG operator pass
+// (K z + beta q with the fused p, x updates and p.q; the first step of an inner solve separately), and
+// the V-cycle residual f - K u.
+const char* const kProfNames[] = {"fine_gs", "fine_apply", "fine_residual", "fine_gs_zero", "fine_apply_first"};
+constexpr int kProfKinds = 5;
+enum { kProfGs = 0, kProfApply = 1, kProfResidual = 2, kProfGsZero = 3, kProfApplyFirst = 4 };
 constexpr size_t kEventPool = 8192;
 
 }  // namespace
@@ -304,7 +305,7 @@ void prof_flush(homo3d_handle* h) {
     h->pool_used = 0;
 }
 
-// Event pair around one launch of a profiled kernel kind (0 fine_gs, 1 fine_apply, 2 fine_residual).
+// Event pair around one launch of a profiled kernel kind (kProfNames).
 void prof_begin(homo3d_handle* h, int kind, ProfEvent& ev) {
     if (!h->prof) return;
     if (h->pool_used + 2 > h->pool.size()) prof_flush(h);
@@ -464,25 +465,23 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     // and x stay FP32: an FP16 search direction costs more CG steps than it saves)
     const int zh = (h->cfg.precond_half && L0.n % 8 == 0) ? 1 : 0;
     vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc);  // with rz
-    HIPCHK(h, launch_set_p(h->stream, M, h->Z, zh, h->P, kDotBlocks));
-    // One CG step: q = K p with p.q, formed as K z + beta q_old since p = z + beta p_old; x is advanced by
-    // alpha p one step late, fused with the update of p (and by a last x-only pass on exit), so that p,
-    // x, r and q are each streamed once per step.
+    // One CG step: q = K p with p.q, formed as K z + beta q_old since p = z + beta p_old (the operator
+    // gathers the packed z instead of the FP32 p).  The same pass advances x by the previous step's
+    // alpha p_old and forms the new p, so p, x, r and q are each streamed once per step; a last x-only
+    // pass follows on exit.
     double rr[kRhs];
     bool x_pending = false;
     for (int it = 0; it < maxit; it++) {
         {
             ProfEvent ev;
-            prof_begin(h, kProfApply, ev);
-            // q = K p as K z (+ beta q_old): the operator gathers the packed z instead of the FP32 p
-            launch_fine_apply_dot(h->stream, L0.n, h->d_ke, h->mf(), h->d_w, h->Z, zh, h->P,
-                                  it ? h->d_sc + 4 * kRhs : nullptr, h->Q, h->d_partial, kDotBlocks,
-                                  h->d_sc + kRhs);  // pq
+            prof_begin(h, it ? kProfApply : kProfApplyFirst, ev);
+            launch_fine_apply_cg(h->stream, L0.n, h->d_ke, h->mf(), h->d_w, h->Z, zh, h->P, h->X, h->d_sc, it == 0,
+                                 h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
             prof_end(h, ev);
         }
         launch_pcg_alpha(h->stream, h->d_sc);
         launch_pcg_update_r(h->stream, M, h->d_sc, h->Q, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
-        x_pending = true;
+        x_pending = true;  // alpha p is added by the next operator pass, or by the pass after the loop
         HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         bool changed = false, all = true;
@@ -503,8 +502,6 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
                                      h->stream));
         vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc + 3 * kRhs);  // with rz_new
         launch_pcg_beta(h->stream, h->d_sc);
-        launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, zh, h->P, h->X, 1, kDotBlocks);
-        x_pending = false;
     }
     if (x_pending) launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, zh, h->P, h->X, 0, kDotBlocks);
     return HOMO3D_OK;

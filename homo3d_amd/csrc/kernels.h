@@ -24,12 +24,14 @@ int fine_apply_mfma_ok(int n);
 // runs in element form on the matrix cores.
 void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
                        int u_half, const float* f, float* out);
-// CG step q = K p with the per-load-case FP64 dot p . q -> out6.  With beta (device, per load case) the
-// operator is applied to u = z and q = K z + beta q_old (p = z + beta p_old); without, u must be p.
-// u is FP32 or packed FP16 (u_half).  partial holds 6 * n^3 / 512 doubles (and at least 6 * dot_blocks).
-void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
-                           int u_half, const float* p, const double* beta, float* out, double* partial, int dot_blocks,
-                           double* out6);
+// One CG step's operator pass with the vector updates fused (z the preconditioned residual, FP32 or
+// packed FP16 by z_half; sc the device CG scalars, alpha at sc + 12, beta at sc + 24):
+//   first: p = z, q = K p;  else: x += alpha p_old, p = z + beta p_old, q = K z + beta q_old (= K p),
+// and the per-load-case FP64 dot p . q -> out6.  partial holds 6 * n^3 / 512 doubles (and at least
+// 6 * dot_blocks).
+void launch_fine_apply_cg(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* z,
+                          int z_half, float* p, float* x, const double* sc, int first, float* q, double* partial,
+                          int dot_blocks, double* out6);
 // zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
                     const float* w, void* u, int u_half, const float* f);

@@ -358,14 +358,20 @@ __device__ __forceinline__ uint32_t colour_block(int colour) { return (uint32_t)
 // holds a vertex (64 | (n/2)^3), so no thread leaves before the workgroup reduction.
 // A: storage of u (float, or half2_t pairs of load cases).  BETA selects the recurrence form at compile
 // time (a run-time test in the epilogue makes the compiler spill the gather pipeline).
-template <int R, int MODE, bool DOT, typename A, bool BETA = false>
+// PX (with DOT, u = z): the CG vector updates of the step, fused: p = z (PX 1, first step) or
+// x += alpha p_old, p = z + beta p_old (PX 2), per load case with alpha, beta from the device; the dot
+// is then p . out with the new p (dotv unused).  Load cases that have converged carry alpha = beta = 0.
+template <int R, int MODE, bool DOT, typename A, bool BETA = false, int PX = 0>
 __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
                                                                 const float* __restrict__ w,
                                                                 const A* __restrict__ u,
                                                                 const float* __restrict__ f, float* __restrict__ out,
                                                                 double* __restrict__ partial,
                                                                 const float* __restrict__ dotv,
-                                                                const double* __restrict__ beta) {
+                                                                const double* __restrict__ beta,
+                                                                float* __restrict__ pp = nullptr,
+                                                                float* __restrict__ xx = nullptr,
+                                                                const double* __restrict__ alpha = nullptr) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
     uint32_t v;
@@ -376,29 +382,42 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
     fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
     fine_gather<R, kRowAll, float, A>(c, L.n, N, ke_s, u, acc);
+    double dot[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        float b = 0.f;
+        float b = 0.f, a = 0.f;
         if constexpr (BETA) b = (float)beta[r];
+        if constexpr (PX == 2) a = (float)alpha[r];
+        dot[r] = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const size_t o = (size_t)(r * 3 + i) * N + v;
             float val = (MODE == 1) ? f[o] - acc[r][i] : acc[r][i];
             if constexpr (BETA) val = fmaf(b, out[o], val);
-            acc[r][i] = (v == 0) ? 0.f : val;
-            out[o] = acc[r][i];
+            val = (v == 0) ? 0.f : val;
+            out[o] = val;
+            float dv = 0.f;
+            if constexpr (PX != 0) {
+                float zc;  // z of the vertex itself
+                if constexpr (CasesPerWord<A>::value == 2)
+                    zc = (float)u[(size_t)((r >> 1) * 3 + i) * N + v][r & 1];
+                else
+                    zc = (float)u[o];
+                dv = zc;
+                if constexpr (PX == 2) {
+                    const float po = pp[o];
+                    xx[o] = fmaf(a, po, xx[o]);
+                    dv = fmaf(b, po, zc);
+                }
+                pp[o] = dv;
+            } else if constexpr (DOT) {
+                dv = dotv[o];
+            }
+            if constexpr (DOT) dot[r] = fma((double)dv, (double)val, dot[r]);
         }
     }
     if constexpr (DOT) {
         __shared__ double sh[kFullThreads / 64];
-        double dot[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            dot[r] = 0.0;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                dot[r] = fma((double)dotv[(size_t)(r * 3 + i) * N + v], (double)acc[r][i], dot[r]);
-        }
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const double d = block_sum<kFullThreads>(dot[r], sh);
@@ -499,15 +518,18 @@ __device__ __forceinline__ void ef_cells(const uint32_t* nodes, int nb0, half8_t
 
 // MODE 0: out = K u (BETA: + beta out_old); MODE 1: out = f - K u.  DOT (MODE 0): per-workgroup FP64
 // partial sums of dotv . out per load case in partial[r * gridDim.x + block].  Pinned origin row zero.
+// PX: the CG vector updates fused as in k_fine_apply (z of the vertex read from the staged node box).
 // KE = c0 T (NT = 1) or c0 Kl + c1 Km (NT = 2).  Needs 72 n^3 < 2^32 (32-bit buffer offsets).
-template <int MODE, bool DOT, bool BETA, int NT>
+template <int MODE, bool DOT, bool BETA, int NT, int PX = 0>
 __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, const half8_t* __restrict__ amat,
                                                               float c0, float c1, const float* __restrict__ w,
                                                               const uint32_t* __restrict__ u,
                                                               const float* __restrict__ f, float* __restrict__ out,
                                                               double* __restrict__ partial,
                                                               const float* __restrict__ dotv,
-                                                              const double* __restrict__ beta) {
+                                                              const double* __restrict__ beta,
+                                                              float* __restrict__ pp, float* __restrict__ xx,
+                                                              const double* __restrict__ alpha) {
     __shared__ uint32_t nodes[3 * kEfNodes];       // packed words of the node box, one load-case pair
     __shared__ float part[4][6][kEfVerts];         // per cell-row class, (case of the pair, component)
     __shared__ float seam[4][6][2][kEfYZ];         // x = 0 / 16 contributions of the first lanes
@@ -523,7 +545,8 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
     const Rsrc ru = make_rsrc(u, 9 * N * sizeof(uint32_t));
     const Rsrc ro = make_rsrc(out, 18 * N * sizeof(float));
     const Rsrc rf = make_rsrc(f, f ? 18 * N * sizeof(float) : 0);
-    const Rsrc rd = make_rsrc(dotv, dotv ? 18 * N * sizeof(float) : 0);
+    const Rsrc rd = make_rsrc(PX ? pp : dotv, (PX ? pp != nullptr : dotv != nullptr) ? 18 * N * sizeof(float) : 0);
+    const Rsrc rx = make_rsrc(xx, xx ? 18 * N * sizeof(float) : 0);
     for (int c = tid; c < kEfCells; c += kEfThreads) {
         const int ex = c % kEfCX - 1, ey = (c / kEfCX) % kEfCY - 1, ez = c / (kEfCX * kEfCY) - 1;
         wcell[c] = w[L.idx(L.wrap(x0 + ex), L.wrap(y0 + ey), L.wrap(z0 + ez))];
@@ -557,6 +580,7 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
     // epilogue vertex of this thread
     const int vx = tid % kEfVX, vyz = tid / kEfVX;
     const uint32_t vg = L.idx(x0 + vx, y0 + vyz % kEfVY, z0 + vyz / kEfVY), vb = 4u * vg;
+    const int nb_self = ((vyz / kEfVY + 1) * kEfBY + (vyz % kEfVY + 1)) * kEfBX + vx + 1;  // the vertex in the node box
     const uint32_t plane_bytes = (uint32_t)(N * sizeof(float));
     // lane offsets of the full-row groups: node (0, gy, gz) of cell (col, 0, 0) in the node box; vertex
     // (col + 1, gy, gz) in the tile
@@ -566,13 +590,15 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
     for (int p = 0; p < 3; p++) {
         if (p < 2) fetch(p + 1);
         // epilogue inputs of this pair, in flight during the cell products
-        float fin[6], oin[6], din[6];
+        float fin[6], oin[6], din[6], xin[6];
 #pragma unroll
         for (int pl = 0; pl < 6; pl++) {
             const uint32_t so = (uint32_t)(6 * p + pl) * plane_bytes;
             if constexpr (MODE == 1) fin[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rf, (int)vb, (int)so, 0));
             if constexpr (BETA) oin[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ro, (int)vb, (int)so, 0));
-            if constexpr (DOT) din[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rd, (int)vb, (int)so, 0));
+            if constexpr ((DOT && PX == 0) || PX == 2)  // dotv, or p_old
+                din[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rd, (int)vb, (int)so, 0));
+            if constexpr (PX == 2) xin[pl] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, (int)vb, (int)so, 0));
         }
         // cells x = 0 .. VX-1 of every cell row, 16 per group
         for (int gi = wave; gi < kEfFull; gi += kEfThreads / 64) {
@@ -614,6 +640,13 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
                     for (int i = 0; i < 3; i++) halo[cls][hh * 3 + i][vz * kEfVY + vy] = val[hh][3 + i];  // node x = 0
             }
         }
+        // z of this thread's vertex for the fused CG update, read before the barrier: after it, the node
+        // box is refilled with the next pair while other threads may still be in the epilogue
+        uint32_t zself[3];
+        if constexpr (PX != 0) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) zself[i] = nodes[i * kEfNodes + nb_self];
+        }
         __syncthreads();
 #pragma unroll
         for (int pl = 0; pl < 6; pl++) {
@@ -625,12 +658,26 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
                 val += (seam[0][pl][sx][vyz] + seam[1][pl][sx][vyz]) + (seam[2][pl][sx][vyz] + seam[3][pl][sx][vyz]);
             }
             if (vx == 0) val += (halo[0][pl][vyz] + halo[1][pl][vyz]) + (halo[2][pl][vyz] + halo[3][pl][vyz]);
+            const uint32_t so = (uint32_t)(6 * p + pl) * plane_bytes;
             if constexpr (MODE == 1) val = fin[pl] - val;
             if constexpr (BETA) val = fmaf((float)beta[r], oin[pl], val);
             if (vg == 0) val = 0.f;
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val), ro, (int)vb,
-                                                  (int)((uint32_t)(6 * p + pl) * plane_bytes), 0);
-            if constexpr (DOT) dot[r] = fma((double)din[pl], (double)val, dot[r]);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val), ro, (int)vb, (int)so, 0);
+            float dv = 0.f;
+            if constexpr (PX != 0) {
+                const half2_t zw = __builtin_bit_cast(half2_t, zself[pl % 3]);
+                const float zc = (float)zw[pl / 3];
+                dv = zc;
+                if constexpr (PX == 2) {
+                    __builtin_amdgcn_raw_buffer_store_b32(
+                        __builtin_bit_cast(uint32_t, fmaf((float)alpha[r], din[pl], xin[pl])), rx, (int)vb, (int)so, 0);
+                    dv = fmaf((float)beta[r], din[pl], zc);
+                }
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, dv), rd, (int)vb, (int)so, 0);
+            } else if constexpr (DOT) {
+                dv = din[pl];
+            }
+            if constexpr (DOT) dot[r] = fma((double)dv, (double)val, dot[r]);
         }
         if (p < 2) put();
         __syncthreads();
@@ -1925,19 +1972,20 @@ static unsigned ef_grid(int n) {
     return (h / kEfTX) * (h / kEfTY) * (h / kEfTZ);
 }
 
-template <int MODE, bool DOT, bool BETA>
+template <int MODE, bool DOT, bool BETA, int PX = 0>
 static void fine_apply_ef(hipStream_t s, int n, const FineMfma* mf, const float* w, const void* u, const float* f,
-                          float* out, double* partial, const float* dotv, const double* beta) {
+                          float* out, double* partial, const float* dotv, const double* beta, float* pp = nullptr,
+                          float* xx = nullptr, const double* alpha = nullptr) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
     const half8_t* am = static_cast<const half8_t*>(mf->table);
     const uint32_t* uw = static_cast<const uint32_t*>(u);
     if (mf->nt == 1)
-        hipLaunchKernelGGL((k_fine_apply_ef<MODE, DOT, BETA, 1>), dim3(ef_grid(n)), dim3(kEfThreads), 0, s, L, N, am,
-                           mf->c0, mf->c1, w, uw, f, out, partial, dotv, beta);
+        hipLaunchKernelGGL((k_fine_apply_ef<MODE, DOT, BETA, 1, PX>), dim3(ef_grid(n)), dim3(kEfThreads), 0, s, L, N,
+                           am, mf->c0, mf->c1, w, uw, f, out, partial, dotv, beta, pp, xx, alpha);
     else
-        hipLaunchKernelGGL((k_fine_apply_ef<MODE, DOT, BETA, 2>), dim3(ef_grid(n)), dim3(kEfThreads), 0, s, L, N, am,
-                           mf->c0, mf->c1, w, uw, f, out, partial, dotv, beta);
+        hipLaunchKernelGGL((k_fine_apply_ef<MODE, DOT, BETA, 2, PX>), dim3(ef_grid(n)), dim3(kEfThreads), 0, s, L, N,
+                           am, mf->c0, mf->c1, w, uw, f, out, partial, dotv, beta, pp, xx, alpha);
 }
 
 void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
@@ -1956,55 +2004,65 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf
         const half2_t* uh = static_cast<const half2_t*>(u);
         if (f)
             hipLaunchKernelGGL((k_fine_apply<6, 1, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr,
-                               nullptr, nullptr);
+                               nullptr, nullptr, nullptr, nullptr, nullptr);
         else
             hipLaunchKernelGGL((k_fine_apply<6, 0, false, half2_t>), g, b, 0, s, L, N, ke, w, uh, f, out, nullptr,
-                               nullptr, nullptr);
+                               nullptr, nullptr, nullptr, nullptr, nullptr);
         return;
     }
     const float* uf = static_cast<const float*>(u);
     if (f)
         hipLaunchKernelGGL((k_fine_apply<6, 1, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr, nullptr,
-                           nullptr);
+                           nullptr, nullptr, nullptr, nullptr);
     else
         hipLaunchKernelGGL((k_fine_apply<6, 0, false, float>), g, b, 0, s, L, N, ke, w, uf, f, out, nullptr, nullptr,
-                           nullptr);
+                           nullptr, nullptr, nullptr, nullptr);
 }
 
-void launch_fine_apply_dot(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* u,
-                           int u_half, const float* p, const double* beta, float* out, double* partial, int dot_blocks,
-                           double* out6) {
-    const size_t N = (size_t)n * n * n;
+template <typename A, bool FIRST>
+static void fine_apply_cg_row(hipStream_t s, const Lay& L, size_t N, const float* ke, const float* w, const A* z,
+                              float* p, float* x, const double* sc, float* q, double* partial) {
+    constexpr int R = 6;
+    hipLaunchKernelGGL((k_fine_apply<R, 0, true, A, !FIRST, FIRST ? 1 : 2>), dim3(full_grid(L)), dim3(kFullThreads), 0,
+                       s, L, N, ke, w, z, nullptr, q, partial, nullptr, sc + 4 * R, p, x, sc + 2 * R);
+}
+
+void launch_fine_apply_cg(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* z,
+                          int z_half, float* p, float* x, const double* sc, int first, float* q, double* partial,
+                          int dot_blocks, double* out6) {
+    constexpr int R = 6;
+    const size_t N = (size_t)n * n * n, M = 3 * N;
     const Lay L = make_lay(n, 0);
-    if (u_half && mf && fine_apply_mfma_ok(n)) {
-        if (beta)
-            fine_apply_ef<0, true, true>(s, n, mf, w, u, nullptr, out, partial, p, beta);
+    unsigned nb = full_grid(L);
+    if (z_half && mf && fine_apply_mfma_ok(n)) {
+        if (first)
+            fine_apply_ef<0, true, false, 1>(s, n, mf, w, z, nullptr, q, partial, nullptr, nullptr, p, x, nullptr);
         else
-            fine_apply_ef<0, true, false>(s, n, mf, w, u, nullptr, out, partial, p, beta);
-        hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)ef_grid(n), partial, out6);
+            fine_apply_ef<0, true, true, 2>(s, n, mf, w, z, nullptr, q, partial, nullptr, sc + 4 * R, p, x, sc + 2 * R);
+        nb = ef_grid(n);
+    } else if (L.h3 % 64) {
+        // partial waves (FP32 z only: packed FP16 needs n % 8 == 0): the same step from separate passes
+        if (first) {
+            (void)launch_set_p(s, M, z, 0, p, dot_blocks);
+        } else {
+            launch_pcg_update_px(s, M, sc, z, 0, p, x, 1, dot_blocks);
+        }
+        launch_fine_apply(s, n, ke, nullptr, w, p, 0, nullptr, q);
+        launch_dot6(s, M, p, q, partial, dot_blocks, out6);
         return;
+    } else if (z_half) {
+        const half2_t* zh = static_cast<const half2_t*>(z);
+        if (first)
+            fine_apply_cg_row<half2_t, true>(s, L, N, ke, w, zh, p, x, sc, q, partial);
+        else
+            fine_apply_cg_row<half2_t, false>(s, L, N, ke, w, zh, p, x, sc, q, partial);
+    } else {
+        const float* zf = static_cast<const float*>(z);
+        if (first)
+            fine_apply_cg_row<float, true>(s, L, N, ke, w, zf, p, x, sc, q, partial);
+        else
+            fine_apply_cg_row<float, false>(s, L, N, ke, w, zf, p, x, sc, q, partial);
     }
-    const unsigned nb = full_grid(L);
-    if (L.h3 % 64) {  // partial waves (FP32 fields only: packed FP16 needs n % 8 == 0): K p, separate dot
-        launch_fine_apply(s, n, ke, nullptr, w, p, 0, nullptr, out);
-        launch_dot6(s, 3 * N, p, out, partial, dot_blocks, out6);
-        return;
-    }
-    const dim3 g(nb), b(kFullThreads);
-    const half2_t* uh = static_cast<const half2_t*>(u);
-    const float* uf = static_cast<const float*>(u);
-    if (u_half && beta)
-        hipLaunchKernelGGL((k_fine_apply<6, 0, true, half2_t, true>), g, b, 0, s, L, N, ke, w, uh, nullptr, out,
-                           partial, p, beta);
-    else if (u_half)
-        hipLaunchKernelGGL((k_fine_apply<6, 0, true, half2_t, false>), g, b, 0, s, L, N, ke, w, uh, nullptr, out,
-                           partial, p, beta);
-    else if (beta)
-        hipLaunchKernelGGL((k_fine_apply<6, 0, true, float, true>), g, b, 0, s, L, N, ke, w, uf, nullptr, out,
-                           partial, p, beta);
-    else
-        hipLaunchKernelGGL((k_fine_apply<6, 0, true, float, false>), g, b, 0, s, L, N, ke, w, uf, nullptr, out,
-                           partial, p, beta);
     hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)nb, partial, out6);
 }
 

@@ -257,12 +257,14 @@ def test_mg_solve_and_elastic_matrix_32(h3d, orc):
     assert rel(C, C_same) < 1e-5
 
 
-@pytest.mark.parametrize("mixed", [True, False])
-def test_pcg_tight_solve_matches_oracle(h3d, orc, mixed):
-    """mixed: the production configuration (FP16 z, p and intermediate-level stencils); otherwise all
-    inner storage FP32.  Either way the FP64-refined relres meets the tolerance."""
-    say("pcg: n=32 relres 1e-5 mixed=%s" % mixed)
-    hom = h3d.Homogenization(h3d.HomoConfig(reso=32, solver="pcg", coarse_half=mixed, precond_half=mixed))
+@pytest.mark.parametrize("mixed,mfma", [(True, False), (False, False), (True, True)])
+def test_pcg_tight_solve_matches_oracle(h3d, orc, mixed, mfma):
+    """mixed: the production configuration (FP16 z and intermediate-level stencils); otherwise all inner
+    storage FP32.  mfma: the finest-level operator on FP16 z in element form on the matrix cores.  Either
+    way the FP64-refined relres meets the tolerance."""
+    say("pcg: n=32 relres 1e-5 mixed=%s mfma=%s" % (mixed, mfma))
+    hom = h3d.Homogenization(h3d.HomoConfig(reso=32, solver="pcg", coarse_half=mixed, precond_half=mixed,
+                                            fine_mfma=mfma))
     rho = h3d.randc_density(32, 2, 0.2, "rotate3")
     x = (rho.astype(np.float64) ** 3).astype(np.float32)
     hom.update(x)
@@ -286,6 +288,25 @@ def test_pcg_tight_solve_matches_oracle(h3d, orc, mixed):
     # warm start from the stored solution converges immediately
     st2 = hom.solve(1e-5, warm=True)
     assert sum(st2["iters"]) == 0
+
+
+def test_pcg_mfma_matches_row_form(h3d):
+    """The two forms of the finest-level operator are the same operator to FP32 rounding, so a 64^3 solve
+    takes the same CG steps (to one) and gives the same C^H with either."""
+    say("pcg: n=64 mfma vs row form")
+    rho = h3d.randc_density(64, 3, 0.2, "rotate3")
+    x = (rho.astype(np.float64) ** 3).astype(np.float32)
+    out = []
+    for mfma in (False, True):
+        hom = h3d.Homogenization(h3d.HomoConfig(reso=64, solver="pcg", fine_mfma=mfma))
+        hom.update(x)
+        C = hom.elasticMatrix(tol=1e-6, warm=False)
+        st = hom.last_stats
+        say("  mfma=%s iters %s relres %.2e" % (mfma, st["iters"], max(st["relres"])))
+        assert st["status"] == 0 and max(st["relres"]) <= 1e-6
+        out.append((np.asarray(st["iters"]), C))
+    assert np.abs(out[0][0] - out[1][0]).max() <= 1
+    assert rel(out[1][1], out[0][1]) < 1e-5
 
 
 def test_solid_and_laminate(h3d, orc):
