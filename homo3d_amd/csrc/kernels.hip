@@ -464,6 +464,11 @@ constexpr int kEfRowGroups = kEfVX / 16;                 // groups of cells x = 
 constexpr int kEfFull = kEfRows * kEfRowGroups;
 constexpr int kEfGroups = kEfFull + (kEfRows + 15) / 16;  // + groups of the x = -1 cells
 constexpr int kEfYZ = kEfVY * kEfVZ;                      // vertex rows of the tile
+// one LDS row per (cell-row class, case of the pair, component): the tile's vertices, the x-seam and
+// x = 0 contributions, and a slot per lane that absorbs the stores of lanes with nothing for the tile
+// (stores stay unconditional: no exec-mask branches in the cell loop)
+constexpr int kEfSeam = kEfVerts, kEfHalo = kEfSeam + 2 * kEfYZ, kEfDummy = kEfHalo + kEfYZ;
+constexpr int kEfSlots = kEfDummy + 64;
 constexpr int kEfThreads = 512;
 constexpr int kEfStage = (3 * kEfNodes + kEfThreads - 1) / kEfThreads;  // staged words per thread and pair
 static_assert(kEfVerts == kEfThreads, "epilogue maps one vertex per thread and plane");
@@ -474,22 +479,25 @@ __device__ __forceinline__ float dpp_next(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x101, 0xf, 0xf, true));
 }
 
-// One group of 16 cells and the two load cases of a packed word pair: B operands from the node box,
-// NT + 1 output tiles per case (NT = 1: one exact table T; NT = 2: Kl and Km), and the lane's 6 rows
-// of w_e KE u_e per case in val[case][k] (k = 3 ox + component).
-template <int NT>
-__device__ __forceinline__ void ef_cells(const uint32_t* nodes, int nb0, half8_t a0, half8_t a1, half8_t a2, float w0,
-                                         float w1, float (&val)[2][6]) {
+// One group of 16 cells and the two load cases of a packed word pair: the B operands (the lane's 6 node
+// words, split into the two load cases) from the node box ...
+__device__ __forceinline__ void ef_load_b(const uint32_t* nodes, int nb0, half8_t (&b)[2]) {
     half2_t wv[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) wv[k] = __builtin_bit_cast(half2_t, nodes[(k % 3) * kEfNodes + nb0 + k / 3]);
-    half8_t b[2];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         b[0][k] = wv[k].x;
         b[1][k] = wv[k].y;
     }
     b[0][6] = b[0][7] = b[1][6] = b[1][7] = (_Float16)0.f;
+}
+
+// ... and NT + 1 output tiles per case (NT = 1: one exact table T; NT = 2: Kl and Km), giving the lane's
+// 6 rows of w_e KE u_e per case in val[case][k] (k = 3 ox + component); w0, w1 = w_e c0, w_e c1.
+template <int NT>
+__device__ __forceinline__ void ef_mma(const half8_t (&b)[2], half8_t a0, half8_t a1, half8_t a2, float w0, float w1,
+                                       float (&val)[2][6]) {
     const float4_t zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int hh = 0; hh < 2; hh++) {
@@ -516,12 +524,38 @@ __device__ __forceinline__ void ef_cells(const uint32_t* nodes, int nb0, half8_t
     }
 }
 
+// A group of cells x = 16 half .. 16 half + 15 of cell row gi / 2 and where its lanes' results go
+struct EfRowGroup {
+    int nb0;   // node box index of the lane's node (0, gy, gz)
+    int wi;    // cell weight index
+    int v1;    // tile vertex of the lane's node (1, gy, gz): x = 16 half + col + 1
+    int yz;    // vertex row of the lane's nodes
+    int cls;   // parity class of the cell row
+    int half;
+    bool own;  // the lane's vertex row lies in the tile
+};
+
+__device__ __forceinline__ EfRowGroup ef_row_group(int gi, int col, int gy, int gz, int lane_nb, int lane_v) {
+    EfRowGroup G;
+    const int row = gi >> 1;
+    G.half = gi & 1;
+    const int ey = row % kEfCY - 1, ez = row / kEfCY - 1;
+    G.cls = ((ey + 1) & 1) | (((ez + 1) & 1) << 1);
+    G.wi = ((ez + 1) * kEfCY + (ey + 1)) * kEfCX + 16 * G.half + 1 + col;
+    G.nb0 = ((ez + 1) * kEfBY + (ey + 1)) * kEfBX + 16 * G.half + 1 + lane_nb;
+    const int vy = ey + gy, vz = ez + gz;
+    G.own = vy >= 0 && vy < kEfVY && vz >= 0 && vz < kEfVZ;
+    G.v1 = (ez * kEfVY + ey) * kEfVX + 16 * G.half + 1 + lane_v;
+    G.yz = vz * kEfVY + vy;
+    return G;
+}
+
 // MODE 0: out = K u (BETA: + beta out_old); MODE 1: out = f - K u.  DOT (MODE 0): per-workgroup FP64
 // partial sums of dotv . out per load case in partial[r * gridDim.x + block].  Pinned origin row zero.
 // PX: the CG vector updates fused as in k_fine_apply (z of the vertex read from the staged node box).
 // KE = c0 T (NT = 1) or c0 Kl + c1 Km (NT = 2).  Needs 72 n^3 < 2^32 (32-bit buffer offsets).
 template <int MODE, bool DOT, bool BETA, int NT, int PX = 0>
-__global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, const half8_t* __restrict__ amat,
+__global__ __launch_bounds__(kEfThreads, 2) void k_fine_apply_ef(Lay L, size_t N, const half8_t* __restrict__ amat,
                                                               float c0, float c1, const float* __restrict__ w,
                                                               const uint32_t* __restrict__ u,
                                                               const float* __restrict__ f, float* __restrict__ out,
@@ -531,9 +565,7 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
                                                               float* __restrict__ pp, float* __restrict__ xx,
                                                               const double* __restrict__ alpha) {
     __shared__ uint32_t nodes[3 * kEfNodes];       // packed words of the node box, one load-case pair
-    __shared__ float part[4][6][kEfVerts];         // per cell-row class, (case of the pair, component)
-    __shared__ float seam[4][6][2][kEfYZ];         // x = 0 / 16 contributions of the first lanes
-    __shared__ float halo[4][6][kEfYZ];            // x = 0 contributions of the cells at x = -1
+    __shared__ float acc[4][6][kEfSlots];          // per cell-row class, (case of the pair, component)
     __shared__ float wcell[kEfCells];              // w_e of the cells from -1
     const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, col = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: group loops stay scalar
@@ -602,25 +634,24 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
         }
         // cells x = 0 .. VX-1 of every cell row, 16 per group
         for (int gi = wave; gi < kEfFull; gi += kEfThreads / 64) {
-            const int row = gi >> 1, half = gi & 1;
-            const int ey = row % kEfCY - 1, ez = row / kEfCY - 1;
-            const int cls = ((ey + 1) & 1) | (((ez + 1) & 1) << 1);
-            const float wc = wcell[((ez + 1) * kEfCY + (ey + 1)) * kEfCX + 16 * half + 1 + col];
+            const EfRowGroup g = ef_row_group(gi, col, gy, gz, lane_nb, lane_v);
+            half8_t b[2];
+            ef_load_b(nodes, g.nb0, b);
+            const float wc = wcell[g.wi];
             float val[2][6];
-            ef_cells<NT>(nodes, ((ez + 1) * kEfBY + (ey + 1)) * kEfBX + 16 * half + 1 + lane_nb, a0, a1, a2, wc * c0,
-                         wc * c1, val);
-            const int vy = ey + gy, vz = ez + gz;
-            const bool own = vy >= 0 && vy < kEfVY && vz >= 0 && vz < kEfVZ;
-            const int v1 = (ez * kEfVY + ey) * kEfVX + 16 * half + 1 + lane_v;  // vertex x = 16 half + col + 1
-            const int yz = vz * kEfVY + vy;
+            ef_mma<NT>(b, a0, a1, a2, wc * c0, wc * c1, val);
+            // vertex x = 16 half + col + 1 (all lanes but the last of the second half), and the seam x = 16 half
+            // of the first lane
+            float* row = &acc[g.cls][0][0];
+            const int s1 = g.own && (g.half == 0 || col < 15) ? g.v1 : kEfDummy + lane;
+            const int sm = g.own && col == 0 ? kEfSeam + g.half * kEfYZ + g.yz : kEfDummy + lane;
 #pragma unroll
             for (int hh = 0; hh < 2; hh++)
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
                     // node ox = 1 of cell c is node ox = 0 of cell c + 1
-                    const float s1 = val[hh][3 + i] + dpp_next(val[hh][i]);
-                    if (own && (half == 0 || col < 15)) part[cls][hh * 3 + i][v1] = s1;
-                    if (own && col == 0) seam[cls][hh * 3 + i][half][yz] = val[hh][i];
+                    row[(hh * 3 + i) * kEfSlots + s1] = val[hh][3 + i] + dpp_next(val[hh][i]);
+                    row[(hh * 3 + i) * kEfSlots + sm] = val[hh][i];
                 }
         }
         // cells x = -1 of the cell rows, 16 rows per group
@@ -630,15 +661,18 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
             const int kk = valid ? k : 0, ey = kk % kEfCY - 1, ez = kk / kEfCY - 1;
             const int cls = ((ey + 1) & 1) | (((ez + 1) & 1) << 1);
             const float wc = wcell[((ez + 1) * kEfCY + (ey + 1)) * kEfCX];
+            half8_t b[2];
+            ef_load_b(nodes, ((ez + 1 + gz) * kEfBY + (ey + 1 + gy)) * kEfBX, b);
             float val[2][6];
-            ef_cells<NT>(nodes, ((ez + 1 + gz) * kEfBY + (ey + 1 + gy)) * kEfBX, a0, a1, a2, wc * c0, wc * c1, val);
+            ef_mma<NT>(b, a0, a1, a2, wc * c0, wc * c1, val);
             const int vy = ey + gy, vz = ez + gz;
-            if (valid && vy >= 0 && vy < kEfVY && vz >= 0 && vz < kEfVZ) {
+            const int hs = valid && vy >= 0 && vy < kEfVY && vz >= 0 && vz < kEfVZ ? kEfHalo + vz * kEfVY + vy
+                                                                                   : kEfDummy + lane;
+            float* row = &acc[cls][0][0];
 #pragma unroll
-                for (int hh = 0; hh < 2; hh++)
+            for (int hh = 0; hh < 2; hh++)
 #pragma unroll
-                    for (int i = 0; i < 3; i++) halo[cls][hh * 3 + i][vz * kEfVY + vy] = val[hh][3 + i];  // node x = 0
-            }
+                for (int i = 0; i < 3; i++) row[(hh * 3 + i) * kEfSlots + hs] = val[hh][3 + i];  // node x = 0
         }
         // z of this thread's vertex for the fused CG update, read before the barrier: after it, the node
         // box is refilled with the next pair while other threads may still be in the epilogue
@@ -652,12 +686,15 @@ __global__ __launch_bounds__(kEfThreads) void k_fine_apply_ef(Lay L, size_t N, c
         for (int pl = 0; pl < 6; pl++) {
             const int r = 2 * p + pl / 3;
             float val = 0.f;
-            if (vx != 0) val = (part[0][pl][tid] + part[1][pl][tid]) + (part[2][pl][tid] + part[3][pl][tid]);
+            if (vx != 0) val = (acc[0][pl][tid] + acc[1][pl][tid]) + (acc[2][pl][tid] + acc[3][pl][tid]);
             if (vx == 0 || vx == 16) {
-                const int sx = vx >> 4;
-                val += (seam[0][pl][sx][vyz] + seam[1][pl][sx][vyz]) + (seam[2][pl][sx][vyz] + seam[3][pl][sx][vyz]);
+                const int sl = kEfSeam + (vx >> 4) * kEfYZ + vyz;
+                val += (acc[0][pl][sl] + acc[1][pl][sl]) + (acc[2][pl][sl] + acc[3][pl][sl]);
             }
-            if (vx == 0) val += (halo[0][pl][vyz] + halo[1][pl][vyz]) + (halo[2][pl][vyz] + halo[3][pl][vyz]);
+            if (vx == 0) {
+                const int sl = kEfHalo + vyz;
+                val += (acc[0][pl][sl] + acc[1][pl][sl]) + (acc[2][pl][sl] + acc[3][pl][sl]);
+            }
             const uint32_t so = (uint32_t)(6 * p + pl) * plane_bytes;
             if constexpr (MODE == 1) val = fin[pl] - val;
             if constexpr (BETA) val = fmaf((float)beta[r], oin[pl], val);
