@@ -871,17 +871,23 @@ __global__ __launch_bounds__(kFullThreads) void k_macro_rhs(Lay L, size_t N, con
         }
 }
 
-// Iterative-refinement residual in FP64: r = f(w) - K u with u the FP64 solution, f the macro-strain
+// Iterative-refinement residual in FP64 (the row of grid.cu:1277-1461 in FP64): r = f(w) - K u with u the
+// FP64 solution, f the macro-strain
 // loads rebuilt from the cell weights in FP64 (kchi64 = KE chi0 in FP64) and every product
 // accumulated in FP64.  Writes r rounded to FP32 (the inner solve's right-hand side) and per-block
 // FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0 (r = -K u); DO_K = false
 // gives r = f (u = 0, nothing read from u).
-template <int R, bool DO_F, bool DO_K>
-__global__ __launch_bounds__(kFullThreads, 2) void k_fine_residual64(Lay L, size_t N, int r0, const double* __restrict__ ke,
-                                                            const double* __restrict__ kchi,
-                                                            const float* __restrict__ xc, double emin,
-                                                            const double* __restrict__ u, float* __restrict__ r32,
-                                                            double* __restrict__ partial) {
+// All six load cases in one pass: the 27 FP64 blocks are built once per vertex and applied to every
+// load case.  The neighbour loop stays rolled (d is uniform over the wave): fully unrolled, the FP64
+// gather spills.
+template <bool DO_F, bool DO_K>
+__global__ __launch_bounds__(kFullThreads, 1) void k_fine_residual64(Lay L, size_t N, const double* __restrict__ ke,
+                                                                         const double* __restrict__ kchi,
+                                                                         const float* __restrict__ xc, double emin,
+                                                                         const double* __restrict__ u,
+                                                                         float* __restrict__ r32,
+                                                                         double* __restrict__ partial) {
+    constexpr int R = 6;
     __shared__ double sh[kFullThreads / 64];
     __shared__ double ke_s[576];
     stage_ke(ke_s, ke);
@@ -896,13 +902,43 @@ __global__ __launch_bounds__(kFullThreads, 2) void k_fine_residual64(Lay L, size
         FineCtx c;
         fine_ctx64(c, x, y, z, L, xc, emin);
         double acc[R][3];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) acc[r][i] = 0.0;
         if (DO_K) {
-            fine_gather<R, kRowAll, double, double>(c, L.n, N, ke_s, u, acc, r0);
-        } else {
+            const Rsrc ru = make_rsrc(u, 18 * N * sizeof(double));
+            const uint32_t plane = (uint32_t)(N * sizeof(double));
+#pragma unroll 1
+            for (int d = 0; d < 27; d++) {
+                const int dx = d % 3 - 1, dy = d / 3 % 3 - 1, dz = d / 9 - 1;
+                const uint32_t qo = c.nb(d) * (uint32_t)sizeof(double);
+                double un[R][3];
 #pragma unroll
-            for (int r = 0; r < R; r++)
+                for (int r = 0; r < R; r++)
 #pragma unroll
-                for (int i = 0; i < 3; i++) acc[r][i] = 0.0;
+                    for (int i = 0; i < 3; i++) un[r][i] = bload<double>(ru, qo, (uint32_t)(r * 3 + i) * plane);
+                double S[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) S[k] = 0.0;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    // cell e at offset (e&1, e>>1&1, e>>2) - 1 holds v as node 7 - e and v + d as node b
+                    const int bx = dx + 1 - (e & 1), by = dy + 1 - ((e >> 1) & 1), bz = dz + 1 - (e >> 2);
+                    if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+                    const int a = 7 - e, b = bx + 2 * by + 4 * bz;
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++)
+                            S[i * 3 + j] = fma(c.wd[e], ke_s[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+                        acc[r][i] = fma(S[i * 3], un[r][0], fma(S[i * 3 + 1], un[r][1], fma(S[i * 3 + 2], un[r][2], acc[r][i])));
+            }
         }
 #pragma unroll
         for (int r = 0; r < R; r++)
@@ -911,17 +947,17 @@ __global__ __launch_bounds__(kFullThreads, 2) void k_fine_residual64(Lay L, size
                 double f = 0.0;
                 if (DO_F) {
 #pragma unroll
-                    for (int e = 0; e < 8; e++) f = fma(c.wd[e], kchi[(r0 + r) * 24 + 3 * (7 - e) + i], f);
+                    for (int e = 0; e < 8; e++) f = fma(c.wd[e], kchi[r * 24 + 3 * (7 - e) + i], f);
                 }
                 const double val = (v == 0) ? 0.0 : f - acc[r][i];
-                r32[(size_t)((r0 + r) * 3 + i) * N + v] = (float)val;
+                r32[(size_t)(r * 3 + i) * N + v] = (float)val;
                 nrm[r] = fma(val, val, nrm[r]);
             }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const double s = block_sum<kFullThreads>(nrm[r], sh);
-        if (threadIdx.x == 0) partial[(r0 + r) * gridDim.x + blockIdx.x] = s;
+        if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -2193,27 +2229,17 @@ void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, f
 
 // One load case per FP64 residual launch: with two, the FP64 gather pipeline no longer fits the
 // register file.
-constexpr int kResid64Cases = 1;
-
-template <bool DO_F, bool DO_K>
-static void residual64_case(hipStream_t s, const Lay& L, size_t N, int r0, const double* ke, const double* kchi,
-                            const float* xc, double emin, const double* u, float* r32, double* partial) {
-    hipLaunchKernelGGL((k_fine_residual64<kResid64Cases, DO_F, DO_K>), dim3(full_grid(L)), dim3(kFullThreads), 0, s, L,
-                       N, r0, ke, kchi, xc, emin, u, r32, partial);
-}
-
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
                             const double* u, float* r32, double* partial, double* out6, int do_f, int do_k) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
-    for (int r0 = 0; r0 < 6; r0 += kResid64Cases) {
-        if (!do_k)
-            residual64_case<true, false>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
-        else if (do_f)
-            residual64_case<true, true>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
-        else
-            residual64_case<false, true>(s, L, N, r0, ke, kchi, xc, emin, u, r32, partial);
-    }
+    const dim3 g(full_grid(L)), b(kFullThreads);
+    if (!do_k)
+        hipLaunchKernelGGL((k_fine_residual64<true, false>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
+    else if (do_f)
+        hipLaunchKernelGGL((k_fine_residual64<true, true>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
+    else
+        hipLaunchKernelGGL((k_fine_residual64<false, true>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)full_grid(L), partial, out6);
 }
 
