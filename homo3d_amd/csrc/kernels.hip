@@ -871,15 +871,13 @@ __global__ __launch_bounds__(kFullThreads) void k_macro_rhs(Lay L, size_t N, con
         }
 }
 
-// Iterative-refinement residual in FP64 (the row of grid.cu:1277-1461 in FP64): r = f(w) - K u with u the
-// FP64 solution, f the macro-strain
-// loads rebuilt from the cell weights in FP64 (kchi64 = KE chi0 in FP64) and every product
-// accumulated in FP64.  Writes r rounded to FP32 (the inner solve's right-hand side) and per-block
-// FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0 (r = -K u); DO_K = false
-// gives r = f (u = 0, nothing read from u).
-// All six load cases in one pass: the 27 FP64 blocks are built once per vertex and applied to every
-// load case.  The neighbour loop stays rolled (d is uniform over the wave): fully unrolled, the FP64
-// gather spills.
+// Iterative-refinement residual in FP64 (the row of grid.cu:1277-1461 in FP64): r = f(w) - K u with u
+// the FP64 solution, f the macro-strain loads rebuilt from the cell weights in FP64 (kchi64 = KE chi0
+// in FP64) and every product accumulated in FP64.  Writes r rounded to FP32 (the inner solve's
+// right-hand side) and per-block FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0
+// (r = -K u); DO_K = false gives r = f (u = 0, nothing read from u).  All six load cases in one pass:
+// the 27 FP64 blocks are built once per vertex and applied to every load case.  The neighbour loop
+// stays rolled (d is uniform over the wave): fully unrolled, the FP64 gather spills.
 template <bool DO_F, bool DO_K>
 __global__ __launch_bounds__(kFullThreads, 1) void k_fine_residual64(Lay L, size_t N, const double* __restrict__ ke,
                                                                          const double* __restrict__ kchi,
