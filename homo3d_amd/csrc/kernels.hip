@@ -1498,6 +1498,8 @@ __global__ void k_galerkin_fine_pin(Lay Lf, Lay Lc, const float* __restrict__ ke
 }
 
 // Coarser stencil from a stored one: st_c[V][D] = sum_{o,d} hat(o) A_f[2V+o][d] hat3(o+d-2D).
+// One thread per coarse vertex and plane Dz (blockIdx.y): every fine block (o, d) is read once and
+// added into the nine directions (Dx, Dy) of the plane it reaches, with the weights hat3(o + d - 2D).
 template <typename SI, typename SO>
 __global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const SI* __restrict__ stf,
                                                          SO* __restrict__ stc, float clamp) {
@@ -1510,34 +1512,40 @@ __global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const S
     const uint32_t ys[3] = {Lf.part(wrap_dec(2 * Y, nf), 1), Lf.part(2 * Y, 1), Lf.part(2 * Y + 1, 1)};
     const uint32_t zs[3] = {Lf.part(wrap_dec(2 * Z, nf), 2), Lf.part(2 * Z, 2), Lf.part(2 * Z + 1, 2)};
     const Rsrc rf = make_rsrc(stf, (size_t)StencilIO<SI>::words * Nf * 4);
-    {
-        const int D = blockIdx.y;  // one coarse stencil direction per grid row
-        const int Dx = D % 3 - 1, Dy = D / 3 % 3 - 1, Dz = D / 9 - 1;
-        float S[9];
+    const int Dz = (int)blockIdx.y - 1;
+    float S[9][9];  // [(Dy + 1) * 3 + Dx + 1][entry]
 #pragma unroll
-        for (int k = 0; k < 9; k++) S[k] = 0.f;
-        for (int o = 0; o < 27; o++) {
-            const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
-            const float wo = hat(ox) * hat(oy) * hat(oz);
-            const size_t vf = xs[ox + 1] + ys[oy + 1] + zs[oz + 1];
-            for (int d = 0; d < 27; d++) {
-                const int dx = d % 3 - 1, dy = d / 3 % 3 - 1, dz = d / 9 - 1;
-                const float wd = hat(ox + dx - 2 * Dx) * hat(oy + dy - 2 * Dy) * hat(oz + dz - 2 * Dz);
-                if (wd == 0.f) continue;
-                const float f = wo * wd;
+    for (int j = 0; j < 9; j++)
 #pragma unroll
-                for (int k = 0; k < 9; k++)
-                    S[k] = fmaf(f, StencilIO<SI>::get(rf, (uint32_t)vf * 4u, (uint32_t)(Nf * 4), d * 9 + k), S[k]);
+        for (int k = 0; k < 9; k++) S[j][k] = 0.f;
+    for (int o = 0; o < 27; o++) {
+        const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
+        const float wo = hat(ox) * hat(oy) * hat(oz);
+        const uint32_t vf = xs[ox + 1] + ys[oy + 1] + zs[oz + 1];
+        for (int d = 0; d < 27; d++) {
+            const int kx = ox + d % 3 - 1, ky = oy + d / 3 % 3 - 1, kz = oz + d / 9 - 1;
+            const float wz = wo * hat(kz - 2 * Dz);
+            if (wz == 0.f) continue;
+            float e[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) e[k] = StencilIO<SI>::get(rf, vf * 4u, (uint32_t)(Nf * 4), d * 9 + k);
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                const float wj = wz * hat(ky - 2 * (j / 3 - 1)) * hat(kx - 2 * (j % 3 - 1));
+#pragma unroll
+                for (int k = 0; k < 9; k++) S[j][k] = fmaf(wj, e[k], S[j][k]);
             }
         }
-        if (D == 13) {
-            if (fabsf(S[0]) < clamp) S[0] = clamp;
-            if (fabsf(S[4]) < clamp) S[4] = clamp;
-            if (fabsf(S[8]) < clamp) S[8] = clamp;
-        }
-#pragma unroll
-        for (int k = 0; k < 9; k++) StencilIO<SO>::put(stc, Nc, V, D * 9 + k, S[k]);
     }
+    if (Dz == 0) {
+        if (fabsf(S[4][0]) < clamp) S[4][0] = clamp;
+        if (fabsf(S[4][4]) < clamp) S[4][4] = clamp;
+        if (fabsf(S[4][8]) < clamp) S[4][8] = clamp;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; j++)
+#pragma unroll
+        for (int k = 0; k < 9; k++) StencilIO<SO>::put(stc, Nc, V, ((Dz + 1) * 9 + j) * 9 + k, S[j][k]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2405,7 +2413,7 @@ void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const
 template <typename SI, typename SO>
 static void galerkin_coarse_case(hipStream_t s, const Lay& Lf, const Lay& Lc, const void* stf, void* stc, float clamp) {
     const size_t Nc = (size_t)Lc.n * Lc.n * Lc.n;
-    hipLaunchKernelGGL((k_galerkin_coarse<SI, SO>), dim3(nblk(Nc), 27), dim3(256), 0, s, Lf, Lc,
+    hipLaunchKernelGGL((k_galerkin_coarse<SI, SO>), dim3(nblk(Nc), 3), dim3(256), 0, s, Lf, Lc,
                        static_cast<const SI*>(stf), static_cast<SO*>(stc), clamp);
 }
 
