@@ -907,10 +907,18 @@ __global__ __launch_bounds__(kFullThreads, 1) void k_fine_residual64(Lay L, size
         if (DO_K) {
             const Rsrc ru = make_rsrc(u, 18 * N * sizeof(double));
             const uint32_t plane = (uint32_t)(N * sizeof(double));
+            // cell weights and index parts as plain scalars: the loop below runs over a run-time d
+            double wd[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) wd[e] = c.wd[e];
+            const uint32_t X0 = c.X[0], X1 = c.X[1], X2 = c.X[2], Y0 = c.Y[0], Y1 = c.Y[1], Y2 = c.Y[2], Z0 = c.Z[0],
+                           Z1 = c.Z[1], Z2 = c.Z[2];
 #pragma unroll 1
             for (int d = 0; d < 27; d++) {
                 const int dx = d % 3 - 1, dy = d / 3 % 3 - 1, dz = d / 9 - 1;
-                const uint32_t qo = c.nb(d) * (uint32_t)sizeof(double);
+                const uint32_t qo = ((dx < 0 ? X0 : (dx == 0 ? X1 : X2)) + (dy < 0 ? Y0 : (dy == 0 ? Y1 : Y2)) +
+                                     (dz < 0 ? Z0 : (dz == 0 ? Z1 : Z2))) *
+                                    (uint32_t)sizeof(double);
                 double un[R][3];
 #pragma unroll
                 for (int r = 0; r < R; r++)
@@ -923,13 +931,14 @@ __global__ __launch_bounds__(kFullThreads, 1) void k_fine_residual64(Lay L, size
                 for (int e = 0; e < 8; e++) {
                     // cell e at offset (e&1, e>>1&1, e>>2) - 1 holds v as node 7 - e and v + d as node b
                     const int bx = dx + 1 - (e & 1), by = dy + 1 - ((e >> 1) & 1), bz = dz + 1 - (e >> 2);
-                    if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
-                    const int a = 7 - e, b = bx + 2 * by + 4 * bz;
+                    if (bx >= 0 && bx <= 1 && by >= 0 && by <= 1 && bz >= 0 && bz <= 1) {
+                        const int a = 7 - e, b = bx + 2 * by + 4 * bz;
 #pragma unroll
-                    for (int i = 0; i < 3; i++)
+                        for (int i = 0; i < 3; i++)
 #pragma unroll
-                        for (int j = 0; j < 3; j++)
-                            S[i * 3 + j] = fma(c.wd[e], ke_s[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
+                            for (int j = 0; j < 3; j++)
+                                S[i * 3 + j] = fma(wd[e], ke_s[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
+                    }
                 }
 #pragma unroll
                 for (int r = 0; r < R; r++)

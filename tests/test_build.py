@@ -1,8 +1,9 @@
 """Build-level guards for the gfx950 kernels (CPU only: hipcc cross-compiles without a GPU).
 
 The fine-level kernels pipeline their neighbour loads; small source changes can make the compiler
-spill that pipeline to scratch memory, which slows the operator by an order of magnitude without
-changing its results.  This test compiles kernels.hip for gfx950 and requires zero VGPR spills.
+spill that pipeline to scratch memory, or place a small array indexed with a run-time value there,
+which slows a kernel several times over without changing its results.  This test compiles
+kernels.hip for gfx950 and requires zero VGPR spills and zero scratch in every kernel.
 """
 import os
 import re
@@ -25,6 +26,9 @@ def test_no_register_spills(tmp_path):
     assert res.returncode == 0, res.stderr[-2000:]
     names = re.findall(r"Function Name: (\S+)", res.stderr)
     spills = [int(v) for v in re.findall(r"VGPRs Spill: (\d+)", res.stderr)]
-    assert names and len(names) == len(spills)
+    scratch = [int(v) for v in re.findall(r"ScratchSize \[bytes/lane\]: (\d+)", res.stderr)]
+    assert names and len(names) == len(spills) == len(scratch)
     spilled = [(n, s) for n, s in zip(names, spills) if s]
     assert not spilled, spilled
+    in_scratch = [(n, s) for n, s in zip(names, scratch) if s]
+    assert not in_scratch, in_scratch
