@@ -362,7 +362,7 @@ bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
             launch_fine_gs(h->stream, L.n, c, omega, reverse, zs, h->d_ke, h->d_w, u, u_half, f);
             prof_end(h, ev);
         } else {
-            launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, L.st_half, static_cast<float*>(u), f);
+            launch_stencil_gs(h->stream, L.n, c, omega, reverse, zs, L.st, L.st_half, u, u_half, f);
         }
     }
     return false;
@@ -376,7 +376,7 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
         launch_fine_apply(h->stream, L.n, h->d_ke, h->mf(), h->d_w, u, u_half, f, r);
         prof_end(h, ev);
     } else {
-        launch_stencil_apply(h->stream, L.n, L.lex, L.st, L.st_half, static_cast<const float*>(u), f, r);
+        launch_stencil_apply(h->stream, L.n, L.lex, L.st, L.st_half, u, u_half, f, r);
     }
 }
 
@@ -388,11 +388,17 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
 void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
             int u0_half = 0, double* dot_out = nullptr) {
     const float omega = (float)h->cfg.sor_omega;
+    // packed FP16 u: level 0 when the caller's iterate is, and then level 1 too where its smoother runs
+    // six load cases per launch (the preconditioner's correction; coarser levels stay FP32)
+    auto u_half_of = [&](int l) {
+        if (l == 0) return u0_half;
+        return l == 1 && u0_half && l < h->nl - 1 && stencil_u_half_ok(h->lv[1].n, h->lv[1].lex) ? 1 : 0;
+    };
     for (int l = 0; l < h->nl - 1; l++) {
         Level& L = h->lv[l];
         void* u = l == 0 ? u0 : L.u;
         const float* f = l == 0 ? f0 : L.f;
-        const int uh = l == 0 ? u0_half : 0;
+        const int uh = u_half_of(l);
         op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero, uh);
         op_residual(h, l, u, f, L.r, uh);
         if (l == 0 && rn2) launch_dot6(h->stream, 3 * L.N, L.r, L.r, h->d_partial, kDotBlocks, rn2);
@@ -404,8 +410,8 @@ void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* 
         Level& L = h->lv[l];
         void* u = l == 0 ? u0 : L.u;
         const float* f = l == 0 ? f0 : L.f;
-        const int uh = l == 0 ? u0_half : 0;
-        launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, u, uh, l == 0 ? 1 : 0);
+        const int uh = u_half_of(l);
+        launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, u_half_of(l + 1), u, uh, l == 0 ? 1 : 0);
         const bool fused = op_smooth(h, l, u, f, omega, symmetric ? 1 : 0, false, uh, l == 0 ? dot_out : nullptr);
         if (l == 0 && dot_out && !fused) launch_dot_rz(h->stream, 3 * L.N, f0, u0, uh, h->d_partial, kDotBlocks, dot_out);
     }
@@ -1078,7 +1084,7 @@ int homo3d_restrict(homo3d_handle* h, int level) {
 int homo3d_prolong(homo3d_handle* h, int level) {
     if (!h || level < 0 || level >= h->nl - 1) return HOMO3D_BAD_ARG;
     if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
-    launch_prolong(h->stream, h->lv[level].n, h->lv[level + 1].lex, field_ptr(h, level + 1, 0), field_ptr(h, level, 0), 0,
+    launch_prolong(h->stream, h->lv[level].n, h->lv[level + 1].lex, field_ptr(h, level + 1, 0), 0, field_ptr(h, level, 0), 0,
                    level == 0 ? 1 : 0);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;

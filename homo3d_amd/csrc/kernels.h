@@ -57,12 +57,15 @@ void launch_pack_half(hipStream_t s, size_t N, const float* u, void* z);
 void launch_unpack_half(hipStream_t s, size_t N, const void* z, float* u);
 void launch_zero_pin64(hipStream_t s, size_t N, double* u);
 // Coarse stencils: FP32 [243][n^3] or, with *_half, packed FP16 [122][n^3] words (entries 2w, 2w + 1).
-void launch_stencil_apply(hipStream_t s, int n, int lex, const void* st, int st_half, const float* u, const float* f,
-                          float* out);
+// u FP32 planes, or (u_half, only where stencil_u_half_ok: six load cases per launch) packed FP16 pairs.
+int stencil_u_half_ok(int n, int lex);
+void launch_stencil_apply(hipStream_t s, int n, int lex, const void* st, int st_half, const void* u, int u_half,
+                          const float* f, float* out);
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const void* st,
-                       int st_half, float* u, const float* f);
+                       int st_half, void* u, int u_half, const float* f);
 void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc);
-void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, void* uf, int u_half, int pin);
+// uf += P uc.  uc FP32, or packed FP16 (uc_half: parity-blocked coarse level, packed FP16 fine u).
+void launch_prolong(hipStream_t s, int nf, int lexc, const void* uc, int uc_half, void* uf, int u_half, int pin);
 void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, void* stc,
                           int out_half, float clamp);
 void launch_galerkin_coarse(hipStream_t s, int nf, int lexc, const void* stf, int in_half, void* stc, int out_half,

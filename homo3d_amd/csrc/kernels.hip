@@ -1067,16 +1067,18 @@ struct StencilIO<half2_t> {
     }
 };
 
-template <int R, uint32_t MASK, typename SA>
+// A: storage of u (float planes, or half2_t words of load-case pairs; r0 then even).
+template <int R, uint32_t MASK, typename SA, typename A = float>
 __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L, size_t N, uint32_t v,
-                                               const SA* __restrict__ st, const float* __restrict__ u,
-                                               float acc[R][3], int r0) {
+                                               const SA* __restrict__ st, const A* __restrict__ u,
+                                               float (&acc)[R][3], int r0) {
+    constexpr int W = R / CasesPerWord<A>::value;
     const uint32_t X[3] = {L.part(wrap_dec(x, L.n), 0), L.part(x, 0), L.part(wrap_inc(x, L.n), 0)};
     const uint32_t Y[3] = {L.part(wrap_dec(y, L.n), 1), L.part(y, 1), L.part(wrap_inc(y, L.n), 1)};
     const uint32_t Z[3] = {L.part(wrap_dec(z, L.n), 2), L.part(z, 2), L.part(wrap_inc(z, L.n), 2)};
     const Rsrc rs = make_rsrc(st, (size_t)StencilIO<SA>::words * N * 4);
-    const Rsrc ru = make_rsrc(u, (size_t)18 * N * sizeof(float));
-    const uint32_t plane = (uint32_t)(N * sizeof(float));
+    const Rsrc ru = make_rsrc(u, (size_t)(18 / CasesPerWord<A>::value) * N * sizeof(A));
+    const uint32_t plane = (uint32_t)(N * sizeof(float));  // stencil and u words are 4 bytes
     const uint32_t vo = v * 4u;
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -1089,22 +1091,16 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L
 #pragma unroll
         for (int k = 0; k < 9; k++) S[k] = StencilIO<SA>::get(rs, vo, plane, d * 9 + k);
         const uint32_t qo = (X[d % 3] + Y[d / 3 % 3] + Z[d / 9]) * 4u;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const float u0 = bload<float>(ru, qo, (uint32_t)((r0 + r) * 3 + 0) * plane);
-            const float u1 = bload<float>(ru, qo, (uint32_t)((r0 + r) * 3 + 1) * plane);
-            const float u2 = bload<float>(ru, qo, (uint32_t)((r0 + r) * 3 + 2) * plane);
-            acc[r][0] = fmaf(S[0], u0, fmaf(S[1], u1, fmaf(S[2], u2, acc[r][0])));
-            acc[r][1] = fmaf(S[3], u0, fmaf(S[4], u1, fmaf(S[5], u2, acc[r][1])));
-            acc[r][2] = fmaf(S[6], u0, fmaf(S[7], u1, fmaf(S[8], u2, acc[r][2])));
-        }
+        A cur[W][3];
+        load_nb<W, A>(cur, ru, qo, plane, r0);
+        block_fma<W, float, A>(S, cur, acc);
     }
 }
 
-// MODE 0: out = A u; MODE 1: out = f - A u (update_residual_kernel_1, grid.cu:1465-1564)
-template <int R, int MODE, typename SA>
+// MODE 0: out = A u; MODE 1: out = f - A u (update_residual_kernel_1, grid.cu:1465-1564).  A: u storage.
+template <int R, int MODE, typename SA, typename A = float>
 __global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N, const SA* __restrict__ st,
-                                                                const float* __restrict__ u,
+                                                                const A* __restrict__ u,
                                                                 const float* __restrict__ f, float* __restrict__ out) {
     uint32_t v;
     if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
@@ -1112,7 +1108,7 @@ __global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N,
     L.pos(v, x, y, z);
     float acc[R][3];
     const int r0 = blockIdx.y * R;
-    stencil_gather<R, kRowAll, SA>(x, y, z, L, N, v, st, u, acc, r0);
+    stencil_gather<R, kRowAll, SA, A>(x, y, z, L, N, v, st, u, acc, r0);
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -1123,10 +1119,10 @@ __global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N,
 }
 
 // One colour of the coarse block GS sweep (gs_relaxation_kernel, grid.cu:556-689); no pin on coarse levels.
-// ZC as in k_fine_gs.
-template <int R, bool REVERSE, int ZC, typename SA>
+// ZC as in k_fine_gs.  A: u storage (float, or half2_t load-case pairs with FP32 arithmetic).
+template <int R, bool REVERSE, int ZC, typename SA, typename A = float>
 __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour, float omega,
-                                                    const SA* __restrict__ st, float* __restrict__ u,
+                                                    const SA* __restrict__ st, A* __restrict__ u,
                                                     const float* __restrict__ f) {
     const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.h3) return;
@@ -1135,23 +1131,41 @@ __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour,
     L.pos(v, x, y, z);
     float acc[R][3];
     const int r0 = blockIdx.y * R;
-    stencil_gather<R, gs_mask(ZC), SA>(x, y, z, L, N, v, st, u, acc, r0);
+    stencil_gather<R, gs_mask(ZC), SA, A>(x, y, z, L, N, v, st, u, acc, r0);
     float Ks[9];
     const Rsrc rs = make_rsrc(st, (size_t)StencilIO<SA>::words * N * 4);
 #pragma unroll
     for (int k = 0; k < 9; k++) Ks[k] = StencilIO<SA>::get(rs, v * 4u, (uint32_t)(N * 4), 13 * 9 + k);
     const float inv[3] = {1.f / Ks[0], 1.f / Ks[4], 1.f / Ks[8]};
+    constexpr int cpw = CasesPerWord<A>::value;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        float uu[3], ff[3];
+    for (int p = 0; p < R / cpw; p++) {
+        A wd[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            uu[i] = ZC >= 0 ? 0.f : u[(size_t)((r0 + r) * 3 + i) * N + v];
-            ff[i] = f[(size_t)((r0 + r) * 3 + i) * N + v];
+        for (int i = 0; i < 3; i++) wd[i] = ZC >= 0 ? A(0) : u[(size_t)((r0 / cpw + p) * 3 + i) * N + v];
+#pragma unroll
+        for (int hh = 0; hh < cpw; hh++) {
+            const int r = p * cpw + hh;
+            float uu[3], ff[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if constexpr (cpw == 2)
+                    uu[i] = (float)wd[i][hh];
+                else
+                    uu[i] = (float)wd[i];
+                ff[i] = f[(size_t)((r0 + r) * 3 + i) * N + v];
+            }
+            block_update<REVERSE>(uu, ff, acc[r], Ks, inv, omega);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if constexpr (cpw == 2)
+                    wd[i][hh] = (_Float16)uu[i];
+                else
+                    wd[i] = uu[i];
+            }
         }
-        block_update<REVERSE>(uu, ff, acc[r], Ks, inv, omega);
 #pragma unroll
-        for (int i = 0; i < 3; i++) u[(size_t)((r0 + r) * 3 + i) * N + v] = uu[i];
+        for (int i = 0; i < 3; i++) u[(size_t)((r0 / cpw + p) * 3 + i) * N + v] = wd[i];
     }
 }
 
@@ -1348,7 +1362,9 @@ __global__ __launch_bounds__(kFullThreads) void k_prolong4(Lay Lf, Lay Lc, const
 
 // k_prolong4 for a fine field in packed FP16 (half2_t word planes): a thread takes the load-case pair
 // blockIdx.y (0..2), both halves of each word.
-__global__ __launch_bounds__(kFullThreads) void k_prolong4h(Lay Lf, Lay Lc, const float* __restrict__ uc,
+// CT: coarse storage, FP32 planes or packed FP16 pairs (the latter on parity-blocked levels only).
+template <typename CT>
+__global__ __launch_bounds__(kFullThreads) void k_prolong4h(Lay Lf, Lay Lc, const CT* __restrict__ uc,
                                                             half2_t* __restrict__ uf, int pin) {
     const uint32_t hf = (uint32_t)Lf.h, q = hf / 4;
     const uint32_t t4 = xcd_chunk(blockIdx.x, gridDim.x) * 64u + (threadIdx.x & 63u);
@@ -1367,29 +1383,51 @@ __global__ __launch_bounds__(kFullThreads) void k_prolong4h(Lay Lf, Lay Lc, cons
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float a[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const float* cp = uc + (size_t)((2 * pr + h) * 3 + c) * Nc;
+        if constexpr (CasesPerWord<CT>::value == 2) {
+            // one word holds both load cases of the pair (parity-blocked coarse level)
+            const half2_t* cp = uc + (size_t)(pr * 3 + c) * Nc;
             for (int iz = 0; iz <= pz; iz++)
                 for (int iy = 0; iy <= py; iy++) {
-                    float cv[5];
-                    if (Lc.lex) {
-                        const uint32_t base = (uint32_t)nc * (Ys[iy] + (uint32_t)nc * Zs[iz]);
-                        const float4 v4 = *reinterpret_cast<const float4*>(cp + base + 4 * j);
-                        cv[0] = v4.x; cv[1] = v4.y; cv[2] = v4.z; cv[3] = v4.w;
-                        cv[4] = px ? cp[base + X4] : 0.f;
-                    } else {
-                        const uint32_t yz = Lc.part(Ys[iy], 1) + Lc.part(Zs[iz], 2);
-                        const float2 ev = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j, 0));
-                        const float2 od = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j + 1, 0));
-                        cv[0] = ev.x; cv[1] = od.x; cv[2] = ev.y; cv[3] = od.y;
-                        cv[4] = px ? cp[yz + Lc.part(X4, 0)] : 0.f;
-                    }
+                    const uint32_t yz = Lc.part(Ys[iy], 1) + Lc.part(Zs[iz], 2);
+                    const uint2 ev = *reinterpret_cast<const uint2*>(cp + yz + Lc.part(4 * j, 0));
+                    const uint2 od = *reinterpret_cast<const uint2*>(cp + yz + Lc.part(4 * j + 1, 0));
+                    const half2_t cw[5] = {__builtin_bit_cast(half2_t, ev.x), __builtin_bit_cast(half2_t, od.x),
+                                           __builtin_bit_cast(half2_t, ev.y), __builtin_bit_cast(half2_t, od.y),
+                                           px ? cp[yz + Lc.part(X4, 0)] : half2_t(0)};
                     const float w = wy[iy] * wz[iz];
 #pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        a[h][k] = fmaf(w, px ? 0.5f * (cv[k] + cv[k + 1]) : cv[k], a[h][k]);
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const float c0 = (float)cw[k][h], c1 = (float)cw[k + 1][h];
+                            a[h][k] = fmaf(w, px ? 0.5f * (c0 + c1) : c0, a[h][k]);
+                        }
                 }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const float* cp = uc + (size_t)((2 * pr + h) * 3 + c) * Nc;
+                for (int iz = 0; iz <= pz; iz++)
+                    for (int iy = 0; iy <= py; iy++) {
+                        float cv[5];
+                        if (Lc.lex) {
+                            const uint32_t base = (uint32_t)nc * (Ys[iy] + (uint32_t)nc * Zs[iz]);
+                            const float4 v4 = *reinterpret_cast<const float4*>(cp + base + 4 * j);
+                            cv[0] = v4.x; cv[1] = v4.y; cv[2] = v4.z; cv[3] = v4.w;
+                            cv[4] = px ? cp[base + X4] : 0.f;
+                        } else {
+                            const uint32_t yz = Lc.part(Ys[iy], 1) + Lc.part(Zs[iz], 2);
+                            const float2 ev = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j, 0));
+                            const float2 od = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j + 1, 0));
+                            cv[0] = ev.x; cv[1] = od.x; cv[2] = ev.y; cv[3] = od.y;
+                            cv[4] = px ? cp[yz + Lc.part(X4, 0)] : 0.f;
+                        }
+                        const float w = wy[iy] * wz[iz];
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            a[h][k] = fmaf(w, px ? 0.5f * (cv[k] + cv[k + 1]) : cv[k], a[h][k]);
+                    }
+            }
         }
         half2_t* dst = uf + (size_t)(pr * 3 + c) * Nf + b * Lf.h3 + row * hf + 4 * j;
         uint4 raw = *reinterpret_cast<const uint4*>(dst);
@@ -2303,46 +2341,51 @@ void launch_zero_pin64(hipStream_t s, size_t N, double* u) {
 // there.  Larger levels keep all six load cases per thread so the stored stencil is read once.
 constexpr int kSplitLoadCasesMaxN = 32;
 
-template <int R, typename SA>
-static void stencil_apply_case(hipStream_t s, const Lay& L, size_t N, const SA* st, const float* u, const float* f,
+template <int R, typename SA, typename A>
+static void stencil_apply_case(hipStream_t s, const Lay& L, size_t N, const SA* st, const A* u, const float* f,
                                float* out) {
     const dim3 g(full_grid(L), 6 / R), b(kFullThreads);
     if (f)
-        hipLaunchKernelGGL((k_stencil_apply<R, 1, SA>), g, b, 0, s, L, N, st, u, f, out);
+        hipLaunchKernelGGL((k_stencil_apply<R, 1, SA, A>), g, b, 0, s, L, N, st, u, f, out);
     else
-        hipLaunchKernelGGL((k_stencil_apply<R, 0, SA>), g, b, 0, s, L, N, st, u, f, out);
+        hipLaunchKernelGGL((k_stencil_apply<R, 0, SA, A>), g, b, 0, s, L, N, st, u, f, out);
 }
 
-void launch_stencil_apply(hipStream_t s, int n, int lex, const void* st, int st_half, const float* u, const float* f,
-                          float* out) {
+template <typename SA>
+static void stencil_apply_pick(hipStream_t s, const Lay& L, size_t N, bool split, const SA* st, const void* u,
+                               int u_half, const float* f, float* out) {
+    if (u_half)
+        stencil_apply_case<6>(s, L, N, st, static_cast<const half2_t*>(u), f, out);
+    else if (split)
+        stencil_apply_case<1>(s, L, N, st, static_cast<const float*>(u), f, out);
+    else
+        stencil_apply_case<6>(s, L, N, st, static_cast<const float*>(u), f, out);
+}
+
+int stencil_u_half_ok(int n, int lex) { return !lex && n > kSplitLoadCasesMaxN && n % 8 == 0; }
+
+void launch_stencil_apply(hipStream_t s, int n, int lex, const void* st, int st_half, const void* u, int u_half,
+                          const float* f, float* out) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, lex);
     const bool split = n <= kSplitLoadCasesMaxN;
-    if (st_half) {
-        const half2_t* sh = static_cast<const half2_t*>(st);
-        if (split)
-            stencil_apply_case<1>(s, L, N, sh, u, f, out);
-        else
-            stencil_apply_case<6>(s, L, N, sh, u, f, out);
-    } else {
-        const float* sf = static_cast<const float*>(st);
-        if (split)
-            stencil_apply_case<1>(s, L, N, sf, u, f, out);
-        else
-            stencil_apply_case<6>(s, L, N, sf, u, f, out);
-    }
+    if (u_half && !stencil_u_half_ok(n, lex)) return;  // the caller checks stencil_u_half_ok
+    if (st_half)
+        stencil_apply_pick(s, L, N, split, static_cast<const half2_t*>(st), u, u_half, f, out);
+    else
+        stencil_apply_pick(s, L, N, split, static_cast<const float*>(st), u, u_half, f, out);
 }
 
-template <int R, bool REV, int ZC, typename SA>
-static void stencil_gs_colour(hipStream_t s, const Lay& L, size_t N, int colour, float omega, const SA* st, float* u,
+template <int R, bool REV, int ZC, typename SA, typename A>
+static void stencil_gs_colour(hipStream_t s, const Lay& L, size_t N, int colour, float omega, const SA* st, A* u,
                               const float* f) {
-    hipLaunchKernelGGL((k_stencil_gs<R, REV, ZC, SA>), dim3(nblk(L.h3), 6 / R), dim3(256), 0, s, L, N, colour, omega,
+    hipLaunchKernelGGL((k_stencil_gs<R, REV, ZC, SA, A>), dim3(nblk(L.h3), 6 / R), dim3(256), 0, s, L, N, colour, omega,
                        st, u, f);
 }
 
-template <int R, typename SA>
+template <int R, typename SA, typename A>
 static void stencil_gs_pick(hipStream_t s, const Lay& L, size_t N, int colour, float omega, int reverse,
-                            int zero_start, const SA* st, float* u, const float* f) {
+                            int zero_start, const SA* st, A* u, const float* f) {
     if (reverse) {
         stencil_gs_colour<R, true, -1>(s, L, N, colour, omega, st, u, f);
         return;
@@ -2360,24 +2403,27 @@ static void stencil_gs_pick(hipStream_t s, const Lay& L, size_t N, int colour, f
     }
 }
 
+template <typename SA>
+static void stencil_gs_st(hipStream_t s, const Lay& L, size_t N, bool split, int colour, float omega, int reverse,
+                          int zero_start, const SA* st, void* u, int u_half, const float* f) {
+    if (u_half)
+        stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, st, static_cast<half2_t*>(u), f);
+    else if (split)
+        stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, st, static_cast<float*>(u), f);
+    else
+        stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, st, static_cast<float*>(u), f);
+}
+
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const void* st,
-                       int st_half, float* u, const float* f) {
+                       int st_half, void* u, int u_half, const float* f) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
     const bool split = n <= kSplitLoadCasesMaxN;
-    if (st_half) {
-        const half2_t* sh = static_cast<const half2_t*>(st);
-        if (split)
-            stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, sh, u, f);
-        else
-            stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, sh, u, f);
-    } else {
-        const float* sf = static_cast<const float*>(st);
-        if (split)
-            stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, sf, u, f);
-        else
-            stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, sf, u, f);
-    }
+    if (u_half && !stencil_u_half_ok(n, 0)) return;  // the caller checks stencil_u_half_ok
+    if (st_half)
+        stencil_gs_st(s, L, N, split, colour, omega, reverse, zero_start, static_cast<const half2_t*>(st), u, u_half, f);
+    else
+        stencil_gs_st(s, L, N, split, colour, omega, reverse, zero_start, static_cast<const float*>(st), u, u_half, f);
 }
 
 // Transfers take one load case per thread (six grid rows): the load cases' planes are disjoint, so
@@ -2391,17 +2437,21 @@ void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc
         hipLaunchKernelGGL((k_restrict<1>), dim3(nblk(Nc), 6), dim3(256), 0, s, Lf, Lc, rf, fc);
 }
 
-void launch_prolong(hipStream_t s, int nf, int lexc, const float* uc, void* uf, int u_half, int pin) {
+void launch_prolong(hipStream_t s, int nf, int lexc, const void* uc, int uc_half, void* uf, int u_half, int pin) {
     const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
-    if (u_half)  // packed FP16 fine fields are only used with nf % 8 == 0
-        hipLaunchKernelGGL(k_prolong4h, dim3(nblk(Lf.h3 / 4, 64), 3), dim3(kFullThreads), 0, s, Lf, Lc, uc,
-                           static_cast<half2_t*>(uf), pin);
+    if (uc_half && (!u_half || lexc)) return;  // packed FP16 coarse u: parity-blocked, into a packed FP16 fine u
+    if (u_half && uc_half)
+        hipLaunchKernelGGL((k_prolong4h<half2_t>), dim3(nblk(Lf.h3 / 4, 64), 3), dim3(kFullThreads), 0, s, Lf, Lc,
+                           static_cast<const half2_t*>(uc), static_cast<half2_t*>(uf), pin);
+    else if (u_half)  // packed FP16 fine fields are only used with nf % 8 == 0
+        hipLaunchKernelGGL((k_prolong4h<float>), dim3(nblk(Lf.h3 / 4, 64), 3), dim3(kFullThreads), 0, s, Lf, Lc,
+                           static_cast<const float*>(uc), static_cast<half2_t*>(uf), pin);
     else if (nf % 8 == 0)
-        hipLaunchKernelGGL(k_prolong4, dim3(nblk(Lf.h3 / 4, 64), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc,
-                           static_cast<float*>(uf), pin);
+        hipLaunchKernelGGL(k_prolong4, dim3(nblk(Lf.h3 / 4, 64), 6), dim3(kFullThreads), 0, s, Lf, Lc,
+                           static_cast<const float*>(uc), static_cast<float*>(uf), pin);
     else
-        hipLaunchKernelGGL((k_prolong<1>), dim3(full_grid(Lf), 6), dim3(kFullThreads), 0, s, Lf, Lc, uc,
-                           static_cast<float*>(uf), pin);
+        hipLaunchKernelGGL((k_prolong<1>), dim3(full_grid(Lf), 6), dim3(kFullThreads), 0, s, Lf, Lc,
+                           static_cast<const float*>(uc), static_cast<float*>(uf), pin);
 }
 
 void launch_galerkin_fine(hipStream_t s, int nf, int lexc, const float* G, const float* ke, const float* w, void* stc,
