@@ -388,11 +388,11 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
 void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
             int u0_half = 0, double* dot_out = nullptr) {
     const float omega = (float)h->cfg.sor_omega;
-    // packed FP16 u: level 0 when the caller's iterate is, and then level 1 too where its smoother runs
-    // six load cases per launch (the preconditioner's correction; coarser levels stay FP32)
+    // packed FP16 u: level 0 when the caller's iterate is, and then every coarser level (but the
+    // coarsest) whose smoother runs six load cases per launch; the small levels stay FP32
     auto u_half_of = [&](int l) {
         if (l == 0) return u0_half;
-        return l == 1 && u0_half && l < h->nl - 1 && stencil_u_half_ok(h->lv[1].n, h->lv[1].lex) ? 1 : 0;
+        return u0_half && l < h->nl - 1 && stencil_u_half_ok(h->lv[l].n, h->lv[l].lex) ? 1 : 0;
     };
     for (int l = 0; l < h->nl - 1; l++) {
         Level& L = h->lv[l];
