@@ -100,6 +100,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--omega", type=float, default=None, help="smoother relaxation factor (default: the library's)")
     ap.add_argument("--mfma", type=int, default=None,
                     help="finest-level operator on FP16 fields in element form on the matrix cores (1) or row form (0); "
                          "default: the library default")
@@ -118,6 +119,8 @@ def main():
     x = torch.from_numpy((rho.astype(np.float64) ** 3).astype(np.float32)).to("cuda:0")
     torch.cuda.synchronize()
     kw = {} if args.mfma is None else {"fine_mfma": bool(args.mfma)}
+    if args.omega is not None:
+        kw["sor_omega"] = args.omega
     hom = homo3d_amd.Homogenization(homo3d_amd.HomoConfig(reso=n, solver="pcg", max_iter=400, **kw))
 
     def step():

@@ -454,8 +454,8 @@ int stage_in(homo3d_handle* h, const float* a, int pitch, int on_device, const f
 }
 
 constexpr int kMaxOuter = 12;
-// Inner solve target per refinement round (measured at 256^3: 1e-4 and 1e-6 both cost more CG steps
-// than they save in FP64 residual rounds).
+// Inner solve target per refinement round (measured at 256^3: 1e-4 and 1e-6 cost more CG steps than
+// they save in FP64 residual rounds; 3e-3 and 1e-2 need a third round).
 constexpr double kInnerTol = 1e-3;
 
 // MG-preconditioned CG on K x = r for the load cases not marked done: x (h->X) starts at zero, r
