@@ -1,4 +1,8 @@
-"""ctypes binding of include/homo3d.h and the reference-named Python façade."""
+"""ctypes binding of include/homo3d.h and the reference-named Python façade.
+
+In a process that also uses PyTorch-ROCm, import torch first: torch ships its own HIP runtime, and
+the one loaded first serves the process; with libhomo3d.so's loaded first, homo3d_create fails once
+torch has initialised the GPU."""
 import ctypes
 import os
 import time
