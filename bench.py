@@ -6,9 +6,11 @@ resident in HBM.  It runs Homogenization::update (Galerkin coarse operators + co
 factorisation), the six macro-strain load cases by MG-PCG from a zero initial guess to
 relres = ||f - K u|| / ||f|| <= 1e-5 (FP64 norms), and C^H.
 
-Prints one JSON line.  The dominant kernel's average launch time is measured with HIP events on the
-library's stream during the timed steps; the CPU baseline is the FP64 oracle (test
-infrastructure) timed on the host on a bounded sample.
+Prints one JSON line.  The dominant kernel's (the CG operator pass, fine_apply) average launch time
+is measured with HIP events on the library's stream during the timed steps, with event pairs around
+that kernel only; one more, untimed step with events around every finest-level kind logs their
+launch times.  The CPU baseline is the FP64 oracle (test infrastructure) timed on the host on a
+bounded sample.
 """
 import argparse
 import json
@@ -31,6 +33,7 @@ PROF_KERNEL = {"fine_gs": ("k_fine_gs2<6, true, -1", "k_fine_gs2ILi6ELb1ELin1E",
                "fine_apply": ("k_fine_applyILi6ELi0ELb1EDv2_DF16_Lb1ELi2E", "k_fine_apply_efILi0ELb1ELb1ELi1ELi2E",
                               "k_fine_apply_efILi0ELb1ELb1ELi2ELi2E", "k_fine_apply<6, 0, true, half2, true, 2>"),
                "fine_residual": ("k_fine_apply<6, 1, false", "k_fine_applyILi6ELi1ELb0E")}
+DOMINANT = "fine_apply"  # the kernel with the largest total time per step
 METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; stencil GB/s"
 
 
@@ -132,7 +135,7 @@ def main():
     for i in range(args.warmup):
         st, C = step()
         log("warmup %d: iters %s relres %.2e %.2fs" % (i, st["iters"], max(st["relres"]), st["seconds"]))
-    hom.profile_enable(True)
+    hom.profile_enable(True, only=DOMINANT)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
@@ -142,17 +145,22 @@ def main():
         log("step %d: iters %s relres %.2e" % (i, st["iters"], max(st["relres"])))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    prof = {k: hom.profile_get(k) for k in PROF_KINDS}
+    dom = DOMINANT
+    cnt, ms = hom.profile_get(dom)
     hom.profile_enable(False)
     s_z = 2 if n % 8 == 0 else 4
-    for k, (cnt, ms) in prof.items():
-        if cnt:
+    # untimed step with events around every kind, for the per-kernel log
+    hom.profile_enable(True)
+    step()
+    prof = {k: hom.profile_get(k) for k in PROF_KINDS}
+    hom.profile_enable(False)
+    for k, (c, t) in prof.items():
+        if c:
             kb = kernel_bytes(k, n, s_z=s_z)
-            log("kernel %-14s launches %6d total %9.2f ms avg %8.4f ms%s" %
-                (k, cnt, ms, ms / cnt, "  %.1f GB/s algorithmic" % (kb / (ms / cnt * 1e-3) / 1e9) if kb else ""))
-    # dominant kernel: the largest total among those with a fixed per-launch byte count
-    dom = max((k for k in prof if kernel_bytes(k, n) is not None), key=lambda k: prof[k][1])
-    cnt, ms = prof[dom]
+            log("kernel %-16s launches %4d total %8.2f ms avg %8.4f ms%s" %
+                (k, c, t, t / c, "  %.1f GB/s algorithmic" % (kb / (t / c * 1e-3) / 1e9) if kb else ""))
+    if max(prof.values(), key=lambda ct: ct[1]) != prof[dom]:
+        log("note: %s is not the kernel kind with the largest total in this run" % dom)
     avg_s = ms / cnt * 1e-3
     achieved = kernel_bytes(dom, n, s_z=s_z) / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -84,6 +84,7 @@ EXPORTS = {
     "homo3d_randc_density": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_void_p]),
     "homo3d_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_profile_only": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "homo3d_profile_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long),
                                           ctypes.POINTER(ctypes.c_double)]),
 }
@@ -312,7 +313,9 @@ class Homogenization:
                                           0), "symmetrize")
         return a
 
-    def profile_enable(self, on=True):
+    def profile_enable(self, on=True, only=None):
+        """Event timing of the finest-level kernels; `only` restricts it to one kind (profile_get names)."""
+        self._chk(lib().homo3d_profile_only(self._h, (only or "").encode()), "profile_only")
         self._chk(lib().homo3d_profile_enable(self._h, int(on)), "profile_enable")
 
     def profile_get(self, kernel):

@@ -94,6 +94,7 @@ struct homo3d_handle {
     bool density_set = false;
     std::string err;
     bool prof = false;
+    int prof_only = -1;  // the one profiled kind, or -1 for all
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     std::vector<ProfEvent> pending;
@@ -312,7 +313,7 @@ void prof_flush(homo3d_handle* h) {
 
 // Event pair around one launch of a profiled kernel kind (kProfNames).
 void prof_begin(homo3d_handle* h, int kind, ProfEvent& ev) {
-    if (!h->prof) return;
+    if (!h->prof || (h->prof_only >= 0 && kind != h->prof_only)) return;
     if (h->pool_used + 2 > h->pool.size()) prof_flush(h);
     while (h->pool.size() < h->pool_used + 2 && h->pool.size() < kEventPool) {
         hipEvent_t e;
@@ -846,7 +847,7 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
         for (int outer = 0;; outer++) {
             double rn2[kRhs];
             launch_fine_residual64(h->stream, L0.n, h->d_ke64, h->d_kchi64, h->d_x, h->cfg.emin, h->U64, h->Rr,
-                                   h->d_partial, h->d_tmp, 1, u_zero ? 0 : 1);
+                                   h->d_partial, h->d_tmp, u_zero ? 0 : 1);
             HIPCHK(h, hipMemcpyAsync(rn2, h->d_tmp, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(h, hipStreamSynchronize(h->stream));
             if (outer == 0 && u_zero) std::copy(rn2, rn2 + kRhs, fn2);
@@ -1270,6 +1271,21 @@ int homo3d_profile_enable(homo3d_handle* h, int enable) {
             h->prof_ms[k] = 0;
         }
     return HOMO3D_OK;
+}
+
+int homo3d_profile_only(homo3d_handle* h, const char* kernel) {
+    if (!h) return HOMO3D_BAD_ARG;
+    prof_flush(h);
+    if (!kernel || !*kernel) {
+        h->prof_only = -1;
+        return HOMO3D_OK;
+    }
+    for (int k = 0; k < kProfKinds; k++)
+        if (std::strcmp(kernel, kProfNames[k]) == 0) {
+            h->prof_only = k;
+            return HOMO3D_OK;
+        }
+    return set_err(h, HOMO3D_BAD_ARG, "unknown profiled kernel %s", kernel);
 }
 
 int homo3d_profile_get(homo3d_handle* h, const char* kernel, long* launches, double* total_ms) {

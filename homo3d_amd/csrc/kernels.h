@@ -45,7 +45,7 @@ void launch_fine_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse,
                      int pstride = 0);
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f);
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
-                            const double* u, float* r32, double* partial, double* out6, int do_f, int do_k);
+                            const double* u, float* r32, double* partial, double* out6, int do_k);
 void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
                        double* out6);
 void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u);

@@ -189,6 +189,7 @@ __device__ __forceinline__ void fine_ctx64(FineCtx& c, int x, int y, int z, cons
 // Neighbour masks: bit d (d = 0..26, offset (d%3-1, d/3%3-1, d/9-1)) selects the neighbours a row
 // gathers.  kRowAll is the whole row, kRowOffDiag the row without the vertex itself (d = 13).
 constexpr uint32_t kRowAll = (1u << 27) - 1u;
+constexpr int kSumThreads = 1024;       // workgroup of k_sum_partials
 constexpr uint32_t kRowOffDiag = kRowAll & ~(1u << 13);
 
 // Colour c of a sweep that starts from u = 0: the neighbours of colours above c still hold zero, so
@@ -874,17 +875,15 @@ __global__ __launch_bounds__(kFullThreads) void k_macro_rhs(Lay L, size_t N, con
 // Iterative-refinement residual in FP64 (the row of grid.cu:1277-1461 in FP64): r = f(w) - K u with u
 // the FP64 solution, f the macro-strain loads rebuilt from the cell weights in FP64 (kchi64 = KE chi0
 // in FP64) and every product accumulated in FP64.  Writes r rounded to FP32 (the inner solve's
-// right-hand side) and per-block FP64 partial sums of |r|^2 per load case.  DO_F = false gives f = 0
-// (r = -K u); DO_K = false gives r = f (u = 0, nothing read from u).  All six load cases in one pass:
+// right-hand side) and per-block FP64 partial sums of |r|^2 per load case.  DO_K = false gives r = f
+// (u = 0, nothing read from u).  All six load cases in one pass:
 // the 27 FP64 blocks are built once per vertex and applied to every load case.  The neighbour loop
-// stays rolled (d is uniform over the wave): fully unrolled, the FP64 gather spills.
-template <bool DO_F, bool DO_K>
-__global__ __launch_bounds__(kFullThreads, 1) void k_fine_residual64(Lay L, size_t N, const double* __restrict__ ke,
-                                                                         const double* __restrict__ kchi,
-                                                                         const float* __restrict__ xc, double emin,
-                                                                         const double* __restrict__ u,
-                                                                         float* __restrict__ r32,
-                                                                         double* __restrict__ partial) {
+// stays rolled (d is uniform over the wave): fully unrolled, the FP64 gather spills.  At most 128
+// VGPRs, so that two 512-thread workgroups share a CU (the FP64 gather is latency-bound).
+template <bool DO_K>
+__global__ __launch_bounds__(kFullThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_fine_residual64(
+    Lay L, size_t N, const double* __restrict__ ke, const double* __restrict__ kchi, const float* __restrict__ xc,
+    double emin, const double* __restrict__ u, float* __restrict__ r32, double* __restrict__ partial) {
     constexpr int R = 6;
     __shared__ double sh[kFullThreads / 64];
     __shared__ double ke_s[576];
@@ -952,10 +951,8 @@ __global__ __launch_bounds__(kFullThreads, 1) void k_fine_residual64(Lay L, size
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 double f = 0.0;
-                if (DO_F) {
 #pragma unroll
-                    for (int e = 0; e < 8; e++) f = fma(c.wd[e], kchi[r * 24 + 3 * (7 - e) + i], f);
-                }
+                for (int e = 0; e < 8; e++) f = fma(c.wd[e], kchi[r * 24 + 3 * (7 - e) + i], f);
                 const double val = (v == 0) ? 0.0 : f - acc[r][i];
                 r32[(size_t)(r * 3 + i) * N + v] = (float)val;
                 nrm[r] = fma(val, val, nrm[r]);
@@ -1646,14 +1643,23 @@ __global__ __launch_bounds__(256) void k_dot(size_t M, const float* __restrict__
 }
 
 // out[r] = sum of partial[r * nb .. r * nb + nb), one workgroup per row r = blockIdx.x
-__global__ __launch_bounds__(256) void k_sum_partials(int nb, const double* __restrict__ partial,
-                                                      double* __restrict__ out) {
-    __shared__ double sh[4];
-    const int r = blockIdx.x;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[(size_t)r * nb + i];
-    s = block_sum<256>(s, sh);
-    if (threadIdx.x == 0) out[r] = s;
+// out[r] = sum of partial[r * nb + i] over i < nb; one workgroup per row, four independent loads in
+// flight per thread.
+__global__ __launch_bounds__(kSumThreads) void k_sum_partials(int nb, const double* __restrict__ partial,
+                                                              double* __restrict__ out) {
+    __shared__ double sh[kSumThreads / 64];
+    const double* p = partial + (size_t)blockIdx.x * nb;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * kSumThreads < nb; i += 4 * kSumThreads) {
+        s0 += p[i];
+        s1 += p[i + kSumThreads];
+        s2 += p[i + 2 * kSumThreads];
+        s3 += p[i + 3 * kSumThreads];
+    }
+    for (; i < nb; i += kSumThreads) s0 += p[i];
+    const double s = block_sum<kSumThreads>((s0 + s1) + (s2 + s3), sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 // PCG scalars per load case.  sc layout: [0..R) rz, [R..2R) pq, [2R..3R) alpha, [3R..4R) rz_new,
@@ -2189,7 +2195,7 @@ void launch_fine_apply_cg(hipStream_t s, int n, const float* ke, const FineMfma*
         else
             fine_apply_cg_row<float, false>(s, L, N, ke, w, zf, p, x, sc, q, partial);
     }
-    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)nb, partial, out6);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, (int)nb, partial, out6);
 }
 
 template <bool REV, int ZC, typename A>
@@ -2283,17 +2289,15 @@ void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, f
 // One load case per FP64 residual launch: with two, the FP64 gather pipeline no longer fits the
 // register file.
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
-                            const double* u, float* r32, double* partial, double* out6, int do_f, int do_k) {
+                            const double* u, float* r32, double* partial, double* out6, int do_k) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
     const dim3 g(full_grid(L)), b(kFullThreads);
     if (!do_k)
-        hipLaunchKernelGGL((k_fine_residual64<true, false>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
-    else if (do_f)
-        hipLaunchKernelGGL((k_fine_residual64<true, true>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
+        hipLaunchKernelGGL((k_fine_residual64<false>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
     else
-        hipLaunchKernelGGL((k_fine_residual64<false, true>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)full_grid(L), partial, out6);
+        hipLaunchKernelGGL((k_fine_residual64<true>), g, b, 0, s, L, N, ke, kchi, xc, emin, u, r32, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, (int)full_grid(L), partial, out6);
 }
 
 void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
@@ -2302,7 +2306,7 @@ void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc
     const Lay L = make_lay(n, 0);
     const unsigned nb = full_grid(L);
     hipLaunchKernelGGL(k_rhs_norm64, dim3(nb), dim3(kFullThreads), 0, s, L, N, kchi, xc, emin, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, (int)nb, partial, out6);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, (int)nb, partial, out6);
 }
 
 void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u) {
@@ -2498,11 +2502,11 @@ void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float*
 void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks,
                  double* out) {
     hipLaunchKernelGGL(k_dot, dim3(nblocks, 6), dim3(256), 0, s, M, a, b, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, nblocks, partial, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, nblocks, partial, out);
 }
 
 void launch_sum_partials(hipStream_t s, int R, int nblocks, const double* partial, double* out) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(R), dim3(256), 0, s, nblocks, partial, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(R), dim3(kSumThreads), 0, s, nblocks, partial, out);
 }
 
 void launch_pcg_alpha(hipStream_t s, double* sc) { hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(64), 0, s, 6, sc); }
@@ -2511,7 +2515,7 @@ void launch_pcg_beta(hipStream_t s, double* sc) { hipLaunchKernelGGL(k_pcg_beta,
 void launch_pcg_update_r(hipStream_t s, size_t M, const double* sc, const float* q, float* r, double* partial,
                          int nblocks, double* out6) {
     hipLaunchKernelGGL(k_pcg_update_r, dim3(nblocks, 6), dim3(256), 0, s, 6, M, sc, q, r, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, nblocks, partial, out6);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, nblocks, partial, out6);
 }
 
 void launch_pcg_update_px(hipStream_t s, size_t M, const double* sc, const void* z, int z_half, float* p, float* x,
@@ -2538,7 +2542,7 @@ void launch_dot_rz(hipStream_t s, size_t M, const float* r, const void* z, int z
         return;
     }
     hipLaunchKernelGGL(k_dot_fh, dim3(nblocks, 3), dim3(256), 0, s, M, r, static_cast<const half2_t*>(z), partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(256), 0, s, nblocks, partial, out6);
+    hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, nblocks, partial, out6);
 }
 
 void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w, const float* chi, double* partial,
@@ -2548,7 +2552,7 @@ void launch_elastic_matrix(hipStream_t s, int n, const float* ke, const float* w
     const unsigned nchunks = full_grid(L);
     const int nb = (int)std::min<unsigned>((unsigned)nblocks, nchunks);
     hipLaunchKernelGGL(k_elastic_matrix, dim3(nb), dim3(kFullThreads), 0, s, L, N, nchunks, ke, w, chi, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(21), dim3(256), 0, s, nb, partial, out21);
+    hipLaunchKernelGGL(k_sum_partials, dim3(21), dim3(kSumThreads), 0, s, nb, partial, out21);
 }
 
 void launch_sensitivity(hipStream_t s, int n, int pitch, const float* ke, const float* chi, const float* wdc,
@@ -2574,7 +2578,7 @@ void launch_oc_trial(hipStream_t s, int n, int pitch, double g, double min_rho, 
                      const float* sens, const float* rho, float* out, double* partial, int nblocks, double* sum) {
     hipLaunchKernelGGL(k_oc_trial, dim3(nblocks), dim3(256), 0, s, n, pitch, g, min_rho, step, damp, sens, rho, out,
                        partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nblocks, partial, sum);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kSumThreads), 0, s, nblocks, partial, sum);
 }
 
 void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* partial, int nblocks) {
@@ -2838,7 +2842,7 @@ void launch_mma_setup(hipStream_t s, int m, long n, int iter, double move, const
                       double* P, double* Q, double* part, int nb, double* out) {
     hipLaunchKernelGGL(k_mma_setup, dim3(nb), dim3(256), 0, s, m, n, iter, move, x, xmin, xmax, xold1, xold2, df0, dfdx,
                        low, upp, alfa, beta, p0, q0, P, Q, part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(m), dim3(256), 0, s, nb, part, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(m), dim3(kSumThreads), 0, s, nb, part, out);
 }
 
 void launch_mma_resid(hipStream_t s, int m, long n, const double* lam, double epsi, const double* x,
@@ -2850,7 +2854,7 @@ void launch_mma_resid(hipStream_t s, int m, long n, const double* lam, double ep
     d.epsi = epsi;
     hipLaunchKernelGGL(k_mma_resid, dim3(nb), dim3(256), 0, s, m, n, d, x, xsi, eta, low, upp, alfa, beta, p0, q0, P,
                        Q, part, partmax);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1 + m), dim3(256), 0, s, nb, part, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1 + m), dim3(kSumThreads), 0, s, nb, part, out);
     hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, s, nb, partmax, out + 1 + m);
 }
 
@@ -2864,7 +2868,7 @@ void launch_mma_newton(hipStream_t s, int m, long n, const double* lam, double e
     const int NS = kMmaMaxM + kMmaMaxM * (kMmaMaxM + 1) / 2;
     hipLaunchKernelGGL(k_mma_newton, dim3(nb), dim3(256), 0, s, m, n, d, x, xsi, eta, low, upp, alfa, beta, p0, q0, P,
                        Q, delx, diagx, part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(NS), dim3(256), 0, s, nb, part, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(NS), dim3(kSumThreads), 0, s, nb, part, out);
 }
 
 void launch_mma_dir(hipStream_t s, int m, long n, double epsi, const double* dlam, const double* x, const double* xsi,

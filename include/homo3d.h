@@ -162,8 +162,10 @@ int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, float* out);
  * kernel: "fine_gs" (full GS colour passes), "fine_gs_zero" (colour passes of sweeps that start from
  * zero), "fine_apply" (the CG operator pass: q = K z + beta q_old with the fused p, x updates and p.q;
  * also homo3d_apply_K), "fine_apply_first" (the first CG pass of an inner solve: q = K z, p = z) and
- * "fine_residual" (f - K u inside the V-cycle). */
+ * "fine_residual" (f - K u inside the V-cycle).  Each event pair can delay the next launch by tens of
+ * microseconds; homo3d_profile_only restricts the pairs to one kernel kind (NULL or "": all kinds). */
 int homo3d_profile_enable(homo3d_handle* h, int enable);
+int homo3d_profile_only(homo3d_handle* h, const char* kernel);
 int homo3d_profile_get(homo3d_handle* h, const char* kernel, long* launches, double* total_ms);
 
 #ifdef __cplusplus
