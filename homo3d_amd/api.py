@@ -58,6 +58,7 @@ EXPORTS = {
     "homo3d_apply_K": (ctypes.c_int, [ctypes.c_void_p]),
     "homo3d_apply_K16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_smooth16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
+    "homo3d_presmooth16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "homo3d_smooth": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "homo3d_restrict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_prolong": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -285,6 +286,12 @@ class Homogenization:
     def smooth16(self, omega=1.0, reverse=False):
         """One finest-level sweep on fp16(u[0]) with FP16 storage; u[0] receives the result."""
         self._chk(lib().homo3d_smooth16(self._h, float(omega), int(reverse)), "smooth16")
+
+    def presmooth16(self, omega=1.0):
+        """The preconditioner's level-0 pre-smoothing: a forward sweep from fp16 u = 0 with f[0]; u[0]
+        receives the result and r[0] the residual f - K u, formed from the sweep's own-colour residuals
+        and the later-colour terms."""
+        self._chk(lib().homo3d_presmooth16(self._h, float(omega)), "presmooth16")
 
     def smooth(self, level=0, omega=1.0, reverse=False):
         self._chk(lib().homo3d_smooth(self._h, level, float(omega), int(reverse)), "smooth")

@@ -342,8 +342,10 @@ void prof_end(homo3d_handle* h, ProfEvent& ev) {
 // dot_out (level 0, reverse sweeps of a packed-FP16 u with fused colour pairs; returns false otherwise):
 // also the per-load-case f . u after the sweep, accumulated by the pair kernels as each vertex gets its
 // final value.
+// rown (level 0, zero-start forward sweeps with fused colour pairs, else ignored): the passes also store
+// their own-colour residuals there, for op_residual_after_zero_sweep.
 bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, int reverse, bool zero_start = false,
-               int u_half = 0, double* dot_out = nullptr) {
+               int u_half = 0, double* dot_out = nullptr, float* rown = nullptr) {
     const Level& L = h->lv[l];
     if (l == 0 && !zero_start) launch_zero_pin(h->stream, L.N, u, u_half);
     const int zs = zero_start && !reverse ? 1 : 0;
@@ -354,7 +356,7 @@ bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
             ProfEvent ev;
             prof_begin(h, zs ? kProfGsZero : kProfGs, ev);
             launch_fine_gs2(h->stream, L.n, reverse ? 7 - cc : cc, omega, reverse, zs, h->d_ke, h->d_w, u, u_half, f,
-                            dot ? h->d_partial + (size_t)(cc / 2) * G : nullptr, 4 * G);
+                            dot ? h->d_partial + (size_t)(cc / 2) * G : nullptr, 4 * G, zs ? rown : nullptr);
             prof_end(h, ev);
         }
         if (dot) launch_sum_partials(h->stream, kRhs, 4 * G, h->d_partial, dot_out);
@@ -372,6 +374,18 @@ bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
         }
     }
     return false;
+}
+
+// Level 0 after a zero-start forward sweep that stored its own-colour residuals in r (op_smooth
+// rown): r = f - K u from the later-colour terms alone.
+bool residual_after_zero_sweep_ok(const homo3d_handle* h) { return fine_gs_pairs_ok(h->lv[0].n) != 0; }
+
+void op_residual_after_zero_sweep(homo3d_handle* h, const void* u, int u_half, float* r) {
+    const Level& L = h->lv[0];
+    ProfEvent ev;
+    prof_begin(h, kProfResidual, ev);
+    launch_fine_residual_later(h->stream, L.n, h->d_ke, h->d_w, u, u_half, r);
+    prof_end(h, ev);
 }
 
 void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* r, int u_half = 0) {
@@ -408,9 +422,15 @@ void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* 
         const float* f = l == 0 ? f0 : L.f;
         const int uh = u_half_of(l);
         const int sweeps = l > 0 && L.n <= kSmallLevelN ? small_sweeps : 1;
-        op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero, uh);
+        // a zero-start level-0 sweep leaves its own-colour residuals behind; the residual then needs
+        // only the later-colour terms
+        const bool rown = l == 0 && u0_zero && residual_after_zero_sweep_ok(h);
+        op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero, uh, nullptr, rown ? L.r : nullptr);
         for (int k = 1; k < sweeps; k++) op_smooth(h, l, u, f, omega, 0, false, uh);
-        op_residual(h, l, u, f, L.r, uh);
+        if (rown)
+            op_residual_after_zero_sweep(h, u, uh, L.r);
+        else
+            op_residual(h, l, u, f, L.r, uh);
         if (l == 0 && rn2) launch_dot6(h->stream, 3 * L.N, L.r, L.r, h->d_partial, kDotBlocks, rn2);
         launch_restrict(h->stream, L.n, h->lv[l + 1].lex, L.r, h->lv[l + 1].f);
     }
@@ -1069,6 +1089,19 @@ int homo3d_smooth16(homo3d_handle* h, double omega, int reverse) {
     const Level& L = h->lv[0];
     launch_pack_half(h->stream, L.N, h->X, h->Z);
     op_smooth(h, 0, h->Z, h->F, (float)omega, reverse, false, 1);
+    launch_unpack_half(h->stream, L.N, h->Z, h->X);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_presmooth16(homo3d_handle* h, double omega) {
+    int rc = fp16_ready(h);
+    if (rc) return rc;
+    if (!residual_after_zero_sweep_ok(h)) return set_err(h, HOMO3D_NO_SUPPORT, "needs fused colour pairs at level 0");
+    const Level& L = h->lv[0];
+    op_smooth(h, 0, h->Z, h->F, (float)omega, 0, true, 1, nullptr, L.r);
+    op_residual_after_zero_sweep(h, h->Z, 1, L.r);
     launch_unpack_half(h->stream, L.N, h->Z, h->X);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);

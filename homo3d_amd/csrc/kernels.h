@@ -40,9 +40,16 @@ void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, 
 // a packed-FP16 u only), each of the (n/2)^3 / 256 workgroups also writes its FP64 sums of f . u_new per
 // load case to partial[r * pstride + workgroup].
 int fine_gs_pairs_ok(int n);
+// rown (zero-start forward passes; ignored otherwise): both colours also store, per vertex, f minus the
+// K u terms of the colours swept so far (the vertex itself included); after the sweep,
+// launch_fine_residual_later turns these into the residual f - K u.
 void launch_fine_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse, int zero_start, const float* ke,
                      const float* w, void* u, int u_half, const float* f, double* partial = nullptr,
-                     int pstride = 0);
+                     int pstride = 0, float* rown = nullptr);
+// r -= the K u terms of later-colour neighbours, colours 0..6 (one launch): with the own-colour
+// residuals of a complete zero-start forward sweep (launch_fine_gs2 rown) in r, leaves r = f - K u.
+void launch_fine_residual_later(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
+                                float* r);
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f);
 void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double* kchi, const float* xc, double emin,
                             const double* u, float* r32, double* partial, double* out6, int do_k);

@@ -214,6 +214,16 @@ constexpr int next_nb(uint32_t mask, int d) {
 // zero-start colour of a GS pass (-1: ordinary pass) -> gathered neighbours
 constexpr uint32_t gs_mask(int zc) { return zc < 0 ? kRowOffDiag : zero_start_mask(zc); }
 
+// neighbours of a colour-c vertex whose colour comes after c in a forward sweep
+__host__ __device__ constexpr uint32_t later_mask(int c) {
+    uint32_t m = 0;
+    for (int d = 0; d < 27; d++) {
+        const int f = (d % 3 != 1 ? 1 : 0) | (d / 3 % 3 != 1 ? 2 : 0) | (d / 9 != 1 ? 4 : 0);
+        if ((c ^ f) > c) m |= 1u << d;
+    }
+    return m;
+}
+
 // the stored words of one neighbour: W = R / (load cases per word) words per component; r0 counts
 // load cases (a multiple of the cases per word)
 template <int W, typename A>
@@ -735,13 +745,16 @@ __global__ __launch_bounds__(kEfThreads, 2) void k_fine_apply_ef(Lay L, size_t N
 // starts from u = 0 (colours above ZC are never read, so u needs no clearing beforehand; the old value
 // of the vertex itself is zero).  XL: the x-neighbours come from LDS; TO_LDS: the new words are also
 // stored to slot threadIdx.x of out_lds.  DOT: dot[r] += f . u_new over the vertex (stored values;
-// FP32 over the vertex's 3 components, FP64 across vertices).
-// The pinned origin keeps u = 0.
-template <int R, bool REVERSE, int ZC, typename A, bool XL, bool TO_LDS, bool DOT = false>
+// FP32 over the vertex's 3 components, FP64 across vertices).  ROWN (ZC >= 0): also
+// rown = f - sum over the neighbours of colours up to ZC and the vertex itself of K_vj u_j, with the
+// stored values; once the sweep is complete, subtracting the later colours' terms gives f - K u.
+// The pinned origin keeps u = 0 (and rown = 0).
+template <int R, bool REVERSE, int ZC, typename A, bool XL, bool TO_LDS, bool DOT = false, bool ROWN = false>
 __device__ __forceinline__ void gs_vertex(const Lay& L, size_t N, uint32_t v, float omega, const float* ke_s,
                                           const float* __restrict__ w, A* __restrict__ u,
                                           const float* __restrict__ f, int r0, const XNbLds<A>& xl, A* out_lds,
-                                          float* dot = nullptr) {
+                                          float* dot = nullptr, float* __restrict__ rown = nullptr) {
+    static_assert(!ROWN || ZC >= 0, "the own-colour residual needs a zero-start forward pass");
     constexpr int cpw = CasesPerWord<A>::value;
     int x, y, z;
     L.pos(v, x, y, z);
@@ -771,6 +784,7 @@ __device__ __forceinline__ void gs_vertex(const Lay& L, size_t N, uint32_t v, fl
         }
 #pragma unroll
         for (int h = 0; h < cpw; h++) block_update<REVERSE>(uu[h], ff[h], acc[p * cpw + h], Ks, inv, omega);
+        float us[cpw][3];  // the stored values
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const size_t o = (size_t)((r0 / cpw + p) * 3 + i) * N + v;
@@ -780,6 +794,12 @@ __device__ __forceinline__ void gs_vertex(const Lay& L, size_t N, uint32_t v, fl
             else
                 word = uu[0][i];
             if (pin) word = A(0);
+            if constexpr (cpw == 2) {
+                us[0][i] = (float)word.x;
+                us[1][i] = (float)word.y;
+            } else {
+                us[0][i] = word;
+            }
             u[o] = word;
             if constexpr (TO_LDS) out_lds[(p * 3 + i) * 256 + threadIdx.x] = word;
             if constexpr (DOT) {
@@ -790,6 +810,16 @@ __device__ __forceinline__ void gs_vertex(const Lay& L, size_t N, uint32_t v, fl
                     dot[p] = fmaf(ff[0][i], word, dot[p]);
                 }
             }
+        }
+        if constexpr (ROWN) {
+#pragma unroll
+            for (int hh = 0; hh < cpw; hh++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const float kv = fmaf(Ks[3 * i], us[hh][0], fmaf(Ks[3 * i + 1], us[hh][1], Ks[3 * i + 2] * us[hh][2]));
+                    const float ro = (ff[hh][i] - acc[p * cpw + hh][i]) - kv;
+                    rown[(size_t)((r0 + p * cpw + hh) * 3 + i) * N + v] = pin ? 0.f : ro;
+                }
         }
     }
 }
@@ -816,11 +846,13 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
 // updates the first colour of its rows, keeps the new values in LDS, and updates the second colour
 // reading those x-neighbours from LDS.  Other workgroups never read the rows' first-colour values
 // before they are final, so the result equals the two colour passes in sequence.
-template <int R, bool REVERSE, int ZC, typename A, bool DOT = false>
+// ROWN (zero-start passes): both colours also store their own-colour residuals to rown (gs_vertex).
+template <int R, bool REVERSE, int ZC, typename A, bool DOT = false, bool ROWN = false>
 __global__ __launch_bounds__(256, 4) void k_fine_gs2(Lay L, size_t N, int cfirst, float omega,
                                                      const float* __restrict__ ke, const float* __restrict__ w,
                                                      A* __restrict__ u, const float* __restrict__ f,
-                                                     double* __restrict__ partial, int pstride) {
+                                                     double* __restrict__ partial, int pstride,
+                                                     float* __restrict__ rown) {
     constexpr int W = R / CasesPerWord<A>::value;
     __shared__ float ke_s[576];
     __shared__ A xs[3 * W * 256];
@@ -830,8 +862,8 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs2(Lay L, size_t N, int cfirst
     float dot[R];
 #pragma unroll
     for (int r = 0; r < R; r++) dot[r] = 0.f;
-    gs_vertex<R, REVERSE, ZC, A, false, true, DOT>(L, N, colour_block(cfirst) * L.h3 + t, omega, ke_s, w, u, f, 0,
-                                                  XNbLds<A>(), xs, dot);
+    gs_vertex<R, REVERSE, ZC, A, false, true, DOT, ROWN>(L, N, colour_block(cfirst) * L.h3 + t, omega, ke_s, w, u,
+                                                        f, 0, XNbLds<A>(), xs, dot, rown);
     __syncthreads();
     XNbLds<A> xl;
     xl.base = xs;
@@ -839,8 +871,8 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs2(Lay L, size_t N, int cfirst
     const bool odd = (colour_block(c2) & 1u) != 0;  // x-parity of the second colour
     xl.slot[0] = row0 + (odd ? lx : (lx == 0 ? h - 1 : lx - 1));
     xl.slot[1] = row0 + (odd ? (lx + 1 == h ? 0 : lx + 1) : lx);
-    gs_vertex<R, REVERSE, (ZC < 0 ? -1 : ZC + 1), A, true, false, DOT>(L, N, colour_block(c2) * L.h3 + t, omega, ke_s,
-                                                                       w, u, f, 0, xl, nullptr, dot);
+    gs_vertex<R, REVERSE, (ZC < 0 ? -1 : ZC + 1), A, true, false, DOT, ROWN>(
+        L, N, colour_block(c2) * L.h3 + t, omega, ke_s, w, u, f, 0, xl, nullptr, dot, rown);
     if constexpr (DOT) {
         __shared__ double sh[4];
 #pragma unroll
@@ -848,6 +880,61 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs2(Lay L, size_t N, int cfirst
             const double sdot = block_sum<256>((double)dot[r], sh);
             if (threadIdx.x == 0) partial[(size_t)r * pstride + blockIdx.x] = sdot;
         }
+    }
+}
+
+// The residual f - K u after a zero-start forward sweep whose colour passes stored their own-colour
+// residuals in r (gs_vertex ROWN): r -= sum over the neighbours of colours after C of K_vj u_j, for
+// a vertex v of colour C (C < 7; colour 7 has no later neighbours, so its r is final).  About half
+// the neighbour terms of a full residual pass, averaged over the colours.
+// zero is 0 at run time: chaining the three components through it keeps the compiler converting the
+// gathered FP16 words inside the FMAs (v_fma_mix); with independent components it converts every word
+// up front and spills.
+template <int R, int C, typename A>
+__device__ __forceinline__ void later_vertex(const Lay& L, size_t N, uint32_t v, const float* ke_s,
+                                             const float* __restrict__ w, const A* __restrict__ u,
+                                             float* __restrict__ r, float zero) {
+    int x, y, z;
+    L.pos(v, x, y, z);
+    FineCtx c;
+    fine_ctx(c, x, y, z, L, w);
+    float acc[R][3];
+    fine_gather<R, later_mask(C), float, A>(c, L.n, N, ke_s, u, acc);
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        float ro[3], rn[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) ro[i] = r[(size_t)(q * 3 + i) * N + v];
+        rn[0] = ro[0] - acc[q][0];
+        rn[1] = fmaf(-zero, rn[0], ro[1] - acc[q][1]);
+        rn[2] = fmaf(-zero, rn[1], ro[2] - acc[q][2]);
+#pragma unroll
+        for (int i = 0; i < 3; i++) r[(size_t)(q * 3 + i) * N + v] = (v == 0) ? 0.f : rn[i];
+    }
+}
+
+// Colours 0..6 in one launch: workgroups [C P, (C + 1) P) take colour C (P = workgroups per colour, a
+// multiple of 8), each colour's chunks dealt to the XCDs as in xcd_chunk.  The branch is uniform per
+// workgroup, and the dispatch order keeps the CUs on one or two colours at a time.
+template <int R, typename A>
+__global__ __launch_bounds__(256, 4) void k_fine_resid_later(Lay L, size_t N, const float* __restrict__ ke,
+                                                             const float* __restrict__ w, const A* __restrict__ u,
+                                                             float* __restrict__ r, uint32_t P, float zero) {
+    __shared__ float ke_s[576];
+    stage_ke(ke_s, ke);
+    const uint32_t C = blockIdx.x / P;
+    const uint32_t t = xcd_chunk(blockIdx.x - C * P, P) * 256u + threadIdx.x;
+    if (t >= L.h3) return;
+    const uint32_t v = colour_block((int)C) * L.h3 + t;
+    switch (C) {
+        case 0: later_vertex<R, 0, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 1: later_vertex<R, 1, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 2: later_vertex<R, 2, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 3: later_vertex<R, 3, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 4: later_vertex<R, 4, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 5: later_vertex<R, 5, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 6: later_vertex<R, 6, A>(L, N, v, ke_s, w, u, r, zero); break;
+        default: break;
     }
 }
 
@@ -2242,42 +2329,67 @@ int fine_gs_pairs_ok(int n) {
     return n % 2 == 0 && h <= 256 && 256 % h == 0 && h3 % 256 == 0;
 }
 
-template <bool REV, int ZC, typename A, bool DOT = false>
+template <bool REV, int ZC, typename A, bool DOT = false, bool ROWN = false>
 static void fine_gs2_pair(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, const float* ke,
-                          const float* w, A* u, const float* f, double* partial = nullptr, int pstride = 0) {
-    hipLaunchKernelGGL((k_fine_gs2<6, REV, ZC, A, DOT>), dim3(L.h3 / 256), dim3(256), 0, s, L, N, cfirst, omega, ke, w,
-                       u, f, partial, pstride);
+                          const float* w, A* u, const float* f, double* partial = nullptr, int pstride = 0,
+                          float* rown = nullptr) {
+    hipLaunchKernelGGL((k_fine_gs2<6, REV, ZC, A, DOT, ROWN>), dim3(L.h3 / 256), dim3(256), 0, s, L, N, cfirst, omega,
+                       ke, w, u, f, partial, pstride, rown);
+}
+
+template <int ZC, typename A>
+static void fine_gs2_zero(hipStream_t s, const Lay& L, size_t N, float omega, const float* ke, const float* w, A* u,
+                          const float* f, float* rown) {
+    if (rown)
+        fine_gs2_pair<false, ZC, A, false, true>(s, L, N, ZC, omega, ke, w, u, f, nullptr, 0, rown);
+    else
+        fine_gs2_pair<false, ZC>(s, L, N, ZC, omega, ke, w, u, f);
 }
 
 template <typename A>
 static void fine_gs2_pick(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, int reverse,
-                          int zero_start, const float* ke, const float* w, A* u, const float* f) {
+                          int zero_start, const float* ke, const float* w, A* u, const float* f, float* rown) {
     if (reverse) {
         fine_gs2_pair<true, -1>(s, L, N, cfirst, omega, ke, w, u, f);
         return;
     }
     switch (zero_start ? cfirst : -1) {
-        case 0: fine_gs2_pair<false, 0>(s, L, N, cfirst, omega, ke, w, u, f); break;
-        case 2: fine_gs2_pair<false, 2>(s, L, N, cfirst, omega, ke, w, u, f); break;
-        case 4: fine_gs2_pair<false, 4>(s, L, N, cfirst, omega, ke, w, u, f); break;
-        case 6: fine_gs2_pair<false, 6>(s, L, N, cfirst, omega, ke, w, u, f); break;
+        case 0: fine_gs2_zero<0>(s, L, N, omega, ke, w, u, f, rown); break;
+        case 2: fine_gs2_zero<2>(s, L, N, omega, ke, w, u, f, rown); break;
+        case 4: fine_gs2_zero<4>(s, L, N, omega, ke, w, u, f, rown); break;
+        case 6: fine_gs2_zero<6>(s, L, N, omega, ke, w, u, f, rown); break;
         default: fine_gs2_pair<false, -1>(s, L, N, cfirst, omega, ke, w, u, f); break;
     }
 }
 
 void launch_fine_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse, int zero_start, const float* ke,
-                     const float* w, void* u, int u_half, const float* f, double* partial, int pstride) {
+                     const float* w, void* u, int u_half, const float* f, double* partial, int pstride, float* rown) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
+    if (reverse || !zero_start) rown = nullptr;  // own-colour residuals only for zero-start passes
     if (partial && reverse && u_half) {  // f . u_new per workgroup (the r . z of a CG step)
         fine_gs2_pair<true, -1, half2_t, true>(s, L, N, cfirst, omega, ke, w, static_cast<half2_t*>(u), f, partial,
                                                pstride);
         return;
     }
     if (u_half)
-        fine_gs2_pick(s, L, N, cfirst, omega, reverse, zero_start, ke, w, static_cast<half2_t*>(u), f);
+        fine_gs2_pick(s, L, N, cfirst, omega, reverse, zero_start, ke, w, static_cast<half2_t*>(u), f, rown);
     else
-        fine_gs2_pick(s, L, N, cfirst, omega, reverse, zero_start, ke, w, static_cast<float*>(u), f);
+        fine_gs2_pick(s, L, N, cfirst, omega, reverse, zero_start, ke, w, static_cast<float*>(u), f, rown);
+}
+
+void launch_fine_residual_later(hipStream_t s, int n, const float* ke, const float* w, const void* u, int u_half,
+                                float* r) {
+    const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
+    const uint32_t P = (nblk(L.h3) + 7u) & ~7u;  // workgroups per colour, a multiple of 8
+    const dim3 g(7 * P), b(256);
+    if (u_half)
+        hipLaunchKernelGGL((k_fine_resid_later<6, half2_t>), g, b, 0, s, L, N, ke, w, static_cast<const half2_t*>(u),
+                           r, P, 0.f);
+    else
+        hipLaunchKernelGGL((k_fine_resid_later<6, float>), g, b, 0, s, L, N, ke, w, static_cast<const float*>(u), r,
+                           P, 0.f);
 }
 
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {

@@ -128,6 +128,11 @@ int homo3d_smooth(homo3d_handle* h, int level, double omega, int reverse);
    FP16, then u[0] = u16.  Need n % 8 == 0 (HOMO3D_NO_SUPPORT otherwise). */
 int homo3d_apply_K16(homo3d_handle* h, int residual);
 int homo3d_smooth16(homo3d_handle* h, double omega, int reverse);
+/* The preconditioner's level-0 pre-smoothing: one forward sweep from u16 = 0 (whatever u[0] holds) with
+ * f[0], then u[0] = u16 and r[0] = f - K u16, the residual formed from the own-colour residuals the
+ * sweep leaves behind plus the later-colour terms.  Needs n % 8 == 0 and fused colour pairs
+ * ((n/2) | 256, 256 | (n/2)^3); HOMO3D_NO_SUPPORT otherwise. */
+int homo3d_presmooth16(homo3d_handle* h, double omega);
 int homo3d_restrict(homo3d_handle* h, int level);  /* f[level] = P^T r[level-1] */
 int homo3d_prolong(homo3d_handle* h, int level);   /* u[level] += P u[level+1] */
 int homo3d_coarse_solve(homo3d_handle* h);

@@ -152,6 +152,35 @@ def test_fine_gauss_seidel_fp16(h3d, orc, n, reverse):
         assert rel(got[s], ref) < 3e-3
 
 
+@pytest.mark.parametrize("n,omega", [(32, 1.0), (64, 1.0), (32, 0.8)])
+def test_presmooth_residual_fp16(h3d, orc, n, omega):
+    """The preconditioner's level-0 pre-smoothing: a forward sweep from u = 0 whose colour passes leave
+    their own-colour residuals behind, completed by the later-colour terms.  u against the FP64 sweep
+    from zero (FP16 storage tolerance, as for the sweep test; whatever u held before is ignored), and r
+    against the FP64 residual f - K u of the returned FP16 u, to the tolerance of the full residual pass
+    on the same u (the same terms, summed in another order)."""
+    say("presmooth residual: n=%d omega=%g" % (n, omega))
+    hom, mg = fp16_case(h3d, orc, n, 11, mfma=False)
+    rng = np.random.default_rng(n + 5)
+    # loads small enough that the sweep's u (large in the near-void cells) stays within FP16 range
+    f = (rng.standard_normal(hom.level_shape(0)) * 1e-3).astype(np.float32)
+    hom.set_field(0, 1, f)
+    hom.set_field(0, 0, rng.standard_normal(hom.level_shape(0)).astype(np.float32))  # ignored
+    hom.presmooth16(omega)
+    u = hom.get_field(0, 0)
+    r = hom.get_field(0, 2)
+    hom.apply_K16(residual=True)  # full residual pass on the same u
+    r_full = hom.get_field(0, 2)
+    zero = np.zeros(u.shape[1:], np.float64)
+    for s in range(6):
+        ref = orc.gs_fine(n, mg.KE, mg.w, zero, f[s].astype(np.float64), omega, False)
+        rr = orc.residual(n, mg.KE, mg.w, u[s].astype(np.float64), f[s].astype(np.float64))
+        say("  case %d: u %.2e, r %.2e, full pass %.2e" % (s, rel(u[s], ref), rel(r[s], rr), rel(r_full[s], rr)))
+        assert rel(u[s], ref) < 3e-3
+        assert rel(r_full[s], rr) < 2e-6
+        assert rel(r[s], rr) < 2e-6
+
+
 def test_galerkin_stencils(h3d, orc):
     say("galerkin: n=32")
     hom, x = make(h3d, 32, seed=4)
