@@ -72,6 +72,11 @@ struct homo3d_handle {
     size_t stage_elems = 0;
     double* d_Ainv = nullptr;
     int mc = 0;
+    // coarsest factorisation, deferred from the density update to the first coarse solve after it
+    float* h_cst = nullptr;          // pinned copy of the coarsest stencil
+    double* h_ainv = nullptr;        // pinned staging of the inverse
+    hipEvent_t ev_coarse = nullptr;  // the stencil copy has landed
+    bool coarse_pending = false;
     // solution fields (warm start) and PCG work vectors, level-0 sized, 6 load cases
     float* X = nullptr;
     float* F = nullptr;
@@ -400,6 +405,40 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
     }
 }
 
+// FP64 Cholesky inverse of the coarsest Galerkin operator (assembled densely from its stencil), once
+// per density update, uploaded ahead of the coarse solve that needs it.  A system that is not positive
+// definite leaves the handle without a density.
+int coarse_factor(homo3d_handle* h) {
+    if (!h->coarse_pending) return HOMO3D_OK;
+    h->coarse_pending = false;
+    HIPCHK(h, hipEventSynchronize(h->ev_coarse));
+    const Level& Lc = h->lv[h->nl - 1];
+    const float* st = h->h_cst;
+    const int m = h->mc, nc = Lc.n;
+    std::vector<double> A((size_t)m * m, 0.0);
+    for (size_t v = 0; v < Lc.N; v++) {
+        const int x0 = (int)(v % nc), y0 = (int)((v / nc) % nc), z0 = (int)(v / ((size_t)nc * nc));
+        for (int d = 0; d < 27; d++) {
+            const int xq = (x0 + d % 3 - 1 + nc) % nc, yq = (y0 + d / 3 % 3 - 1 + nc) % nc, zq = (z0 + d / 9 - 1 + nc) % nc;
+            const size_t q = xq + (size_t)nc * (yq + (size_t)nc * zq);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) A[(3 * v + i) * m + 3 * q + j] += st[(size_t)(d * 9 + i * 3 + j) * Lc.N + v];
+        }
+    }
+    for (int i = 0; i < m; i++)
+        for (int j = i + 1; j < m; j++) {
+            const double s = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
+            A[(size_t)i * m + j] = A[(size_t)j * m + i] = s;
+        }
+    if (!cholesky_inverse(m, A)) {
+        h->density_set = false;
+        return set_err(h, HOMO3D_DIVERGED, "coarsest system is not positive definite");
+    }
+    std::copy(A.begin(), A.end(), h->h_ainv);
+    HIPCHK(h, hipMemcpyAsync(h->d_Ainv, h->h_ainv, sizeof(double) * A.size(), hipMemcpyHostToDevice, h->stream));
+    return HOMO3D_OK;
+}
+
 // One V(1,1) cycle on the level-0 pair (u0, f0) (MG::v_cycle, MG.cpp:46-70).  When rn2 is given it
 // receives, per load case, the squared norm of the level-0 residual after pre-smoothing.  u0_zero:
 // the cycle starts from u0 = 0 (u0's contents are ignored), as the coarse levels always do.
@@ -407,8 +446,8 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
 // f0 . u0 of the result (the r . z of a CG step).
 // small_sweeps: smoothing sweeps per side on the levels that run one load case per launch (n <= 32);
 // every other level takes one.
-void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
-            int u0_half = 0, double* dot_out = nullptr, int small_sweeps = 1) {
+int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
+           int u0_half = 0, double* dot_out = nullptr, int small_sweeps = 1) {
     const float omega = (float)h->cfg.sor_omega;
     // packed FP16 u: level 0 when the caller's iterate is, and then every coarser level (but the
     // coarsest) whose smoother runs six load cases per launch; the small levels stay FP32
@@ -435,6 +474,8 @@ void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* 
         launch_restrict(h->stream, L.n, h->lv[l + 1].lex, L.r, h->lv[l + 1].f);
     }
     const Level& Lc = h->lv[h->nl - 1];
+    int rc = coarse_factor(h);
+    if (rc) return rc;
     launch_coarse_solve(h->stream, Lc.n, h->d_Ainv, Lc.f, Lc.u);
     for (int l = h->nl - 2; l >= 0; l--) {
         Level& L = h->lv[l];
@@ -447,6 +488,7 @@ void vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* 
         const bool fused = op_smooth(h, l, u, f, omega, symmetric ? 1 : 0, false, uh, l == 0 ? dot_out : nullptr);
         if (l == 0 && dot_out && !fused) launch_dot_rz(h->stream, 3 * L.N, f0, u0, uh, h->d_partial, kDotBlocks, dot_out);
     }
+    return HOMO3D_OK;
 }
 
 int ensure_stage(homo3d_handle* h, size_t elems) {
@@ -502,7 +544,8 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     // the preconditioner's level-0 iterate z in packed FP16 when enabled and the layout allows it (p, q, r
     // and x stay FP32: an FP16 search direction costs more CG steps than it saves)
     const int zh = (h->cfg.precond_half && L0.n % 8 == 0) ? 1 : 0;
-    vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc, kPrecondSmallSweeps);  // with rz
+    int rc = vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc, kPrecondSmallSweeps);  // with rz
+    if (rc) return rc;
     // One CG step: q = K p with p.q, formed as K z + beta q_old since p = z + beta p_old (the operator
     // gathers the packed z instead of the FP32 p).  The same pass advances x by the previous step's
     // alpha p_old and forms the new p, so p, x, r and q are each streamed once per step; a last x-only
@@ -538,7 +581,8 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         if (changed)
             HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                                      h->stream));
-        vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc + 3 * kRhs, kPrecondSmallSweeps);  // with rz_new
+        rc = vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc + 3 * kRhs, kPrecondSmallSweeps);  // with rz_new
+        if (rc) return rc;
         launch_pcg_beta(h->stream, h->d_sc);
     }
     if (x_pending) launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, zh, h->P, h->X, 0, kDotBlocks);
@@ -735,6 +779,9 @@ int homo3d_destroy(homo3d_handle* h) {
                     h->Rr, h->d_partial, h->d_sc, h->d_tmp, h->d_ftmp, h->d_gperm, h->d_gflip};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (h->h_cst) (void)hipHostFree(h->h_cst);
+    if (h->h_ainv) (void)hipHostFree(h->h_ainv);
+    if (h->ev_coarse) (void)hipEventDestroy(h->ev_coarse);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HOMO3D_OK;
@@ -767,30 +814,17 @@ int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_de
     for (int l = 2; l < h->nl; l++)
         launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l].lex, h->lv[l - 1].st, h->lv[l - 1].st_half,
                                h->lv[l].st, h->lv[l].st_half, clamp);
-    // coarsest dense system
+    // coarsest dense system: its stencil goes to pinned host memory now; the factorisation runs at the
+    // first coarse solve (coarse_factor), when the host has already queued the finest-level work
     const Level& Lc = h->lv[h->nl - 1];
-    std::vector<float> st(243 * Lc.N);
-    HIPCHK(h, hipMemcpyAsync(st.data(), Lc.st, sizeof(float) * st.size(), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    const int m = h->mc, nc = Lc.n;
-    std::vector<double> A((size_t)m * m, 0.0);
-    for (size_t v = 0; v < Lc.N; v++) {
-        const int x0 = (int)(v % nc), y0 = (int)((v / nc) % nc), z0 = (int)(v / ((size_t)nc * nc));
-        for (int d = 0; d < 27; d++) {
-            const int xq = (x0 + d % 3 - 1 + nc) % nc, yq = (y0 + d / 3 % 3 - 1 + nc) % nc, zq = (z0 + d / 9 - 1 + nc) % nc;
-            const size_t q = xq + (size_t)nc * (yq + (size_t)nc * zq);
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) A[(3 * v + i) * m + 3 * q + j] += st[(size_t)(d * 9 + i * 3 + j) * Lc.N + v];
-        }
+    if (!h->h_cst) {
+        HIPCHK(h, hipHostMalloc((void**)&h->h_cst, sizeof(float) * 243 * Lc.N, hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void**)&h->h_ainv, sizeof(double) * h->mc * h->mc, hipHostMallocDefault));
+        HIPCHK(h, hipEventCreateWithFlags(&h->ev_coarse, hipEventDisableTiming));
     }
-    for (int i = 0; i < m; i++)
-        for (int j = i + 1; j < m; j++) {
-            const double s = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
-            A[(size_t)i * m + j] = A[(size_t)j * m + i] = s;
-        }
-    if (!cholesky_inverse(m, A)) return set_err(h, HOMO3D_DIVERGED, "coarsest system is not positive definite");
-    HIPCHK(h, hipMemcpyAsync(h->d_Ainv, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_cst, Lc.st, sizeof(float) * 243 * Lc.N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipEventRecord(h->ev_coarse, h->stream));
+    h->coarse_pending = true;
     h->density_set = true;
     return HOMO3D_OK;
 }
@@ -811,20 +845,21 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
     if (tol <= 0) tol = h->cfg.fem_rel_thres;
     const int maxit = h->cfg.max_iter;
     launch_macro_rhs(h->stream, L0.n, h->d_kchi, h->d_w, h->F);
-    if (!warm) HIPCHK(h, hipMemsetAsync(h->X, 0, vb, h->stream));
     double fn2[kRhs];
-    launch_dot6(h->stream, M, h->F, h->F, h->d_partial, kDotBlocks, h->d_tmp);
-    HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
     int iters[kRhs] = {0, 0, 0, 0, 0, 0};
     bool done[kRhs] = {false, false, false, false, false, false};
     int status = HOMO3D_OK;
     if (h->cfg.solver == HOMO3D_SOLVER_MG) {
+        if (!warm) HIPCHK(h, hipMemsetAsync(h->X, 0, vb, h->stream));
+        launch_dot6(h->stream, M, h->F, h->F, h->d_partial, kDotBlocks, h->d_tmp);
+        HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
         int overflow[kRhs] = {2, 2, 2, 2, 2, 2};
         int it = 0;
         bool all = false;
         while (!all && it++ < maxit) {
-            vcycle(h, h->X, h->F, 0, h->d_sc);
+            rc = vcycle(h, h->X, h->F, 0, h->d_sc);
+            if (rc) return rc;
             double rn2[kRhs];
             HIPCHK(h, hipMemcpyAsync(rn2, h->d_sc, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1140,6 +1175,8 @@ int homo3d_coarse_solve(homo3d_handle* h) {
     int rc = require_density(h);
     if (rc) return rc;
     const Level& Lc = h->lv[h->nl - 1];
+    rc = coarse_factor(h);
+    if (rc) return rc;
     launch_coarse_solve(h->stream, Lc.n, h->d_Ainv, Lc.f, Lc.u);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
@@ -1149,7 +1186,8 @@ int homo3d_vcycle(homo3d_handle* h, int symmetric) {
     if (!h) return HOMO3D_BAD_ARG;
     int rc = require_density(h);
     if (rc) return rc;
-    vcycle(h, h->X, h->F, symmetric, nullptr);
+    rc = vcycle(h, h->X, h->F, symmetric, nullptr);
+    if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
     return HOMO3D_OK;
