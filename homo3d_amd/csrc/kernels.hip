@@ -913,27 +913,27 @@ __device__ __forceinline__ void later_vertex(const Lay& L, size_t N, uint32_t v,
     }
 }
 
-// Colours 0..6 in one launch: workgroups [C P, (C + 1) P) take colour C (P = workgroups per colour, a
-// multiple of 8), each colour's chunks dealt to the XCDs as in xcd_chunk.  The branch is uniform per
-// workgroup, and the dispatch order keeps the CUs on one or two colours at a time.
+// Colours 0..6 in one launch, on the 512-thread tiles of full_vertex: wave b of a workgroup holds
+// parity block b, i.e. colour 7 - b, and takes that colour's later neighbours (a wave-uniform branch;
+// the colour-7 wave has none).  The tile's eight blocks share their neighbour rows in L1/L2, so the
+// gathered u is fetched from HBM about once.
 template <int R, typename A>
-__global__ __launch_bounds__(256, 4) void k_fine_resid_later(Lay L, size_t N, const float* __restrict__ ke,
-                                                             const float* __restrict__ w, const A* __restrict__ u,
-                                                             float* __restrict__ r, uint32_t P, float zero) {
+__global__ __launch_bounds__(kFullThreads) void k_fine_resid_later(Lay L, size_t N, const float* __restrict__ ke,
+                                                                   const float* __restrict__ w,
+                                                                   const A* __restrict__ u, float* __restrict__ r,
+                                                                   float zero) {
     __shared__ float ke_s[576];
     stage_ke(ke_s, ke);
-    const uint32_t C = blockIdx.x / P;
-    const uint32_t t = xcd_chunk(blockIdx.x - C * P, P) * 256u + threadIdx.x;
-    if (t >= L.h3) return;
-    const uint32_t v = colour_block((int)C) * L.h3 + t;
-    switch (C) {
-        case 0: later_vertex<R, 0, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 1: later_vertex<R, 1, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 2: later_vertex<R, 2, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 3: later_vertex<R, 3, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 4: later_vertex<R, 4, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 5: later_vertex<R, 5, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 6: later_vertex<R, 6, A>(L, N, v, ke_s, w, u, r, zero); break;
+    uint32_t v;
+    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
+    switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
+        case 7: later_vertex<R, 0, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 6: later_vertex<R, 1, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 5: later_vertex<R, 2, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 4: later_vertex<R, 3, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 3: later_vertex<R, 4, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 2: later_vertex<R, 5, A>(L, N, v, ke_s, w, u, r, zero); break;
+        case 1: later_vertex<R, 6, A>(L, N, v, ke_s, w, u, r, zero); break;
         default: break;
     }
 }
@@ -2382,14 +2382,13 @@ void launch_fine_residual_later(hipStream_t s, int n, const float* ke, const flo
                                 float* r) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
-    const uint32_t P = (nblk(L.h3) + 7u) & ~7u;  // workgroups per colour, a multiple of 8
-    const dim3 g(7 * P), b(256);
+    const dim3 g(full_grid(L)), b(kFullThreads);
     if (u_half)
         hipLaunchKernelGGL((k_fine_resid_later<6, half2_t>), g, b, 0, s, L, N, ke, w, static_cast<const half2_t*>(u),
-                           r, P, 0.f);
+                           r, 0.f);
     else
         hipLaunchKernelGGL((k_fine_resid_later<6, float>), g, b, 0, s, L, N, ke, w, static_cast<const float*>(u), r,
-                           P, 0.f);
+                           0.f);
 }
 
 void launch_macro_rhs(hipStream_t s, int n, const float* kchi, const float* w, float* f) {
