@@ -77,6 +77,8 @@ struct homo3d_handle {
     double* h_ainv = nullptr;        // pinned staging of the inverse
     hipEvent_t ev_coarse = nullptr;  // the stencil copy has landed
     bool coarse_pending = false;
+    double* h_rr = nullptr;          // pinned landing place of the CG residual norms
+    hipEvent_t ev_rr = nullptr;      // ... and the event that says they have landed
     // solution fields (warm start) and PCG work vectors, level-0 sized, 6 load cases
     float* X = nullptr;
     float* F = nullptr;
@@ -348,16 +350,19 @@ void prof_end(homo3d_handle* h, ProfEvent& ev) {
 // also the per-load-case f . u after the sweep, accumulated by the pair kernels as each vertex gets its
 // final value.
 // rown (level 0, zero-start forward sweeps with fused colour pairs, else ignored): the passes also store
-// their own-colour residuals there, for op_residual_after_zero_sweep.
+// their own-colour residuals there, for op_residual_after_zero_sweep.  [pair_begin, pair_end): the
+// colour pairs to launch (level 0 with fused pairs, forward sweeps; a sweep split in two launches the
+// same kernels as a whole one).
 bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, int reverse, bool zero_start = false,
-               int u_half = 0, double* dot_out = nullptr, float* rown = nullptr) {
+               int u_half = 0, double* dot_out = nullptr, float* rown = nullptr, int pair_begin = 0,
+               int pair_end = 4) {
     const Level& L = h->lv[l];
     if (l == 0 && !zero_start) launch_zero_pin(h->stream, L.N, u, u_half);
     const int zs = zero_start && !reverse ? 1 : 0;
     if (l == 0 && fine_gs_pairs_ok(L.n)) {  // colours two at a time (same result as one by one)
         const bool dot = dot_out && reverse && u_half;
         const int G = (int)(L.N / 8 / 256);  // workgroups per pair launch
-        for (int cc = 0; cc < 8; cc += 2) {
+        for (int cc = reverse ? 0 : 2 * pair_begin; cc < (reverse ? 8 : 2 * pair_end); cc += 2) {
             ProfEvent ev;
             prof_begin(h, zs ? kProfGsZero : kProfGs, ev);
             launch_fine_gs2(h->stream, L.n, reverse ? 7 - cc : cc, omega, reverse, zs, h->d_ke, h->d_w, u, u_half, f,
@@ -446,8 +451,9 @@ int coarse_factor(homo3d_handle* h) {
 // f0 . u0 of the result (the r . z of a CG step).
 // small_sweeps: smoothing sweeps per side on the levels that run one load case per launch (n <= 32);
 // every other level takes one.
+// head_done: vcycle_head has already launched the first colour pair of the level-0 pre-smoothing.
 int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
-           int u0_half = 0, double* dot_out = nullptr, int small_sweeps = 1) {
+           int u0_half = 0, double* dot_out = nullptr, int small_sweeps = 1, bool head_done = false) {
     const float omega = (float)h->cfg.sor_omega;
     // packed FP16 u: level 0 when the caller's iterate is, and then every coarser level (but the
     // coarsest) whose smoother runs six load cases per launch; the small levels stay FP32
@@ -464,7 +470,8 @@ int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* r
         // a zero-start level-0 sweep leaves its own-colour residuals behind; the residual then needs
         // only the later-colour terms
         const bool rown = l == 0 && u0_zero && residual_after_zero_sweep_ok(h);
-        op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero, uh, nullptr, rown ? L.r : nullptr);
+        op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero, uh, nullptr, rown ? L.r : nullptr,
+                  l == 0 && head_done ? 1 : 0);
         for (int k = 1; k < sweeps; k++) op_smooth(h, l, u, f, omega, 0, false, uh);
         if (rown)
             op_residual_after_zero_sweep(h, u, uh, L.r);
@@ -489,6 +496,17 @@ int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* r
         if (l == 0 && dot_out && !fused) launch_dot_rz(h->stream, 3 * L.N, f0, u0, uh, h->d_partial, kDotBlocks, dot_out);
     }
     return HOMO3D_OK;
+}
+
+// The first launch of a preconditioning V-cycle (u0_zero, level 0 with fused colour pairs): colour
+// pair 0 of the pre-smoothing, queued ahead so that the GPU has work while the host reads the CG
+// norms; vcycle(..., head_done = true) then continues from pair 1.
+bool vcycle_head_ok(const homo3d_handle* h) { return fine_gs_pairs_ok(h->lv[0].n) != 0; }
+
+void vcycle_head(homo3d_handle* h, void* u0, const float* f0, int u0_half) {
+    Level& L = h->lv[0];
+    op_smooth(h, 0, u0, f0, (float)h->cfg.sor_omega, 0, true, u0_half, nullptr,
+              residual_after_zero_sweep_ok(h) ? L.r : nullptr, 0, 1);
 }
 
 int ensure_stage(homo3d_handle* h, size_t elems) {
@@ -525,7 +543,11 @@ constexpr double kInnerTol = 1e-3;
 // (h->Rr) holds the right-hand side and is updated in place to the recursive residual.  A load case
 // stops when ||r|| <= tol[r] * sqrt(r0n2[r]).  Symmetric V(1,1) preconditioner, FP64 inner products.
 // iters[] accumulates the iterations spent per load case.
-int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const bool* done, int maxit, int* iters) {
+// On return *x_pending tells whether x still lacks the last step's alpha p (launch_refine_accumulate
+// adds it on the way into the FP64 iterate).
+int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const bool* done, int maxit, int* iters,
+              bool* x_pending_out) {
+    *x_pending_out = false;
     const Level& L0 = h->lv[0];
     const size_t M = 3 * L0.N;
     const size_t vb = sizeof(float) * kRhs * M;
@@ -548,8 +570,15 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     if (rc) return rc;
     // One CG step: q = K p with p.q, formed as K z + beta q_old since p = z + beta p_old (the operator
     // gathers the packed z instead of the FP32 p).  The same pass advances x by the previous step's
-    // alpha p_old and forms the new p, so p, x, r and q are each streamed once per step; a last x-only
-    // pass follows on exit.
+    // alpha p_old and forms the new p, so p, x, r and q are each streamed once per step; the caller
+    // adds the last step's alpha p.  While the host waits for the residual norms, the next V-cycle's
+    // first colour pair is already queued (if the step turns out to be the last, it only rewrites z
+    // and the level-0 residual, which nothing reads afterwards).
+    if (!h->h_rr) {
+        HIPCHK(h, hipHostMalloc((void**)&h->h_rr, sizeof(double) * kRhs, hipHostMallocDefault));
+        HIPCHK(h, hipEventCreateWithFlags(&h->ev_rr, hipEventDisableTiming));
+    }
+    const bool ahead = vcycle_head_ok(h);
     double rr[kRhs];
     bool x_pending = false;
     for (int it = 0; it < maxit; it++) {
@@ -563,8 +592,12 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         launch_pcg_alpha(h->stream, h->d_sc);
         launch_pcg_update_r(h->stream, M, h->d_sc, h->Q, h->Rr, h->d_partial, kDotBlocks, h->d_tmp);
         x_pending = true;  // alpha p is added by the next operator pass, or by the pass after the loop
-        HIPCHK(h, hipMemcpyAsync(rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->h_rr, h->d_tmp, sizeof(rr), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipEventRecord(h->ev_rr, h->stream));
+        const bool head = ahead && it + 1 < maxit;
+        if (head) vcycle_head(h, h->Z, h->Rr, zh);
+        HIPCHK(h, hipEventSynchronize(h->ev_rr));
+        std::copy(h->h_rr, h->h_rr + kRhs, rr);
         bool changed = false, all = true;
         for (int r = 0; r < kRhs; r++) {
             if (!act[r]) continue;
@@ -581,11 +614,12 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         if (changed)
             HIPCHK(h, hipMemcpyAsync(h->d_sc + 5 * kRhs, sc + 5 * kRhs, sizeof(double) * kRhs, hipMemcpyHostToDevice,
                                      h->stream));
-        rc = vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc + 3 * kRhs, kPrecondSmallSweeps);  // with rz_new
+        rc = vcycle(h, h->Z, h->Rr, 1, nullptr, true, zh, h->d_sc + 3 * kRhs, kPrecondSmallSweeps,
+                    head);  // with rz_new
         if (rc) return rc;
         launch_pcg_beta(h->stream, h->d_sc);
     }
-    if (x_pending) launch_pcg_update_px(h->stream, M, h->d_sc, h->Z, zh, h->P, h->X, 0, kDotBlocks);
+    *x_pending_out = x_pending;
     return HOMO3D_OK;
 }
 
@@ -782,6 +816,8 @@ int homo3d_destroy(homo3d_handle* h) {
     if (h->h_cst) (void)hipHostFree(h->h_cst);
     if (h->h_ainv) (void)hipHostFree(h->h_ainv);
     if (h->ev_coarse) (void)hipEventDestroy(h->ev_coarse);
+    if (h->h_rr) (void)hipHostFree(h->h_rr);
+    if (h->ev_rr) (void)hipEventDestroy(h->ev_rr);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HOMO3D_OK;
@@ -919,12 +955,11 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             double tol_in[kRhs];
             for (int r = 0; r < kRhs; r++) tol_in[r] = std::max(kInnerTol, 0.5 * tol / std::max(rel[r], 1e-300));
             HIPCHK(h, hipMemsetAsync(h->X, 0, sizeof(float) * M6, h->stream));
-            rc = pcg_inner(h, rn2, tol_in, done, maxit, iters);
+            bool x_pending = false;
+            rc = pcg_inner(h, rn2, tol_in, done, maxit, iters, &x_pending);
             if (rc) return rc;
-            if (u_zero)
-                launch_widen_f64(h->stream, M6, h->X, h->U64);
-            else
-                launch_add_correction(h->stream, M6, h->X, h->U64);
+            // U64 = or += the correction x + alpha p (the last CG step's x update, fused)
+            launch_refine_accumulate(h->stream, M, h->d_sc, x_pending ? 1 : 0, h->X, h->P, h->U64, u_zero ? 0 : 1);
             u_zero = false;
         }
         if (u_zero) HIPCHK(h, hipMemsetAsync(h->U64, 0, sizeof(double) * M6, h->stream));

@@ -55,8 +55,11 @@ void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double
                             const double* u, float* r32, double* partial, double* out6, int do_k);
 void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
                        double* out6);
-void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u);
 void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x);
+// u (+)= x + alpha p over 6 load cases of M floats (M % 4 == 0; alpha = sc + 12, only when pending):
+// the last CG step's x update fused into the FP64 refinement iterate (add) or its first value.
+void launch_refine_accumulate(hipStream_t s, size_t M, const double* sc, int pending, const float* x, const float* p,
+                              double* u, int add);
 void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u);
 void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half);
 // FP32 field [6][3][N] <-> packed FP16 field [3][3][N] words (round to nearest)

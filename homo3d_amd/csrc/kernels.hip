@@ -1083,9 +1083,37 @@ __global__ __launch_bounds__(kFullThreads) void k_rhs_norm64(Lay L, size_t N, co
 }
 
 // U64 += d (FP32 correction), all load cases
-__global__ __launch_bounds__(256) void k_add_correction(size_t M, const float* __restrict__ d, double* __restrict__ u) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < M) u[i] += (double)d[i];
+// u (+)= x + alpha p per load case (M floats each, M % 4 == 0; alpha = sc[12 + r], taken as 0 unless
+// pending): the last CG step's x update, rounded to FP32 as the CG pass rounds it, added to the FP64
+// iterate (add) or copied into it, in one pass.
+__global__ __launch_bounds__(256) void k_refine_accumulate(size_t M, const double* __restrict__ sc, int pending,
+                                                           const float* __restrict__ x, const float* __restrict__ p,
+                                                           double* __restrict__ u, int add) {
+    const int r = blockIdx.y;
+    const float a = pending ? (float)sc[12 + r] : 0.f;
+    const float4* x4 = reinterpret_cast<const float4*>(x + (size_t)r * M);
+    const float4* p4 = reinterpret_cast<const float4*>(p + (size_t)r * M);
+    double2* u2 = reinterpret_cast<double2*>(u + (size_t)r * M);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
+        float4 xv = x4[i];
+        if (a != 0.f) {
+            const float4 pv = p4[i];
+            xv.x = fmaf(a, pv.x, xv.x);
+            xv.y = fmaf(a, pv.y, xv.y);
+            xv.z = fmaf(a, pv.z, xv.z);
+            xv.w = fmaf(a, pv.w, xv.w);
+        }
+        double2 lo = make_double2((double)xv.x, (double)xv.y), hi = make_double2((double)xv.z, (double)xv.w);
+        if (add) {
+            const double2 ulo = u2[2 * i], uhi = u2[2 * i + 1];
+            lo.x += ulo.x;
+            lo.y += ulo.y;
+            hi.x += uhi.x;
+            hi.y += uhi.y;
+        }
+        u2[2 * i] = lo;
+        u2[2 * i + 1] = hi;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_round_f32(size_t M, const double* __restrict__ u, float* __restrict__ x) {
@@ -2420,8 +2448,9 @@ void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc
     hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, (int)nb, partial, out6);
 }
 
-void launch_add_correction(hipStream_t s, size_t M, const float* d, double* u) {
-    hipLaunchKernelGGL(k_add_correction, dim3(nblk(M)), dim3(256), 0, s, M, d, u);
+void launch_refine_accumulate(hipStream_t s, size_t M, const double* sc, int pending, const float* x, const float* p,
+                              double* u, int add) {
+    hipLaunchKernelGGL(k_refine_accumulate, dim3(2048, 6), dim3(256), 0, s, M, sc, pending, x, p, u, add);
 }
 
 void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x) {
