@@ -372,6 +372,11 @@ bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
         if (dot) launch_sum_partials(h->stream, kRhs, 4 * G, h->d_partial, dot_out);
         return dot;
     }
+    if (l > 0 && stencil_gs_pairs_ok(L.n, L.lex)) {  // colours two at a time (same result as one by one)
+        for (int cc = 0; cc < 8; cc += 2)
+            launch_stencil_gs2(h->stream, L.n, reverse ? 7 - cc : cc, omega, reverse, zs, L.st, L.st_half, u, u_half, f);
+        return false;
+    }
     for (int cc = 0; cc < 8; cc++) {
         const int c = reverse ? 7 - cc : cc;
         if (l == 0) {

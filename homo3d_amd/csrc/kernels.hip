@@ -1179,11 +1179,12 @@ struct StencilIO<half2_t> {
     }
 };
 
-// A: storage of u (float planes, or half2_t words of load-case pairs; r0 then even).
-template <int R, uint32_t MASK, typename SA, typename A = float>
+// A: storage of u (float planes, or half2_t words of load-case pairs; r0 then even).  XL: the two
+// x-neighbours (d = 12, 14) come from LDS (xl).
+template <int R, uint32_t MASK, typename SA, typename A = float, bool XL = false>
 __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L, size_t N, uint32_t v,
                                                const SA* __restrict__ st, const A* __restrict__ u,
-                                               float (&acc)[R][3], int r0) {
+                                               float (&acc)[R][3], int r0, const XNbLds<A>& xl = XNbLds<A>()) {
     constexpr int W = R / CasesPerWord<A>::value;
     const uint32_t X[3] = {L.part(wrap_dec(x, L.n), 0), L.part(x, 0), L.part(wrap_inc(x, L.n), 0)};
     const uint32_t Y[3] = {L.part(wrap_dec(y, L.n), 1), L.part(y, 1), L.part(wrap_inc(y, L.n), 1)};
@@ -1202,9 +1203,13 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L
         float S[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) S[k] = StencilIO<SA>::get(rs, vo, plane, d * 9 + k);
-        const uint32_t qo = (X[d % 3] + Y[d / 3 % 3] + Z[d / 9]) * 4u;
         A cur[W][3];
-        load_nb<W, A>(cur, ru, qo, plane, r0);
+        if (XL && (d == 12 || d == 14)) {
+            load_nb_lds<W, A>(cur, xl, d == 12 ? 0 : 1);
+        } else {
+            const uint32_t qo = (X[d % 3] + Y[d / 3 % 3] + Z[d / 9]) * 4u;
+            load_nb<W, A>(cur, ru, qo, plane, r0);
+        }
         block_fma<W, float, A>(S, cur, acc);
     }
 }
@@ -1230,20 +1235,18 @@ __global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N,
         }
 }
 
-// One colour of the coarse block GS sweep (gs_relaxation_kernel, grid.cu:556-689); no pin on coarse levels.
-// ZC as in k_fine_gs.  A: u storage (float, or half2_t load-case pairs with FP32 arithmetic).
-template <int R, bool REVERSE, int ZC, typename SA, typename A = float>
-__global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour, float omega,
-                                                    const SA* __restrict__ st, A* __restrict__ u,
-                                                    const float* __restrict__ f) {
-    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= L.h3) return;
-    const uint32_t v = colour_block(colour) * L.h3 + t;
+// GS / SOR update of the colour vertex v of a coarse level (load cases r0 .. r0 + R - 1), in place.
+// ZC as in k_fine_gs.  A: u storage (float, or half2_t load-case pairs with FP32 arithmetic).  XL: the
+// x-neighbours come from LDS; TO_LDS: the new words also go to slot threadIdx.x of out_lds.
+template <int R, bool REVERSE, int ZC, typename SA, typename A, bool XL, bool TO_LDS>
+__device__ __forceinline__ void stencil_gs_vertex(const Lay& L, size_t N, uint32_t v, float omega,
+                                                  const SA* __restrict__ st, A* __restrict__ u,
+                                                  const float* __restrict__ f, int r0, const XNbLds<A>& xl,
+                                                  A* out_lds) {
     int x, y, z;
     L.pos(v, x, y, z);
     float acc[R][3];
-    const int r0 = blockIdx.y * R;
-    stencil_gather<R, gs_mask(ZC), SA, A>(x, y, z, L, N, v, st, u, acc, r0);
+    stencil_gather<R, gs_mask(ZC), SA, A, XL>(x, y, z, L, N, v, st, u, acc, r0, xl);
     float Ks[9];
     const Rsrc rs = make_rsrc(st, (size_t)StencilIO<SA>::words * N * 4);
 #pragma unroll
@@ -1277,8 +1280,47 @@ __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour,
             }
         }
 #pragma unroll
-        for (int i = 0; i < 3; i++) u[(size_t)((r0 / cpw + p) * 3 + i) * N + v] = wd[i];
+        for (int i = 0; i < 3; i++) {
+            u[(size_t)((r0 / cpw + p) * 3 + i) * N + v] = wd[i];
+            if constexpr (TO_LDS) out_lds[(p * 3 + i) * 256 + threadIdx.x] = wd[i];
+        }
     }
+}
+
+// One colour of the coarse block GS sweep (gs_relaxation_kernel, grid.cu:556-689); no pin on coarse levels.
+template <int R, bool REVERSE, int ZC, typename SA, typename A = float>
+__global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour, float omega,
+                                                    const SA* __restrict__ st, A* __restrict__ u,
+                                                    const float* __restrict__ f) {
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= L.h3) return;
+    stencil_gs_vertex<R, REVERSE, ZC, SA, A, false, false>(L, N, colour_block(colour) * L.h3 + t, omega, st, u, f,
+                                                           blockIdx.y * R, XNbLds<A>(), nullptr);
+}
+
+// Two consecutive colours of a coarse sweep in one launch, as k_fine_gs2 does at the finest level
+// (whole grid rows per workgroup; the first colour's new words stay in LDS for the second colour's
+// x-neighbours).
+template <int R, bool REVERSE, int ZC, typename SA, typename A>
+__global__ __launch_bounds__(256) void k_stencil_gs2(Lay L, size_t N, int cfirst, float omega,
+                                                     const SA* __restrict__ st, A* __restrict__ u,
+                                                     const float* __restrict__ f) {
+    constexpr int W = R / CasesPerWord<A>::value;
+    __shared__ A xs[3 * W * 256];
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+    const int c2 = REVERSE ? cfirst - 1 : cfirst + 1;
+    const int r0 = blockIdx.y * R;
+    stencil_gs_vertex<R, REVERSE, ZC, SA, A, false, true>(L, N, colour_block(cfirst) * L.h3 + t, omega, st, u, f, r0,
+                                                          XNbLds<A>(), xs);
+    __syncthreads();
+    XNbLds<A> xl;
+    xl.base = xs;
+    const int h = L.h, lx = (int)(t % (uint32_t)h), row0 = (int)threadIdx.x - lx;
+    const bool odd = (colour_block(c2) & 1u) != 0;  // x-parity of the second colour
+    xl.slot[0] = row0 + (odd ? lx : (lx == 0 ? h - 1 : lx - 1));
+    xl.slot[1] = row0 + (odd ? (lx + 1 == h ? 0 : lx + 1) : lx);
+    stencil_gs_vertex<R, REVERSE, (ZC < 0 ? -1 : ZC + 1), SA, A, true, false>(L, N, colour_block(c2) * L.h3 + t, omega,
+                                                                              st, u, f, r0, xl, nullptr);
 }
 
 // Coarse-vertex kernels that read the fine level around 2V take coarse vertices in lexicographic
@@ -2556,6 +2598,58 @@ static void stencil_gs_st(hipStream_t s, const Lay& L, size_t N, bool split, int
         stencil_gs_pick<1>(s, L, N, colour, omega, reverse, zero_start, st, static_cast<float*>(u), f);
     else
         stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, st, static_cast<float*>(u), f);
+}
+
+template <int R, bool REV, int ZC, typename SA, typename A>
+static void stencil_gs2_colour(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, const SA* st, A* u,
+                               const float* f) {
+    hipLaunchKernelGGL((k_stencil_gs2<R, REV, ZC, SA, A>), dim3(L.h3 / 256, 6 / R), dim3(256), 0, s, L, N, cfirst,
+                       omega, st, u, f);
+}
+
+template <int R, typename SA, typename A>
+static void stencil_gs2_pick(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, int reverse,
+                             int zero_start, const SA* st, A* u, const float* f) {
+    if (reverse) {
+        stencil_gs2_colour<R, true, -1>(s, L, N, cfirst, omega, st, u, f);
+        return;
+    }
+    switch (zero_start ? cfirst : -1) {
+        case 0: stencil_gs2_colour<R, false, 0>(s, L, N, cfirst, omega, st, u, f); break;
+        case 2: stencil_gs2_colour<R, false, 2>(s, L, N, cfirst, omega, st, u, f); break;
+        case 4: stencil_gs2_colour<R, false, 4>(s, L, N, cfirst, omega, st, u, f); break;
+        case 6: stencil_gs2_colour<R, false, 6>(s, L, N, cfirst, omega, st, u, f); break;
+        default: stencil_gs2_colour<R, false, -1>(s, L, N, cfirst, omega, st, u, f); break;
+    }
+}
+
+template <typename SA>
+static void stencil_gs2_st(hipStream_t s, const Lay& L, size_t N, bool split, int cfirst, float omega, int reverse,
+                           int zero_start, const SA* st, void* u, int u_half, const float* f) {
+    if (u_half)
+        stencil_gs2_pick<6>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<half2_t*>(u), f);
+    else if (split)
+        stencil_gs2_pick<1>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<float*>(u), f);
+    else
+        stencil_gs2_pick<6>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<float*>(u), f);
+}
+
+// pairs only where a launch runs six load cases per thread: on the one-case levels (<= 32^3) the
+// launch is latency-bound and two colours in one cost more than two launches (14 vs 11 us at 32^3)
+int stencil_gs_pairs_ok(int n, int lex) { return !lex && n > kSplitLoadCasesMaxN && fine_gs_pairs_ok(n); }
+
+void launch_stencil_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse, int zero_start, const void* st,
+                        int st_half, void* u, int u_half, const float* f) {
+    const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
+    const bool split = n <= kSplitLoadCasesMaxN;
+    if (!stencil_gs_pairs_ok(n, 0)) return;            // the caller checks stencil_gs_pairs_ok
+    if (u_half && !stencil_u_half_ok(n, 0)) return;    // ... and stencil_u_half_ok
+    if (st_half)
+        stencil_gs2_st(s, L, N, split, cfirst, omega, reverse, zero_start, static_cast<const half2_t*>(st), u, u_half,
+                       f);
+    else
+        stencil_gs2_st(s, L, N, split, cfirst, omega, reverse, zero_start, static_cast<const float*>(st), u, u_half, f);
 }
 
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const void* st,

@@ -181,6 +181,27 @@ def test_presmooth_residual_fp16(h3d, orc, n, omega):
         assert rel(r[s], rr) < 2e-6
 
 
+@pytest.mark.parametrize("n,reverse", [(32, False), (32, True), (128, False), (128, True)])
+def test_coarse_gauss_seidel_sweep(h3d, orc, n, reverse):
+    """One level-1 sweep against the FP64 oracle sweep on the same stored stencil: 16^3 runs one colour
+    and one load case per thread, 64^3 colour pairs (whole grid rows per workgroup) with six."""
+    say("coarse sweep: n=%d reverse=%s" % (n, reverse))
+    hom, x = make(h3d, n, seed=3)
+    nc = hom.sizes[1]
+    st = hom.stencil(1)
+    rng = np.random.default_rng(n + 9)
+    u = rng.standard_normal(hom.level_shape(1)).astype(np.float32)
+    f = rng.standard_normal(hom.level_shape(1)).astype(np.float32)
+    hom.set_field(1, 0, u)
+    hom.set_field(1, 1, f)
+    hom.smooth(1, 1.0, reverse)
+    got = hom.get_field(1, 0)
+    for s in range(6):
+        ref = orc.stencil_gs(nc, st, u[s].astype(np.float64), f[s].astype(np.float64), 1.0, reverse)
+        say("  case %d: %.2e" % (s, rel(got[s], ref)))
+        assert rel(got[s], ref) < 1e-5
+
+
 def test_galerkin_stencils(h3d, orc):
     say("galerkin: n=32")
     hom, x = make(h3d, 32, seed=4)
