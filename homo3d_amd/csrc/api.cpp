@@ -47,10 +47,12 @@ const char* const kProfNames[] = {"fine_gs", "fine_apply", "fine_residual", "fin
 constexpr int kProfKinds = 5;
 enum { kProfGs = 0, kProfApply = 1, kProfResidual = 2, kProfGsZero = 3, kProfApplyFirst = 4 };
 constexpr size_t kEventPool = 8192;
-// Levels this small run their smoother one load case per launch; in the PCG preconditioner they take
-// kPrecondSmallSweeps sweeps per side (measured at 256^3: 2 sweeps take 7.14 solves/s with one CG
-// step fewer per load case, against 6.39 with one, and 3 or 4 sweeps 6.50 / 5.95).
+// Levels of kSmallLevelMinN < n <= kSmallLevelN (the largest that run their smoother one load case per
+// launch) take kPrecondSmallSweeps sweeps per side in the PCG preconditioner.  Measured at 256^3,
+// solves/s: 2 sweeps at 32^3 only 7.79; at 32^3 and 16^3 7.71; at 32^3, 16^3 and 8^3 7.69; 3 at
+// 32^3 7.75; 2 at 16^3 and 8^3 7.75; 2 at 8^3 only 6.79; 1 everywhere 6.39 (an older build).
 constexpr int kSmallLevelN = 32;
+constexpr int kSmallLevelMinN = 16;
 constexpr int kPrecondSmallSweeps = 2;
 
 }  // namespace
@@ -454,7 +456,7 @@ int coarse_factor(homo3d_handle* h) {
 // the cycle starts from u0 = 0 (u0's contents are ignored), as the coarse levels always do.
 // u0_half: u0 is packed FP16 (level 0 only; every other field is FP32).  dot_out: per-load-case
 // f0 . u0 of the result (the r . z of a CG step).
-// small_sweeps: smoothing sweeps per side on the levels that run one load case per launch (n <= 32);
+// small_sweeps: smoothing sweeps per side on the levels with kSmallLevelMinN < n <= kSmallLevelN;
 // every other level takes one.
 // head_done: vcycle_head has already launched the first colour pair of the level-0 pre-smoothing.
 int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* rn2, bool u0_zero = false,
@@ -471,7 +473,7 @@ int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* r
         void* u = l == 0 ? u0 : L.u;
         const float* f = l == 0 ? f0 : L.f;
         const int uh = u_half_of(l);
-        const int sweeps = l > 0 && L.n <= kSmallLevelN ? small_sweeps : 1;
+        const int sweeps = l > 0 && L.n <= kSmallLevelN && L.n > kSmallLevelMinN ? small_sweeps : 1;
         // a zero-start level-0 sweep leaves its own-colour residuals behind; the residual then needs
         // only the later-colour terms
         const bool rown = l == 0 && u0_zero && residual_after_zero_sweep_ok(h);
@@ -495,7 +497,7 @@ int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* r
         const float* f = l == 0 ? f0 : L.f;
         const int uh = u_half_of(l);
         launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, u_half_of(l + 1), u, uh, l == 0 ? 1 : 0);
-        const int sweeps = l > 0 && L.n <= kSmallLevelN ? small_sweeps : 1;
+        const int sweeps = l > 0 && L.n <= kSmallLevelN && L.n > kSmallLevelMinN ? small_sweeps : 1;
         for (int k = 1; k < sweeps; k++) op_smooth(h, l, u, f, omega, symmetric ? 1 : 0, false, uh);
         const bool fused = op_smooth(h, l, u, f, omega, symmetric ? 1 : 0, false, uh, l == 0 ? dot_out : nullptr);
         if (l == 0 && dot_out && !fused) launch_dot_rz(h->stream, 3 * L.N, f0, u0, uh, h->d_partial, kDotBlocks, dot_out);
