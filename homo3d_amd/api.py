@@ -59,6 +59,7 @@ EXPORTS = {
     "homo3d_apply_K16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_smooth16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
     "homo3d_presmooth16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "homo3d_presmooth": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     "homo3d_smooth": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "homo3d_restrict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_prolong": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -292,6 +293,11 @@ class Homogenization:
         receives the result and r[0] the residual f - K u, formed from the sweep's own-colour residuals
         and the later-colour terms."""
         self._chk(lib().homo3d_presmooth16(self._h, float(omega)), "presmooth16")
+
+    def presmooth(self, level, omega=1.0):
+        """A coarse level's pre-smoothing: a forward sweep from u = 0 with f[level]; u[level] receives the
+        result and r[level] the residual, formed from the own-colour residuals and the later-colour terms."""
+        self._chk(lib().homo3d_presmooth(self._h, int(level), float(omega)), "presmooth")
 
     def smooth(self, level=0, omega=1.0, reverse=False):
         self._chk(lib().homo3d_smooth(self._h, level, float(omega), int(reverse)), "smooth")

@@ -351,8 +351,8 @@ void prof_end(homo3d_handle* h, ProfEvent& ev) {
 // dot_out (level 0, reverse sweeps of a packed-FP16 u with fused colour pairs; returns false otherwise):
 // also the per-load-case f . u after the sweep, accumulated by the pair kernels as each vertex gets its
 // final value.
-// rown (level 0, zero-start forward sweeps with fused colour pairs, else ignored): the passes also store
-// their own-colour residuals there, for op_residual_after_zero_sweep.  [pair_begin, pair_end): the
+// rown (zero-start forward sweeps with fused colour pairs, else ignored): the passes also store their
+// own-colour residuals there, for op_residual_after_zero_sweep.  [pair_begin, pair_end): the
 // colour pairs to launch (level 0 with fused pairs, forward sweeps; a sweep split in two launches the
 // same kernels as a whole one).
 bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, int reverse, bool zero_start = false,
@@ -376,7 +376,8 @@ bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
     }
     if (l > 0 && stencil_gs_pairs_ok(L.n, L.lex)) {  // colours two at a time (same result as one by one)
         for (int cc = 0; cc < 8; cc += 2)
-            launch_stencil_gs2(h->stream, L.n, reverse ? 7 - cc : cc, omega, reverse, zs, L.st, L.st_half, u, u_half, f);
+            launch_stencil_gs2(h->stream, L.n, reverse ? 7 - cc : cc, omega, reverse, zs, L.st, L.st_half, u, u_half, f,
+                               zs ? rown : nullptr);
         return false;
     }
     for (int cc = 0; cc < 8; cc++) {
@@ -393,12 +394,20 @@ bool op_smooth(homo3d_handle* h, int l, void* u, const float* f, float omega, in
     return false;
 }
 
-// Level 0 after a zero-start forward sweep that stored its own-colour residuals in r (op_smooth
-// rown): r = f - K u from the later-colour terms alone.
-bool residual_after_zero_sweep_ok(const homo3d_handle* h) { return fine_gs_pairs_ok(h->lv[0].n) != 0; }
+// After a zero-start forward sweep that stored its own-colour residuals in r (op_smooth rown):
+// r = f - K u from the later-colour terms alone.  Level 0 needs fused colour pairs; coarse levels
+// their colour pairs (six load cases per launch).
+bool residual_after_zero_sweep_ok(const homo3d_handle* h, int l = 0) {
+    const Level& L = h->lv[l];
+    return l == 0 ? fine_gs_pairs_ok(L.n) != 0 : stencil_gs_pairs_ok(L.n, L.lex) != 0;
+}
 
-void op_residual_after_zero_sweep(homo3d_handle* h, const void* u, int u_half, float* r) {
-    const Level& L = h->lv[0];
+void op_residual_after_zero_sweep(homo3d_handle* h, int l, const void* u, int u_half, float* r) {
+    const Level& L = h->lv[l];
+    if (l > 0) {
+        launch_stencil_residual_later(h->stream, L.n, L.st, L.st_half, u, u_half, r);
+        return;
+    }
     ProfEvent ev;
     prof_begin(h, kProfResidual, ev);
     launch_fine_residual_later(h->stream, L.n, h->d_ke, h->d_w, u, u_half, r);
@@ -474,14 +483,14 @@ int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* r
         const float* f = l == 0 ? f0 : L.f;
         const int uh = u_half_of(l);
         const int sweeps = l > 0 && L.n <= kSmallLevelN && L.n > kSmallLevelMinN ? small_sweeps : 1;
-        // a zero-start level-0 sweep leaves its own-colour residuals behind; the residual then needs
+        // a single zero-start sweep leaves its own-colour residuals behind; the residual then needs
         // only the later-colour terms
-        const bool rown = l == 0 && u0_zero && residual_after_zero_sweep_ok(h);
+        const bool rown = (l > 0 || u0_zero) && sweeps == 1 && residual_after_zero_sweep_ok(h, l);
         op_smooth(h, l, u, f, omega, 0, l > 0 || u0_zero, uh, nullptr, rown ? L.r : nullptr,
                   l == 0 && head_done ? 1 : 0);
         for (int k = 1; k < sweeps; k++) op_smooth(h, l, u, f, omega, 0, false, uh);
         if (rown)
-            op_residual_after_zero_sweep(h, u, uh, L.r);
+            op_residual_after_zero_sweep(h, l, u, uh, L.r);
         else
             op_residual(h, l, u, f, L.r, uh);
         if (l == 0 && rn2) launch_dot6(h->stream, 3 * L.N, L.r, L.r, h->d_partial, kDotBlocks, rn2);
@@ -1178,10 +1187,23 @@ int homo3d_presmooth16(homo3d_handle* h, double omega) {
     if (!residual_after_zero_sweep_ok(h)) return set_err(h, HOMO3D_NO_SUPPORT, "needs fused colour pairs at level 0");
     const Level& L = h->lv[0];
     op_smooth(h, 0, h->Z, h->F, (float)omega, 0, true, 1, nullptr, L.r);
-    op_residual_after_zero_sweep(h, h->Z, 1, L.r);
+    op_residual_after_zero_sweep(h, 0, h->Z, 1, L.r);
     launch_unpack_half(h->stream, L.N, h->Z, h->X);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_presmooth(homo3d_handle* h, int level, double omega) {
+    if (!h || level < 1 || level >= h->nl - 1) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    if (!residual_after_zero_sweep_ok(h, level))
+        return set_err(h, HOMO3D_NO_SUPPORT, "level %d has no colour pairs", level);
+    Level& L = h->lv[level];
+    op_smooth(h, level, L.u, L.f, (float)omega, 0, true, 0, nullptr, L.r);
+    op_residual_after_zero_sweep(h, level, L.u, 0, L.r);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
 

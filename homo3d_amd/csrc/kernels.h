@@ -76,9 +76,13 @@ void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int revers
 // Colours cfirst and cfirst + 1 (cfirst even; reverse: cfirst and cfirst - 1, cfirst odd) of a coarse
 // parity-blocked level in one launch, identical to two launch_stencil_gs calls; only when
 // stencil_gs_pairs_ok(n, lex) (levels above 32^3 whose grid rows fit a workgroup).
+// rown (zero-start forward passes; ignored otherwise): the own-colour residuals, as launch_fine_gs2
+// stores them; launch_stencil_residual_later completes them to f - A u after the sweep.
 int stencil_gs_pairs_ok(int n, int lex);
 void launch_stencil_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse, int zero_start, const void* st,
-                        int st_half, void* u, int u_half, const float* f);
+                        int st_half, void* u, int u_half, const float* f, float* rown = nullptr);
+void launch_stencil_residual_later(hipStream_t s, int n, const void* st, int st_half, const void* u, int u_half,
+                                   float* r);
 void launch_restrict(hipStream_t s, int nf, int lexc, const float* rf, float* fc);
 // uf += P uc.  uc FP32, or packed FP16 (uc_half: parity-blocked coarse level, packed FP16 fine u).
 void launch_prolong(hipStream_t s, int nf, int lexc, const void* uc, int uc_half, void* uf, int u_half, int pin);

@@ -1237,12 +1237,14 @@ __global__ __launch_bounds__(kFullThreads) void k_stencil_apply(Lay L, size_t N,
 
 // GS / SOR update of the colour vertex v of a coarse level (load cases r0 .. r0 + R - 1), in place.
 // ZC as in k_fine_gs.  A: u storage (float, or half2_t load-case pairs with FP32 arithmetic).  XL: the
-// x-neighbours come from LDS; TO_LDS: the new words also go to slot threadIdx.x of out_lds.
-template <int R, bool REVERSE, int ZC, typename SA, typename A, bool XL, bool TO_LDS>
+// x-neighbours come from LDS; TO_LDS: the new words also go to slot threadIdx.x of out_lds.  ROWN
+// (ZC >= 0): also the own-colour residual f - (A u)_{colours <= ZC} to rown, as gs_vertex does.
+template <int R, bool REVERSE, int ZC, typename SA, typename A, bool XL, bool TO_LDS, bool ROWN = false>
 __device__ __forceinline__ void stencil_gs_vertex(const Lay& L, size_t N, uint32_t v, float omega,
                                                   const SA* __restrict__ st, A* __restrict__ u,
                                                   const float* __restrict__ f, int r0, const XNbLds<A>& xl,
-                                                  A* out_lds) {
+                                                  A* out_lds, float* __restrict__ rown = nullptr) {
+    static_assert(!ROWN || ZC >= 0, "the own-colour residual needs a zero-start forward pass");
     int x, y, z;
     L.pos(v, x, y, z);
     float acc[R][3];
@@ -1278,6 +1280,21 @@ __device__ __forceinline__ void stencil_gs_vertex(const Lay& L, size_t N, uint32
                 else
                     wd[i] = uu[i];
             }
+            if constexpr (ROWN) {
+                float us[3];  // the stored values
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    if constexpr (cpw == 2)
+                        us[i] = (float)wd[i][hh];
+                    else
+                        us[i] = (float)wd[i];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const float kv = fmaf(Ks[3 * i], us[0], fmaf(Ks[3 * i + 1], us[1], Ks[3 * i + 2] * us[2]));
+                    rown[(size_t)((r0 + r) * 3 + i) * N + v] = (ff[i] - acc[r][i]) - kv;
+                }
+            }
         }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -1301,17 +1318,17 @@ __global__ __launch_bounds__(256) void k_stencil_gs(Lay L, size_t N, int colour,
 // Two consecutive colours of a coarse sweep in one launch, as k_fine_gs2 does at the finest level
 // (whole grid rows per workgroup; the first colour's new words stay in LDS for the second colour's
 // x-neighbours).
-template <int R, bool REVERSE, int ZC, typename SA, typename A>
+template <int R, bool REVERSE, int ZC, typename SA, typename A, bool ROWN = false>
 __global__ __launch_bounds__(256) void k_stencil_gs2(Lay L, size_t N, int cfirst, float omega,
                                                      const SA* __restrict__ st, A* __restrict__ u,
-                                                     const float* __restrict__ f) {
+                                                     const float* __restrict__ f, float* __restrict__ rown) {
     constexpr int W = R / CasesPerWord<A>::value;
     __shared__ A xs[3 * W * 256];
     const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
     const int c2 = REVERSE ? cfirst - 1 : cfirst + 1;
     const int r0 = blockIdx.y * R;
-    stencil_gs_vertex<R, REVERSE, ZC, SA, A, false, true>(L, N, colour_block(cfirst) * L.h3 + t, omega, st, u, f, r0,
-                                                          XNbLds<A>(), xs);
+    stencil_gs_vertex<R, REVERSE, ZC, SA, A, false, true, ROWN>(L, N, colour_block(cfirst) * L.h3 + t, omega, st, u, f,
+                                                                r0, XNbLds<A>(), xs, rown);
     __syncthreads();
     XNbLds<A> xl;
     xl.base = xs;
@@ -1319,8 +1336,51 @@ __global__ __launch_bounds__(256) void k_stencil_gs2(Lay L, size_t N, int cfirst
     const bool odd = (colour_block(c2) & 1u) != 0;  // x-parity of the second colour
     xl.slot[0] = row0 + (odd ? lx : (lx == 0 ? h - 1 : lx - 1));
     xl.slot[1] = row0 + (odd ? (lx + 1 == h ? 0 : lx + 1) : lx);
-    stencil_gs_vertex<R, REVERSE, (ZC < 0 ? -1 : ZC + 1), SA, A, true, false>(L, N, colour_block(c2) * L.h3 + t, omega,
-                                                                              st, u, f, r0, xl, nullptr);
+    stencil_gs_vertex<R, REVERSE, (ZC < 0 ? -1 : ZC + 1), SA, A, true, false, ROWN>(
+        L, N, colour_block(c2) * L.h3 + t, omega, st, u, f, r0, xl, nullptr, rown);
+}
+
+// The coarse-level counterpart of k_fine_resid_later: r = r_own - the later-colour terms, on the
+// 512-thread parity tiles (wave b takes colour 7 - b; colour 7 is final).  Load cases r0 .. r0 + R - 1
+// with r0 = blockIdx.y R.  zero as in k_fine_resid_later.
+template <int R, int C, typename SA, typename A>
+__device__ __forceinline__ void stencil_later_vertex(const Lay& L, size_t N, uint32_t v, const SA* __restrict__ st,
+                                                     const A* __restrict__ u, float* __restrict__ r, int r0,
+                                                     float zero) {
+    int x, y, z;
+    L.pos(v, x, y, z);
+    float acc[R][3];
+    stencil_gather<R, later_mask(C), SA, A>(x, y, z, L, N, v, st, u, acc, r0);
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        float ro[3], rn[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) ro[i] = r[(size_t)((r0 + q) * 3 + i) * N + v];
+        rn[0] = ro[0] - acc[q][0];
+        rn[1] = fmaf(-zero, rn[0], ro[1] - acc[q][1]);
+        rn[2] = fmaf(-zero, rn[1], ro[2] - acc[q][2]);
+#pragma unroll
+        for (int i = 0; i < 3; i++) r[(size_t)((r0 + q) * 3 + i) * N + v] = rn[i];
+    }
+}
+
+template <int R, typename SA, typename A>
+__global__ __launch_bounds__(kFullThreads) void k_stencil_resid_later(Lay L, size_t N, const SA* __restrict__ st,
+                                                                      const A* __restrict__ u, float* __restrict__ r,
+                                                                      float zero) {
+    uint32_t v;
+    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
+    const int r0 = blockIdx.y * R;
+    switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
+        case 7: stencil_later_vertex<R, 0, SA, A>(L, N, v, st, u, r, r0, zero); break;
+        case 6: stencil_later_vertex<R, 1, SA, A>(L, N, v, st, u, r, r0, zero); break;
+        case 5: stencil_later_vertex<R, 2, SA, A>(L, N, v, st, u, r, r0, zero); break;
+        case 4: stencil_later_vertex<R, 3, SA, A>(L, N, v, st, u, r, r0, zero); break;
+        case 3: stencil_later_vertex<R, 4, SA, A>(L, N, v, st, u, r, r0, zero); break;
+        case 2: stencil_later_vertex<R, 5, SA, A>(L, N, v, st, u, r, r0, zero); break;
+        case 1: stencil_later_vertex<R, 6, SA, A>(L, N, v, st, u, r, r0, zero); break;
+        default: break;
+    }
 }
 
 // Coarse-vertex kernels that read the fine level around 2V take coarse vertices in lexicographic
@@ -2600,38 +2660,47 @@ static void stencil_gs_st(hipStream_t s, const Lay& L, size_t N, bool split, int
         stencil_gs_pick<6>(s, L, N, colour, omega, reverse, zero_start, st, static_cast<float*>(u), f);
 }
 
-template <int R, bool REV, int ZC, typename SA, typename A>
+template <int R, bool REV, int ZC, typename SA, typename A, bool ROWN = false>
 static void stencil_gs2_colour(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, const SA* st, A* u,
-                               const float* f) {
-    hipLaunchKernelGGL((k_stencil_gs2<R, REV, ZC, SA, A>), dim3(L.h3 / 256, 6 / R), dim3(256), 0, s, L, N, cfirst,
-                       omega, st, u, f);
+                               const float* f, float* rown = nullptr) {
+    hipLaunchKernelGGL((k_stencil_gs2<R, REV, ZC, SA, A, ROWN>), dim3(L.h3 / 256, 6 / R), dim3(256), 0, s, L, N, cfirst,
+                       omega, st, u, f, rown);
+}
+
+template <int R, int ZC, typename SA, typename A>
+static void stencil_gs2_zero(hipStream_t s, const Lay& L, size_t N, float omega, const SA* st, A* u, const float* f,
+                             float* rown) {
+    if (rown)
+        stencil_gs2_colour<R, false, ZC, SA, A, true>(s, L, N, ZC, omega, st, u, f, rown);
+    else
+        stencil_gs2_colour<R, false, ZC>(s, L, N, ZC, omega, st, u, f);
 }
 
 template <int R, typename SA, typename A>
 static void stencil_gs2_pick(hipStream_t s, const Lay& L, size_t N, int cfirst, float omega, int reverse,
-                             int zero_start, const SA* st, A* u, const float* f) {
+                             int zero_start, const SA* st, A* u, const float* f, float* rown) {
     if (reverse) {
         stencil_gs2_colour<R, true, -1>(s, L, N, cfirst, omega, st, u, f);
         return;
     }
     switch (zero_start ? cfirst : -1) {
-        case 0: stencil_gs2_colour<R, false, 0>(s, L, N, cfirst, omega, st, u, f); break;
-        case 2: stencil_gs2_colour<R, false, 2>(s, L, N, cfirst, omega, st, u, f); break;
-        case 4: stencil_gs2_colour<R, false, 4>(s, L, N, cfirst, omega, st, u, f); break;
-        case 6: stencil_gs2_colour<R, false, 6>(s, L, N, cfirst, omega, st, u, f); break;
+        case 0: stencil_gs2_zero<R, 0>(s, L, N, omega, st, u, f, rown); break;
+        case 2: stencil_gs2_zero<R, 2>(s, L, N, omega, st, u, f, rown); break;
+        case 4: stencil_gs2_zero<R, 4>(s, L, N, omega, st, u, f, rown); break;
+        case 6: stencil_gs2_zero<R, 6>(s, L, N, omega, st, u, f, rown); break;
         default: stencil_gs2_colour<R, false, -1>(s, L, N, cfirst, omega, st, u, f); break;
     }
 }
 
 template <typename SA>
 static void stencil_gs2_st(hipStream_t s, const Lay& L, size_t N, bool split, int cfirst, float omega, int reverse,
-                           int zero_start, const SA* st, void* u, int u_half, const float* f) {
+                           int zero_start, const SA* st, void* u, int u_half, const float* f, float* rown) {
     if (u_half)
-        stencil_gs2_pick<6>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<half2_t*>(u), f);
+        stencil_gs2_pick<6>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<half2_t*>(u), f, rown);
     else if (split)
-        stencil_gs2_pick<1>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<float*>(u), f);
+        stencil_gs2_pick<1>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<float*>(u), f, rown);
     else
-        stencil_gs2_pick<6>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<float*>(u), f);
+        stencil_gs2_pick<6>(s, L, N, cfirst, omega, reverse, zero_start, st, static_cast<float*>(u), f, rown);
 }
 
 // pairs only where a launch runs six load cases per thread: on the one-case levels (<= 32^3) the
@@ -2639,17 +2708,46 @@ static void stencil_gs2_st(hipStream_t s, const Lay& L, size_t N, bool split, in
 int stencil_gs_pairs_ok(int n, int lex) { return !lex && n > kSplitLoadCasesMaxN && fine_gs_pairs_ok(n); }
 
 void launch_stencil_gs2(hipStream_t s, int n, int cfirst, float omega, int reverse, int zero_start, const void* st,
-                        int st_half, void* u, int u_half, const float* f) {
+                        int st_half, void* u, int u_half, const float* f, float* rown) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
     const bool split = n <= kSplitLoadCasesMaxN;
     if (!stencil_gs_pairs_ok(n, 0)) return;            // the caller checks stencil_gs_pairs_ok
     if (u_half && !stencil_u_half_ok(n, 0)) return;    // ... and stencil_u_half_ok
+    if (reverse || !zero_start) rown = nullptr;        // own-colour residuals only for zero-start passes
     if (st_half)
         stencil_gs2_st(s, L, N, split, cfirst, omega, reverse, zero_start, static_cast<const half2_t*>(st), u, u_half,
-                       f);
+                       f, rown);
     else
-        stencil_gs2_st(s, L, N, split, cfirst, omega, reverse, zero_start, static_cast<const float*>(st), u, u_half, f);
+        stencil_gs2_st(s, L, N, split, cfirst, omega, reverse, zero_start, static_cast<const float*>(st), u, u_half, f,
+                       rown);
+}
+
+template <int R, typename SA, typename A>
+static void stencil_resid_later(hipStream_t s, const Lay& L, size_t N, const SA* st, const A* u, float* r) {
+    hipLaunchKernelGGL((k_stencil_resid_later<R, SA, A>), dim3(full_grid(L), 6 / R), dim3(kFullThreads), 0, s, L, N,
+                       st, u, r, 0.f);
+}
+
+template <typename SA>
+static void stencil_resid_later_st(hipStream_t s, const Lay& L, size_t N, const SA* st, const void* u, int u_half,
+                                   float* r) {
+    if (u_half)
+        stencil_resid_later<6>(s, L, N, st, static_cast<const half2_t*>(u), r);
+    else
+        stencil_resid_later<6>(s, L, N, st, static_cast<const float*>(u), r);
+}
+
+void launch_stencil_residual_later(hipStream_t s, int n, const void* st, int st_half, const void* u, int u_half,
+                                   float* r) {
+    const size_t N = (size_t)n * n * n;
+    const Lay L = make_lay(n, 0);
+    if (!stencil_gs_pairs_ok(n, 0)) return;          // the caller checks stencil_gs_pairs_ok
+    if (u_half && !stencil_u_half_ok(n, 0)) return;  // ... and stencil_u_half_ok
+    if (st_half)
+        stencil_resid_later_st(s, L, N, static_cast<const half2_t*>(st), u, u_half, r);
+    else
+        stencil_resid_later_st(s, L, N, static_cast<const float*>(st), u, u_half, r);
 }
 
 void launch_stencil_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const void* st,

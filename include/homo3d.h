@@ -136,6 +136,10 @@ int homo3d_smooth16(homo3d_handle* h, double omega, int reverse);
  * sweep leaves behind plus the later-colour terms.  Needs n % 8 == 0 and fused colour pairs
  * ((n/2) | 256, 256 | (n/2)^3); HOMO3D_NO_SUPPORT otherwise. */
 int homo3d_presmooth16(homo3d_handle* h, double omega);
+/* The same pre-smoothing on a coarse level (FP32 u): one forward sweep from u[level] = 0 with
+ * f[level], and r[level] = f - A u from the own-colour residuals plus the later-colour terms.  Only
+ * levels that run colour pairs (n > 32, (n/2) | 256, 256 | (n/2)^3); HOMO3D_NO_SUPPORT otherwise. */
+int homo3d_presmooth(homo3d_handle* h, int level, double omega);
 int homo3d_restrict(homo3d_handle* h, int level);  /* f[level] = P^T r[level-1] */
 int homo3d_prolong(homo3d_handle* h, int level);   /* u[level] += P u[level+1] */
 int homo3d_coarse_solve(homo3d_handle* h);

@@ -202,6 +202,32 @@ def test_coarse_gauss_seidel_sweep(h3d, orc, n, reverse):
         assert rel(got[s], ref) < 1e-5
 
 
+@pytest.mark.parametrize("omega", [1.0, 0.8])
+def test_coarse_presmooth_residual(h3d, orc, omega):
+    """A coarse level's zero-start pre-smoothing (64^3, colour pairs, six load cases per thread) with
+    the residual formed from its own-colour residuals plus the later-colour terms: u against the FP64
+    oracle sweep from zero and r against the FP64 stencil residual of the returned u, both on the same
+    stored stencil."""
+    say("coarse presmooth residual: level 1 of 128^3, omega=%g" % omega)
+    hom, x = make(h3d, 128, seed=5)
+    nc = hom.sizes[1]
+    st = hom.stencil(1)
+    rng = np.random.default_rng(17)
+    f = rng.standard_normal(hom.level_shape(1)).astype(np.float32)
+    hom.set_field(1, 1, f)
+    hom.set_field(1, 0, rng.standard_normal(hom.level_shape(1)).astype(np.float32))  # ignored
+    hom.presmooth(1, omega)
+    u = hom.get_field(1, 0)
+    r = hom.get_field(1, 2)
+    zero = np.zeros(u.shape[1:], np.float64)
+    for s in range(6):
+        ref = orc.stencil_gs(nc, st, zero, f[s].astype(np.float64), omega, False)
+        rr = orc.stencil_residual(nc, st, u[s].astype(np.float64), f[s].astype(np.float64))
+        say("  case %d: u %.2e, r %.2e" % (s, rel(u[s], ref), rel(r[s], rr)))
+        assert rel(u[s], ref) < 1e-5
+        assert rel(r[s], rr) < 1e-5
+
+
 def test_galerkin_stencils(h3d, orc):
     say("galerkin: n=32")
     hom, x = make(h3d, 32, seed=4)
