@@ -73,6 +73,8 @@ def measured_traffic(name, n):
 
 
 def cpu_baseline(n_sample, n_target, tol):
+    """The FP64 oracle's solve of the same design (Galerkin set-up + six load cases by MG-PCG), timed on
+    the host threads.  Returns the baseline record and the oracle's C^H."""
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
@@ -82,16 +84,28 @@ def cpu_baseline(n_sample, n_target, tol):
     mg.set_density(rho ** 3)
     C, chi, code, its = mg.homogenize(tol=tol, maxit=400, mode="pcg")
     dt = time.perf_counter() - t0
+    del chi, mg
     scale = (n_sample / n_target) ** 3
-    return {
-        "value": (1.0 / dt) * scale,
-        "unit": "solves/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": ("FP64 oracle, full solve at %d^3 (Galerkin set-up + 6 load cases, MG-PCG to %.0e, iters %s) "
-                   "in %.1f s on %d threads; scaled by (%d/%d)^3 to %d^3" %
-                   (n_sample, tol, its, dt, threads, n_sample, n_target, n_target)),
-    }
+    if n_sample == n_target:
+        sample = ("%d^3 measured: FP64 oracle, one full solve (Galerkin set-up + 6 load cases, MG-PCG to %.0e, "
+                  "CG steps %s) in %.1f s on %d threads" % (n_sample, tol, its, dt, threads))
+    else:
+        sample = ("FP64 oracle, full solve at %d^3 (Galerkin set-up + 6 load cases, MG-PCG to %.0e, iters %s) "
+                  "in %.1f s on %d threads; scaled by (%d/%d)^3 to %d^3" %
+                  (n_sample, tol, its, dt, threads, n_sample, n_target, n_target))
+    return {"value": (1.0 / dt) * scale, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": sample}, C
+
+
+def ch_errors(C, Cref):
+    """Per-entry relative error of C^H against the oracle over the entries that are not zero by
+    symmetry (|Cref_ij| > 1e-6 max|Cref|), and the largest absolute error relative to max|Cref|."""
+    C, Cref = np.asarray(C, float), np.asarray(Cref, float)
+    big = np.abs(Cref) > 1e-6 * np.abs(Cref).max()
+    rel = np.abs(C - Cref)[big] / np.abs(Cref)[big]
+    return {"max_rel_err_per_entry": float("%.3e" % rel.max()),
+            "max_abs_err_over_max_entry": float("%.3e" % (np.abs(C - Cref).max() / np.abs(Cref).max())),
+            "entries_compared": int(big.sum())}
 
 
 def main():
@@ -101,7 +115,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-5)
-    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-n", type=int, default=None, help="oracle grid for the CPU baseline (default: --n)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--omega", type=float, default=None, help="smoother relaxation factor (default: the library's)")
     ap.add_argument("--mfma", type=int, default=None,
@@ -113,38 +127,49 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import homo3d_amd
 
     n = args.n
     log("bench: n=%d tol=%.1e steps=%d warmup=%d" % (n, args.tol, args.steps, args.warmup))
     rho = homo3d_amd.randc_density(n, 0, 0.2, "rotate3")
-    x = torch.from_numpy((rho.astype(np.float64) ** 3).astype(np.float32)).to("cuda:0")
-    torch.cuda.synchronize()
+    x = homo3d_amd.DeviceBuffer.from_array((rho.astype(np.float64) ** 3).astype(np.float32))
+    sync = homo3d_amd.DeviceBuffer.synchronize
+    sync()
     kw = {} if args.mfma is None else {"fine_mfma": bool(args.mfma)}
     if args.omega is not None:
         kw["sor_omega"] = args.omega
     hom = homo3d_amd.Homogenization(homo3d_amd.HomoConfig(reso=n, solver="pcg", max_iter=400, **kw))
 
     def step():
-        hom.update(x.data_ptr(), n, on_device=True)
+        hom.update(x.value, n, on_device=True)
         st = hom.solve(args.tol, warm=False)
         C = hom.elasticMatrix(solve=False)
         return st, C
 
+    def check(st, what):
+        # every counted solve must have met the tolerance on all six load cases
+        if st["status"] != 0 or not max(st["relres"]) <= args.tol:
+            log("%s: status %d relres %s: not converged, no number reported" % (what, st["status"], st["relres"]))
+            sys.exit(1)
+
+    worst = 0.0
     for i in range(args.warmup):
         st, C = step()
+        check(st, "warmup %d" % i)
         log("warmup %d: iters %s relres %.2e %.2fs" % (i, st["iters"], max(st["relres"]), st["seconds"]))
     hom.profile_enable(True, only=DOMINANT)
-    torch.cuda.synchronize()
+    sync()
     t0 = time.perf_counter()
     stats = []
     for i in range(args.steps):
         st, C = step()
         stats.append(st)
-        log("step %d: iters %s relres %.2e" % (i, st["iters"], max(st["relres"])))
-    torch.cuda.synchronize()
+    sync()
     dt = time.perf_counter() - t0
+    for i, st in enumerate(stats):
+        check(st, "step %d" % i)
+        worst = max(worst, max(st["relres"]))
+        log("step %d: iters %s relres %.2e" % (i, st["iters"], max(st["relres"])))
     dom = DOMINANT
     cnt, ms = hom.profile_get(dom)
     hom.profile_enable(False)
@@ -184,12 +209,16 @@ def main():
                                "(V(1,1) 8-colour block GS, FP64 coarsest) to relres %.0e, seeded randc density "
                                "(seed 0, rotate3, vol 0.2), E=10 nu=0.3 p=3" % (n, args.tol),
                    "n": n, "tol": args.tol, "solver": "mg-pcg", "warm_start": "none",
-                   "iters": stats[-1]["iters"], "relres": [float("%.3e" % r) for r in stats[-1]["relres"]]},
+                   "iters": stats[-1]["iters"], "relres": [float("%.3e" % r) for r in stats[-1]["relres"]],
+                   "worst_relres_all_steps": float("%.3e" % worst)},
         "roofline": roofline,
     }
     if not args.no_cpu:
-        log("cpu baseline: oracle at %d^3" % args.cpu_n)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, n, args.tol)
+        cpu_n = args.cpu_n or n
+        log("cpu baseline: oracle at %d^3" % cpu_n)
+        out["cpu_baseline"], Cref = cpu_baseline(cpu_n, n, args.tol)
+        if cpu_n == n:
+            out["config"]["ch_vs_fp64_oracle"] = ch_errors(C, Cref)
     print(json.dumps(out), flush=True)
 
 

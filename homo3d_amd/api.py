@@ -50,6 +50,8 @@ EXPORTS = {
     "homo3d_elastic_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_sensitivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                           ctypes.c_int, ctypes.c_int]),
+    "homo3d_sensitivity_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "homo3d_field_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_int]),
     "homo3d_solution_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
@@ -65,6 +67,7 @@ EXPORTS = {
     "homo3d_prolong": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_coarse_solve": (ctypes.c_int, [ctypes.c_void_p]),
     "homo3d_vcycle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_vcycle_sweeps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "homo3d_get_stencil": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "homo3d_symmetrize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int]),
@@ -79,6 +82,7 @@ EXPORTS = {
     "homo3d_mma_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "homo3d_mma_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "homo3d_mma_set_bound": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "homo3d_mma_set_bounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "homo3d_mma_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     "homo3d_mma_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -109,6 +113,77 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceBuffer:
+    """A device allocation made through the HIP runtime that libhomo3d.so itself links (no PyTorch):
+    for callers that keep densities or sensitivities resident in HBM and pass on_device=True."""
+
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            lib()  # loads libamdhip64 as a dependency of libhomo3d.so
+            h = ctypes.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            for f in (h.hipMalloc, h.hipFree, h.hipMemcpy, h.hipDeviceSynchronize):
+                f.restype = ctypes.c_int
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        rc = self._rt().hipMalloc(ctypes.byref(self.ptr), self.nbytes)
+        if rc:
+            raise Homo3dError(HOMO3D_HIP_ERROR, "hipMalloc(%d) failed (%d)" % (self.nbytes, rc))
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        buf.upload(a)
+        return buf
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        rc = self._rt().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1)  # host to device
+        if rc:
+            raise Homo3dError(HOMO3D_HIP_ERROR, "hipMemcpy H2D failed (%d)" % rc)
+
+    def download(self, dtype=np.float32, count=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize if count is None else count, dtype=dtype)
+        rc = self._rt().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, 2)  # device to host
+        if rc:
+            raise Homo3dError(HOMO3D_HIP_ERROR, "hipMemcpy D2H failed (%d)" % rc)
+        return out
+
+    @classmethod
+    def synchronize(cls):
+        rc = cls._rt().hipDeviceSynchronize()
+        if rc:
+            raise Homo3dError(HOMO3D_HIP_ERROR, "hipDeviceSynchronize failed (%d)" % rc)
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            self._rt().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def randc_density(n, seed, vol, sym="reflect6"):
@@ -240,13 +315,14 @@ class Homogenization:
         self._chk(lib().homo3d_elastic_matrix(self._h, C), "elasticMatrix")
         return np.array(C[:]).reshape(6, 6)
 
-    def Sensitivity(self, dC, pitchT=-1):
-        """Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder=true): returns dJ/dx per cell."""
+    def Sensitivity(self, dC, pitchT=-1, lexiOrder=True):
+        """Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder): returns dJ/dx per cell, lexicographic
+        and pitched, or (lexiOrder=False) in the library's parity-blocked cell order."""
         d = np.ascontiguousarray(dC, dtype=np.float32).ravel()
         pitch = self.n if pitchT < 0 else pitchT
-        out = np.zeros(pitch * self.n * self.n, dtype=np.float32)
-        self._chk(lib().homo3d_sensitivity(self._h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ptr(out),
-                                           int(pitch), 0), "Sensitivity")
+        out = np.zeros((pitch if lexiOrder else self.n) * self.n * self.n, dtype=np.float32)
+        self._chk(lib().homo3d_sensitivity_order(self._h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                 _ptr(out), int(pitch), int(bool(lexiOrder)), 0), "Sensitivity")
         return out
 
     # --- Grid / MG level operations (all 6 load cases) ---
@@ -313,6 +389,10 @@ class Homogenization:
 
     def vcycle(self, symmetric=False):
         self._chk(lib().homo3d_vcycle(self._h, int(symmetric)), "vcycle")
+
+    def v_cycle(self, w_SOR=1.0, pre=1, post=1):
+        """MG::v_cycle(w_SOR, pre, post) on the FP32 level-0 fields."""
+        self._chk(lib().homo3d_vcycle_sweeps(self._h, float(w_SOR), int(pre), int(post)), "v_cycle")
 
     def stencil(self, level):
         out = np.zeros((self.sizes[level] ** 3, 27, 3, 3), dtype=np.float32)
@@ -392,7 +472,13 @@ class MMAOptimizer:
             raise Homo3dError(rc, "%s: %s" % (what, lib().homo3d_mma_last_error(self._h).decode()))
 
     def setBound(self, xmin, xmax):
-        self._chk(lib().homo3d_mma_set_bound(self._h, float(xmin), float(xmax)), "setBound")
+        """setBound(float xmin, float xmax) or, with arrays, setBound(float* xmin, float* xmax)."""
+        if np.isscalar(xmin) and np.isscalar(xmax):
+            self._chk(lib().homo3d_mma_set_bound(self._h, float(xmin), float(xmax)), "setBound")
+            return
+        lo = np.ascontiguousarray(np.broadcast_to(xmin, (self.n,)), dtype=np.float32)
+        hi = np.ascontiguousarray(np.broadcast_to(xmax, (self.n,)), dtype=np.float32)
+        self._chk(lib().homo3d_mma_set_bounds(self._h, _ptr(lo), _ptr(hi), 0), "setBound")
 
     def update(self, itn, x, dfdx, gval, dgdx):
         """One MMA step; returns the new design (float32).  dgdx: sequence of m gradient arrays."""

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -209,8 +210,13 @@ double macro_disp(int node, int s, int c) {
 double hat(int o) { return o == 0 ? 1.0 : ((o == 1 || o == -1) ? 0.5 : 0.0); }
 
 // Level sizes (Grid::buildRoot / coarse2, grid.cpp:21-151) in integer arithmetic.
+// Upper limit: the finest-level gathers address each field through one buffer resource with 32-bit
+// byte offsets, and the largest field, the FP64 iterate [6][3][n^3], must stay below 2^32 bytes:
+// 144 n^3 < 2^32 holds up to n = 309, and the next solvable size after 256 is 320 (c * 2^6).
+constexpr int kMaxReso = 256;
+static_assert(144ull * kMaxReso * kMaxReso * kMaxReso < (1ull << 32), "FP64 field exceeds 32-bit offsets");
 int level_sizes(int n, int* sizes) {
-    if (n < 8 || n > 512) return -1;
+    if (n < 8 || n > kMaxReso) return -1;
     int fl = 0;
     while ((1 << (fl + 1)) <= n) fl++;
     const int k = std::max(fl - 2, 0);
@@ -514,6 +520,34 @@ int vcycle(homo3d_handle* h, void* u0, const float* f0, int symmetric, double* r
     return HOMO3D_OK;
 }
 
+// MG::v_cycle(w_SOR, pre, post) (MG.cpp:46-70) with arbitrary sweep counts and relaxation factor, on
+// the FP32 level-0 fields: `pre` forward sweeps, residual, restriction on the way down; the coarsest
+// solve; prolongation and `post` forward sweeps on the way up.  A coarse level's correction starts
+// at zero (the first sweep skips the not yet visited colours; with pre = 0 it is cleared).
+int vcycle_sweeps(homo3d_handle* h, float omega, int pre, int post) {
+    for (int l = 0; l < h->nl - 1; l++) {
+        Level& L = h->lv[l];
+        float* u = l == 0 ? h->X : L.u;
+        const float* f = l == 0 ? h->F : L.f;
+        if (l > 0 && pre == 0) HIPCHK(h, hipMemsetAsync(L.u, 0, sizeof(float) * 3 * kRhs * L.N, h->stream));
+        for (int k = 0; k < pre; k++) op_smooth(h, l, u, f, omega, 0, l > 0 && k == 0, 0);
+        op_residual(h, l, u, f, L.r, 0);
+        launch_restrict(h->stream, L.n, h->lv[l + 1].lex, L.r, h->lv[l + 1].f);
+    }
+    const Level& Lc = h->lv[h->nl - 1];
+    int rc = coarse_factor(h);
+    if (rc) return rc;
+    launch_coarse_solve(h->stream, Lc.n, h->d_Ainv, Lc.f, Lc.u);
+    for (int l = h->nl - 2; l >= 0; l--) {
+        Level& L = h->lv[l];
+        float* u = l == 0 ? h->X : L.u;
+        const float* f = l == 0 ? h->F : L.f;
+        launch_prolong(h->stream, L.n, h->lv[l + 1].lex, h->lv[l + 1].u, 0, u, 0, l == 0 ? 1 : 0);
+        for (int k = 0; k < post; k++) op_smooth(h, l, u, f, omega, 0, false, 0);
+    }
+    return HOMO3D_OK;
+}
+
 // The first launch of a preconditioning V-cycle (u0_zero, level 0 with fused colour pairs): colour
 // pair 0 of the pre-smoothing, queued ahead so that the GPU has work while the host reads the CG
 // norms; vcycle(..., head_done = true) then continues from pair 1.
@@ -618,6 +652,10 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         for (int r = 0; r < kRhs; r++) {
             if (!act[r]) continue;
             iters[r]++;
+            if (!std::isfinite(rr[r])) {
+                *x_pending_out = true;
+                return set_err(h, HOMO3D_DIVERGED, "non-finite CG residual norm (load case %d)", r);
+            }
             if (rr[r] <= tol[r] * tol[r] * r0n2[r]) {
                 act[r] = false;
                 sc[5 * kRhs + r] = 0.0;
@@ -958,11 +996,17 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             HIPCHK(h, hipMemcpyAsync(rn2, h->d_tmp, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(h, hipStreamSynchronize(h->stream));
             if (outer == 0 && u_zero) std::copy(rn2, rn2 + kRhs, fn2);
-            bool all = true;
+            bool all = true, finite = true;
             for (int r = 0; r < kRhs; r++) {
                 rel[r] = fn2[r] > 0 ? std::sqrt(rn2[r] / fn2[r]) : 0.0;
                 done[r] = rel[r] <= tol;
                 all = all && done[r];
+                finite = finite && std::isfinite(rn2[r]) && std::isfinite(fn2[r]);
+            }
+            if (!finite) {
+                status = HOMO3D_DIVERGED;
+                u_zero = false;
+                break;
             }
             if (all || outer >= kMaxOuter) break;
             int total = 0;
@@ -973,6 +1017,12 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             HIPCHK(h, hipMemsetAsync(h->X, 0, sizeof(float) * M6, h->stream));
             bool x_pending = false;
             rc = pcg_inner(h, rn2, tol_in, done, maxit, iters, &x_pending);
+            if (rc == HOMO3D_DIVERGED) {
+                status = rc;
+                for (int r = 0; r < kRhs; r++) rel[r] = std::numeric_limits<double>::quiet_NaN();
+                u_zero = false;
+                break;
+            }
             if (rc) return rc;
             // U64 = or += the correction x + alpha p (the last CG step's x update, fused)
             launch_refine_accumulate(h->stream, M, h->d_sc, x_pending ? 1 : 0, h->X, h->P, h->U64, u_zero ? 0 : 1);
@@ -984,6 +1034,7 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
         launch_round_f32(h->stream, M6, h->U64, h->X);
         HIPCHK(h, hipStreamSynchronize(h->stream));
         prof_flush(h);
+        if (status == HOMO3D_DIVERGED) set_err(h, status, "non-finite residual in the refined MG-PCG solve");
         if (stats) {
             stats->status = status;
             for (int r = 0; r < kRhs; r++) {
@@ -1032,21 +1083,24 @@ int homo3d_elastic_matrix(homo3d_handle* h, double C[36]) {
     return HOMO3D_OK;
 }
 
-// dJ/dx_e for dJ/dC = dC (Homogenization::Sensitivity, homogenization.cu:860-948), lexicographic pitched.
-int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int on_device) {
+// dJ/dx_e for dJ/dC = dC (Homogenization::Sensitivity, homogenization.cu:860-948).  lexi_order: the
+// lexicographic pitched layout; otherwise the level's own cell order (parity-blocked, n^3 unpitched),
+// the counterpart of the reference's GS-order output for lexiOrder = false.
+static int sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int lexi_order, int on_device) {
     if (!h || !dC || !sens) return HOMO3D_BAD_ARG;
     on_device = resolve_on_device(sens, on_device);
     int rc = require_density(h);
     if (rc) return rc;
     const int n = h->lv[0].n;
-    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    if (lexi_order && pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    if (!lexi_order) pitch = 0;
     float wdc[21];
     int k = 0;
     for (int i = 0; i < 6; i++)
         for (int j = i; j < 6; j++) wdc[k++] = dC[i * 6 + j] + (i != j ? dC[j * 6 + i] : 0.f);
     HIPCHK(h, hipMemcpyAsync(h->d_ftmp, wdc, sizeof(wdc), hipMemcpyHostToDevice, h->stream));
     float* dst = sens;
-    const size_t elems = (size_t)pitch * n * n;
+    const size_t elems = lexi_order ? (size_t)pitch * n * n : h->lv[0].N;
     if (!on_device) {
         rc = ensure_stage(h, elems);
         if (rc) return rc;
@@ -1056,6 +1110,15 @@ int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pi
     if (!on_device) HIPCHK(h, hipMemcpyAsync(sens, dst, sizeof(float) * elems, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
+}
+
+int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int on_device) {
+    return sensitivity(h, dC, sens, pitch, 1, on_device);
+}
+
+int homo3d_sensitivity_order(homo3d_handle* h, const float dC[36], float* sens, int pitch, int lexi_order,
+                             int on_device) {
+    return sensitivity(h, dC, sens, pitch, lexi_order, on_device);
 }
 
 static float* field_ptr(homo3d_handle* h, int level, int field) {
@@ -1176,6 +1239,7 @@ int homo3d_smooth16(homo3d_handle* h, double omega, int reverse) {
     launch_pack_half(h->stream, L.N, h->X, h->Z);
     op_smooth(h, 0, h->Z, h->F, (float)omega, reverse, false, 1);
     launch_unpack_half(h->stream, L.N, h->Z, h->X);
+    h->x_dirty = true;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
     return HOMO3D_OK;
@@ -1189,6 +1253,7 @@ int homo3d_presmooth16(homo3d_handle* h, double omega) {
     op_smooth(h, 0, h->Z, h->F, (float)omega, 0, true, 1, nullptr, L.r);
     op_residual_after_zero_sweep(h, 0, h->Z, 1, L.r);
     launch_unpack_half(h->stream, L.N, h->Z, h->X);
+    h->x_dirty = true;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
     return HOMO3D_OK;
@@ -1212,6 +1277,7 @@ int homo3d_smooth(homo3d_handle* h, int level, double omega, int reverse) {
     int rc = require_density(h);
     if (rc) return rc;
     op_smooth(h, level, field_ptr(h, level, 0), field_ptr(h, level, 1), (float)omega, reverse);
+    if (level == 0) h->x_dirty = true;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);
     return HOMO3D_OK;
@@ -1230,6 +1296,7 @@ int homo3d_prolong(homo3d_handle* h, int level) {
     if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
     launch_prolong(h->stream, h->lv[level].n, h->lv[level + 1].lex, field_ptr(h, level + 1, 0), 0, field_ptr(h, level, 0), 0,
                    level == 0 ? 1 : 0);
+    if (level == 0) h->x_dirty = true;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
@@ -1251,6 +1318,19 @@ int homo3d_vcycle(homo3d_handle* h, int symmetric) {
     int rc = require_density(h);
     if (rc) return rc;
     rc = vcycle(h, h->X, h->F, symmetric, nullptr);
+    h->x_dirty = true;
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_flush(h);
+    return HOMO3D_OK;
+}
+
+int homo3d_vcycle_sweeps(homo3d_handle* h, double omega, int pre, int post) {
+    if (!h || pre < 0 || post < 0 || pre > 64 || post > 64 || !(omega > 0 && omega < 2)) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    rc = vcycle_sweeps(h, (float)omega, pre, post);
+    h->x_dirty = true;
     if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_flush(h);

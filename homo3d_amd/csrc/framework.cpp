@@ -428,24 +428,31 @@ std::vector<double> run_mma(cfg::HomoConfig config, const ObjectiveFn& objective
 // example_opti_shear_isotropy (Framework.cu:282-356): maximise the shear modulus with an isotropy
 // (Zener ratio) constraint and a volume constraint, by MMA.  The reference's Spline4 density filter
 // and sigmoid in front of rho^3 are not applied here.
+// Objective and isotropy constraint of example_opti_shear_isotropy (Framework.cu:300-323) for a given C.
+// The constraint value is scale * (scale (A/B - 1)^2) - 0.1 with A = 2 (C33 + C44 + C55) and
+// B = C00 + C11 + C22 - C01 - C02 - C12, while its gradient is that of the expression the reference
+// back-propagates, scale (A/B - 1)^2: one factor of scale less, as there.
+double shear_objective(const double C[6][6], float dC[6][6]) {
+    dC[3][3] = dC[4][4] = dC[5][5] = -1.f / 3.f;
+    return -(C[3][3] + C[4][4] + C[5][5]) / 3.0;
+}
+
+MmaConstraint shear_isotropy_constraint(const double C[6][6]) {
+    const double A = (C[3][3] + C[4][4] + C[5][5]) * 2, B = C[0][0] + C[1][1] + C[2][2] - C[0][1] - C[0][2] - C[1][2];
+    const double r = A / B - 1.0, scale = 1000.0;
+    MmaConstraint k{};
+    k.value = r * r * scale * scale - 0.1;
+    const double dr = 2 * r * scale;
+    const double dA = dr / B, dB = -dr * A / (B * B);
+    k.dC[3][3] = k.dC[4][4] = k.dC[5][5] = (float)(2 * dA);
+    k.dC[0][0] = k.dC[1][1] = k.dC[2][2] = (float)dB;
+    k.dC[0][1] = k.dC[0][2] = k.dC[1][2] = (float)-dB;
+    return k;
+}
+
 std::vector<double> example_opti_shear_isotropy(cfg::HomoConfig config) {
-    auto obj = [](int, const double C[6][6], float dC[6][6]) {
-        dC[3][3] = dC[4][4] = dC[5][5] = -1.f / 3.f;
-        return -(C[3][3] + C[4][4] + C[5][5]) / 3.0;
-    };
-    auto aniso = [](const double C[6][6]) {
-        const double A = (C[3][3] + C[4][4] + C[5][5]) * 2, B = C[0][0] + C[1][1] + C[2][2] - C[0][1] - C[0][2] - C[1][2];
-        const double r = A / B - 1.0, scale = 1000.0;
-        MmaConstraint k{};
-        k.value = r * r * scale * scale - 0.1;
-        // d/dC of scale^2 (A/B - 1)^2
-        const double dr = 2 * r * scale * scale;
-        const double dA = dr / B, dB = -dr * A / (B * B);
-        k.dC[3][3] = k.dC[4][4] = k.dC[5][5] = (float)(2 * dA);
-        k.dC[0][0] = k.dC[1][1] = k.dC[2][2] = (float)dB;
-        k.dC[0][1] = k.dC[0][2] = k.dC[1][2] = (float)-dB;
-        return std::vector<MmaConstraint>{k};
-    };
+    auto obj = [](int, const double C[6][6], float dC[6][6]) { return shear_objective(C, dC); };
+    auto aniso = [](const double C[6][6]) { return std::vector<MmaConstraint>{shear_isotropy_constraint(C)}; };
     return run_mma(config, obj, aniso);
 }
 
@@ -481,6 +488,21 @@ std::vector<double> runInstance(cfg::HomoConfig config) {
 }  // namespace homo
 
 // main (main.cpp:22-48) as a C entry point: parse flags, run the instance, map exceptions to exit codes.
+// The shear-isotropy objective and constraint terms for a given C (row-major double[36]), for callers
+// that assemble the optimisation themselves: obj, dJ/dC, constraint value and dg/dC (float[36] each).
+extern "C" int homo3d_shear_isotropy_terms(const double* C36, double* obj, float* dobj36, double* g, float* dg36) {
+    if (!C36 || !obj || !dobj36 || !g || !dg36) return 3;
+    double C[6][6];
+    for (int i = 0; i < 36; i++) C[i / 6][i % 6] = C36[i];
+    float dC[6][6] = {};
+    *obj = homo::shear_objective(C, dC);
+    for (int i = 0; i < 36; i++) dobj36[i] = dC[i / 6][i % 6];
+    const auto k = homo::shear_isotropy_constraint(C);
+    *g = k.value;
+    for (int i = 0; i < 36; i++) dg36[i] = k.dC[i / 6][i % 6];
+    return 0;
+}
+
 extern "C" int homo3d_framework_main(int argc, char** argv) {
     try {
         cfg::HomoConfig config;

@@ -2123,7 +2123,8 @@ __global__ __launch_bounds__(kFullThreads) void k_sensitivity(Lay L, size_t N, i
     float s = 0.f;
 #pragma unroll
     for (int k = 0; k < 21; k++) s = fmaf(E[k], wdc[k], s);
-    sens[x + ((size_t)y + (size_t)z * L.n) * pitch] = s / (float)N;
+    // pitch 0: the level's own (parity-blocked) cell order, unpitched
+    sens[pitch ? x + ((size_t)y + (size_t)z * L.n) * pitch : (size_t)e] = s / (float)N;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -138,6 +138,37 @@ int homo3d_mma_set_bound(homo3d_mma* h, double xmin, double xmax) {
     return HOMO3D_OK;
 }
 
+// MMAOptimizer::setBound(float* xmin, float* xmax) (mmaOptimizer.h:30): per-variable bounds, n floats
+// each, in host (on_device = 0) or device memory (1); -1 classifies the pointers.
+int homo3d_mma_set_bounds(homo3d_mma* h, const float* xmin, const float* xmax, int on_device) {
+    if (!h || !xmin || !xmax) return HOMO3D_BAD_ARG;
+    if (on_device < 0) {
+        hipPointerAttribute_t attr;
+        if (hipPointerGetAttributes(&attr, xmin) != hipSuccess) {
+            (void)hipGetLastError();
+            on_device = 0;
+        } else {
+            on_device = (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) ? 1 : 0;
+        }
+    }
+    if (on_device) {
+        launch_f32_to_f64(h->stream, h->n, xmin, h->xmin);
+        launch_f32_to_f64(h->stream, h->n, xmax, h->xmax);
+        MMACHK(h, hipStreamSynchronize(h->stream));
+        return HOMO3D_OK;
+    }
+    std::vector<double> lo(h->n), hi(h->n);
+    for (long i = 0; i < h->n; i++) {
+        if (!(xmax[i] >= xmin[i])) return mma_err(h, HOMO3D_BAD_ARG, "setBound: xmax < xmin");
+        lo[i] = xmin[i];
+        hi[i] = xmax[i];
+    }
+    MMACHK(h, hipMemcpyAsync(h->xmin, lo.data(), sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+    MMACHK(h, hipMemcpyAsync(h->xmax, hi.data(), sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+    MMACHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
 // One MMA step (MMAOptimizer::update, mmaOptimizer.cu:61-77): x (n floats) is updated in place.
 // dfdx: n floats; gval: m floats (host); dgdx: m pointers to n floats.  Arrays are device
 // pointers when on_device != 0, host pointers otherwise.

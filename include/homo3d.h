@@ -25,7 +25,8 @@
  *   homo3d_restrict          Grid::restrict_residual (grid.h:249, grid.cu:1257-1275)
  *   homo3d_prolong           Grid::prolongate_correction (grid.h:247, grid.cu:1238-1255)
  *   homo3d_coarse_solve      Grid::solveHostEquation (grid.h:236, grid.cpp:349-374)
- *   homo3d_vcycle            MG::v_cycle (MG.h:26, MG.cpp:46-70)
+ *   homo3d_vcycle            MG::v_cycle (MG.h:26, MG.cpp:46-70); homo3d_vcycle_sweeps with its
+ *                            w_SOR / pre / post arguments
  *   homo3d_field_copy        Grid::v3_upload / v3_download (grid.h:339-340)
  *   homo3d_solution_copy     Grid::uchar_h read/write, the stored load-case solutions used as warm start
  *                            (grid.h:142, grid.cpp:341-345, 459-462), here the FP64 refinement iterate
@@ -75,7 +76,9 @@ typedef struct homo3d_handle homo3d_handle;
 
 /* Mirrors cfg::HomoConfig (cmdline.h:39-60) for the fields the solve path reads. */
 typedef struct {
-    int reso[3];            /* cells per axis; cubic grids, 8 <= n <= 512 */
+    int reso[3];            /* cells per axis; cubic grids, 8 <= n <= 256 and n = c * 2^k with 4 <= c <= 7
+                               (coarsest level 4^3..7^3); larger n would overflow the 32-bit byte offsets
+                               of the FP64 field gathers and is rejected (HOMO3D_NO_SUPPORT) */
     double youngs_modulus;  /* E (reference default 10) */
     double poisson_ratio;   /* nu (0.3) */
     double emin;            /* stiffness floor added to x_e (reference rhoPenalMin = 1e-9) */
@@ -116,6 +119,11 @@ int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_de
 int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* stats);
 int homo3d_elastic_matrix(homo3d_handle* h, double C[36]);
 int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int on_device);
+/* Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder) (homogenization.h:38): lexi_order = 1 is
+ * homo3d_sensitivity; 0 writes the library's own cell order (parity-blocked: eight (n/2)^3 blocks,
+ * b = (x&1) | (y&1)<<1 | (z&1)<<2, lexicographic in (x>>1, y>>1, z>>1) inside), n^3 floats, pitch ignored. */
+int homo3d_sensitivity_order(homo3d_handle* h, const float dC[36], float* sens, int pitch, int lexi_order,
+                             int on_device);
 
 /* Grid / MG level operations (all six load cases at once). */
 int homo3d_field_copy(homo3d_handle* h, int level, int field, float* host, int to_device);
@@ -144,6 +152,10 @@ int homo3d_restrict(homo3d_handle* h, int level);  /* f[level] = P^T r[level-1] 
 int homo3d_prolong(homo3d_handle* h, int level);   /* u[level] += P u[level+1] */
 int homo3d_coarse_solve(homo3d_handle* h);
 int homo3d_vcycle(homo3d_handle* h, int symmetric);
+/* MG::v_cycle(w_SOR, pre, post) (MG.h:26, MG.cpp:46-70) on the FP32 level-0 fields u, f: `pre` forward
+ * GS sweeps with relaxation omega, residual and restriction per level, the coarsest solve, then
+ * prolongation and `post` forward sweeps.  0 <= pre, post <= 64, 0 < omega < 2. */
+int homo3d_vcycle_sweeps(homo3d_handle* h, double omega, int pre, int post);
 int homo3d_get_stencil(homo3d_handle* h, int level, float* host); /* [N][27][3][3] */
 
 /* Density-update kernels.  Arrays are lexicographic with row pitch `pitch` (elements). */
@@ -160,6 +172,8 @@ int homo3d_mma_create(int m, long n, double a0, double a, double c, double d, in
 int homo3d_mma_destroy(homo3d_mma* h);
 const char* homo3d_mma_last_error(const homo3d_mma* h);
 int homo3d_mma_set_bound(homo3d_mma* h, double xmin, double xmax);
+/* MMAOptimizer::setBound(float* xmin, float* xmax) (mmaOptimizer.h:30): n per-variable bounds each. */
+int homo3d_mma_set_bounds(homo3d_mma* h, const float* xmin, const float* xmax, int on_device);
 /* x: n floats updated in place; dfdx: n floats; gval: m floats on the host; dgdx: m arrays of n floats.
  * on_device selects device (1) or host (0) for x, dfdx and the dgdx arrays. */
 int homo3d_mma_update(homo3d_mma* h, int itn, float* x, const float* dfdx, const float* gval, const float* const* dgdx,

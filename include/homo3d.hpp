@@ -127,8 +127,9 @@ struct MG {
     homo3d_handle* h = nullptr;
     std::vector<std::shared_ptr<Grid>> grids;
     std::shared_ptr<Grid> getRootGrid() { return grids.front(); }
-    void v_cycle(float /*w_SOR*/ = 1.f, int /*pre*/ = 1, int /*post*/ = 1) {
-        int rc = homo3d_vcycle(h, 0);
+    // MG::v_cycle(w_SOR, pre, post) (MG.cpp:46-70): forward GS sweeps with relaxation w_SOR.
+    void v_cycle(float w_SOR = 1.f, int pre = 1, int post = 1) {
+        int rc = homo3d_vcycle_sweeps(h, w_SOR, pre, post);
         if (rc) throw Homo3dError(rc, std::string("v_cycle: ") + homo3d_last_error(h));
     }
     // MG::solveEquation (MG.cpp:113-203): all six load cases; returns the worst relative residual.
@@ -162,7 +163,8 @@ struct Homogenization {
         }
         grid = mg_->grids.front();
     }
-    Homogenization(int xreso, int yreso, int zreso, double youngsModu = 10.0, double poissonRatio = 0.3)
+    // Homogenization(int, int, int, double youngsModu = 1e6, double poissonRatio = 0.3) (homogenization.h:20)
+    Homogenization(int xreso, int yreso, int zreso, double youngsModu = 1e6, double poissonRatio = 0.3)
         : Homogenization(make_cfg(xreso, yreso, zreso, youngsModu, poissonRatio)) {}
     ~Homogenization() {
         if (h_) homo3d_destroy(h_);
@@ -192,10 +194,43 @@ struct Homogenization {
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) C[i][j] = (float)c[i][j];
     }
-    // Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder): lexicographic order only.
-    void Sensitivity(float dC[6][6], float* sens, int pitchT, bool lexiOrder = true) {
-        if (!lexiOrder) throw Homo3dError(HOMO3D_NO_SUPPORT, "Sensitivity: only lexicographic order is supported");
-        chk(homo3d_sensitivity(h_, &dC[0][0], sens, pitchT < 0 ? grid->cellReso[0] : pitchT, -1), "Sensitivity");
+    // Homogenization::elasticMatrix(int i, int j) (homogenization.h:30): C^H_ij from the stored solutions.
+    double elasticMatrix(int i, int j) {
+        if (i < 0 || i > 5 || j < 0 || j > 5) throw Homo3dError(HOMO3D_BAD_ARG, "elasticMatrix: index out of range");
+        double c[36];
+        chk(homo3d_elastic_matrix(h_, c), "elasticMatrix");
+        return c[i * 6 + j];
+    }
+    // Homogenization::elasticMatrix(float* rho, int i, int j) (homogenization.h:32): update with x = rho
+    // (unpitched), solve the six load cases, return C^H_ij.
+    double elasticMatrix(float* rho, int i, int j) {
+        update(rho);
+        double C[6][6];
+        elasticMatrix(C);
+        return C[i][j];
+    }
+    // Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder) (homogenization.h:38).  lexiOrder = false
+    // writes the library's own cell order (parity-blocked, n^3 unpitched; see homo3d_sensitivity_order).
+    void Sensitivity(float dC[6][6], float* sens, int pitchT, bool lexiOrder = false) {
+        chk(homo3d_sensitivity_order(h_, &dC[0][0], sens, pitchT < 0 ? grid->cellReso[0] : pitchT, lexiOrder ? 1 : 0,
+                                     -1),
+            "Sensitivity");
+    }
+    // Homogenization::Sensitivity(int i, int j, float* sens) (homogenization.h:34): dC^H_ij/dx per cell
+    // from the stored solutions, lexicographic and unpitched.
+    void Sensitivity(int i, int j, float* sens) {
+        if (i < 0 || i > 5 || j < 0 || j > 5) throw Homo3dError(HOMO3D_BAD_ARG, "Sensitivity: index out of range");
+        float dC[6][6] = {};
+        dC[i][j] = 1.f;
+        Sensitivity(dC, sens, grid->cellReso[0], true);
+    }
+    // Homogenization::Sensitivity(float* rho, int i, int j, float* sens) (homogenization.h:36): update
+    // with x = rho, solve, then Sensitivity(i, j, sens).
+    void Sensitivity(float* rho, int i, int j, float* sens) {
+        double C[6][6];
+        update(rho);
+        elasticMatrix(C);
+        Sensitivity(i, j, sens);
     }
     std::shared_ptr<Grid> getGrid() { return grid; }
     MG& mg() { return *mg_; }
@@ -284,6 +319,8 @@ struct MMAOptimizer {
     MMAOptimizer(const MMAOptimizer&) = delete;
     MMAOptimizer& operator=(const MMAOptimizer&) = delete;
     void setBound(float xmin, float xmax) { chk(homo3d_mma_set_bound(h_, xmin, xmax), "setBound"); }
+    // setBound(float* xmin, float* xmax) (mmaOptimizer.h:30): n bounds each, host or device memory
+    void setBound(float* xmin, float* xmax) { chk(homo3d_mma_set_bounds(h_, xmin, xmax, -1), "setBound"); }
     // update(int itn, float* x, float* dfdx, float* gval, float** dgdx) (mmaOptimizer.h:34)
     void update(int itn, float* x, float* dfdx, float* gval, float** dgdx) {
         chk(homo3d_mma_update(h_, itn, x, dfdx, gval, (const float* const*)dgdx, -1), "update");

@@ -77,6 +77,8 @@ def lib():
             "orc_mma_update": (c_int, [c_int, c_long, c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                        _dp, _dp, c_double, _dp, _dp, _dp, c_double]),
             "orc_randc_density": (None, [c_int, ctypes.c_uint64, c_double, c_int, _fp]),
+            "orc_randc_coeffs": (None, [ctypes.c_uint64, _dp]),
+            "orc_randc_field": (None, [c_int, _dp, c_double, c_int, _fp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -275,6 +277,18 @@ def mma_update(state, iter_, df0dx, fval, dfdx, a0=1.0, a=None, c=None, d=None, 
 def randc_density(n, seed, vol, sym="reflect6"):
     out = np.zeros(int(n) ** 3, dtype=np.float32)
     lib().orc_randc_density(int(n), int(seed), float(vol), SYM[sym] if isinstance(sym, str) else int(sym), out)
+    return out
+
+
+def randc_coeffs(seed):
+    alpha = np.zeros(512)
+    lib().orc_randc_coeffs(int(seed), alpha)
+    return alpha
+
+
+def randc_field(n, alpha, vol, sym="none"):
+    out = np.zeros(int(n) ** 3, dtype=np.float32)
+    lib().orc_randc_field(int(n), _c(alpha), float(vol), SYM[sym] if isinstance(sym, str) else int(sym), out)
     return out
 
 

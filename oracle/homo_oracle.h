@@ -75,6 +75,8 @@ int orc_mma_update(int m, long n, int iter, double* x, const double* xmin, const
                    double* xold2, const double* df0dx, const double* fval, const double* dfdx, double* low,
                    double* upp, double a0, const double* a, const double* c, const double* d, double move);
 void orc_randc_density(int n, uint64_t seed, double vol, int sym, float* out);
+void orc_randc_coeffs(uint64_t seed, double* alpha);
+void orc_randc_field(int n, const double* alpha, double vol, int sym, float* out);
 
 #ifdef __cplusplus
 }

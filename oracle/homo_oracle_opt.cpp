@@ -451,8 +451,7 @@ int orc_mma_update(int m, long n, int iter, double* x, const double* xmin, const
 //   rho1 = 1/(1+exp(-3 rho~)), rho2 = symmetry projection of rho1,
 //   rho = clamp(s * rho2, 1e-3, 1) with s found by 100 bisection steps so that mean(rho) = vol.
 // Coefficient k = ((((a-1)*4+(b-1))*4+(c-1))*8 + t), t bit j selects sin on axis j.
-void orc_randc_density(int n, uint64_t seed, double vol, int sym, float* out) {
-    double alpha[512];
+void orc_randc_coeffs(uint64_t seed, double* alpha) {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     for (uint32_t j = 0; j < 128; j++) {
         uint32_t ctr[4] = {j, 0x72616e64u, 0, 0}, r[4];
@@ -465,6 +464,16 @@ void orc_randc_density(int n, uint64_t seed, double vol, int sym, float* out) {
             alpha[4 * j + 2 * h + 1] = rad * std::sin(2 * M_PI * u2);
         }
     }
+}
+
+void orc_randc_density(int n, uint64_t seed, double vol, int sym, float* out) {
+    double alpha[512];
+    orc_randc_coeffs(seed, alpha);
+    orc_randc_field(n, alpha, vol, sym, out);
+}
+
+// The density of a given coefficient set alpha[512] (index as above).
+void orc_randc_field(int n, const double* alpha, double vol, int sym, float* out) {
     // per-axis tables T[f][trig][i]
     std::vector<double> T(4 * 2 * n);
     for (int f = 0; f < 4; f++)

@@ -49,3 +49,17 @@ def test_bad_config_is_rejected_without_gpu():
     assert ei.value.code == homo3d_amd.HOMO3D_NO_SUPPORT
     with pytest.raises(homo3d_amd.Homo3dError):
         homo3d_amd.Homogenization(homo3d_amd.HomoConfig(reso=4))
+
+
+@pytest.mark.parametrize("n", [320, 384, 448, 512, 1024])
+def test_sizes_beyond_32bit_offsets_are_rejected_without_gpu(n):
+    """The FP64 iterate [6][3][n^3] must stay below 2^32 bytes for the gathers' 32-bit byte offsets:
+    144 n^3 < 2^32 fails for every solvable n > 256 (the next after 256 is 320 = 5 * 2^6)."""
+    import homo3d_amd
+
+    assert 144 * n ** 3 >= 2 ** 32
+    for solver in ("pcg", "mg"):
+        with pytest.raises(homo3d_amd.Homo3dError) as ei:
+            homo3d_amd.Homogenization(homo3d_amd.HomoConfig(reso=n, solver=solver))
+        assert ei.value.code == homo3d_amd.HOMO3D_NO_SUPPORT
+    assert 144 * 256 ** 3 < 2 ** 32

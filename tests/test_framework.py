@@ -69,3 +69,43 @@ def test_example_opti_npr_mma_and_shear(tmp_path):
                    "-prefix=" + pre2])
     assert rc == 0
     assert np.isfinite(np.loadtxt(pre2 + "objlist")).all()
+
+
+def test_shear_isotropy_terms_follow_reference_scaling():
+    """example_opti_shear_isotropy's constraint (Framework.cu:317-323): the value is
+    1000 * (1000 (A/B - 1)^2) - 0.1, its gradient that of 1000 (A/B - 1)^2, with
+    A = 2 (C33 + C44 + C55), B = C00 + C11 + C22 - C01 - C02 - C12."""
+    lib = ctypes.CDLL(FW)
+    fn = lib.homo3d_shear_isotropy_terms
+    fn.restype = ctypes.c_int
+    rng = np.random.default_rng(5)
+    C = np.diag([5.0, 5.2, 4.9, 1.1, 1.3, 1.2])
+    C[0, 1] = C[1, 0] = 2.0
+    C[0, 2] = C[2, 0] = 2.1
+    C[1, 2] = C[2, 1] = 1.9
+    C += 0.01 * rng.standard_normal((6, 6))
+    Cf = np.ascontiguousarray(C.ravel())
+    obj, g = ctypes.c_double(), ctypes.c_double()
+    dobj = (ctypes.c_float * 36)()
+    dg = (ctypes.c_float * 36)()
+    assert fn(Cf.ctypes.data_as(ctypes.c_void_p), ctypes.byref(obj), dobj, ctypes.byref(g), dg) == 0
+
+    def terms(M):
+        A = 2 * (M[3, 3] + M[4, 4] + M[5, 5])
+        B = M[0, 0] + M[1, 1] + M[2, 2] - M[0, 1] - M[0, 2] - M[1, 2]
+        return -(M[3, 3] + M[4, 4] + M[5, 5]) / 3, 1000.0 * (A / B - 1) ** 2
+
+    o_ref, k_ref = terms(C)
+    assert abs(obj.value - o_ref) < 1e-12
+    assert abs(g.value - (k_ref * 1000.0 - 0.1)) < 1e-9 * max(1.0, abs(k_ref * 1000))
+    # gradients by central differences of the back-propagated expressions, entry by entry
+    for idx in range(36):
+        i, j = divmod(idx, 6)
+        h = 1e-6
+        Cp, Cm = C.copy(), C.copy()
+        Cp[i, j] += h
+        Cm[i, j] -= h
+        do = (terms(Cp)[0] - terms(Cm)[0]) / (2 * h)
+        dk = (terms(Cp)[1] - terms(Cm)[1]) / (2 * h)
+        assert abs(dobj[idx] - do) < 1e-6, (i, j)
+        assert abs(dg[idx] - dk) < 1e-4 * max(1.0, abs(dk)), (i, j, dg[idx], dk)

@@ -1,0 +1,143 @@
+"""GPU tests of the C ABI's reference-shaped entry points beyond the operators themselves:
+MG::v_cycle(w_SOR, pre, post), Sensitivity(..., lexiOrder=false), MMAOptimizer::setBound(float*, float*),
+the divergence status of the refined solve and warm starts after level-0 writes."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+E, NU = 10.0, 0.3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def h3d():
+    import homo3d_amd
+
+    return homo3d_amd
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).max() /
+                 max(np.abs(np.asarray(b, np.float64)).max(), 1e-300))
+
+
+def make(h3d, n, solver="mg", seed=0, vol=0.3, sym="reflect6", **kw):
+    hom = h3d.Homogenization(h3d.HomoConfig(reso=n, solver=solver, coarse_half=False, **kw))
+    rho = h3d.randc_density(n, seed, vol, sym)
+    x = (rho.astype(np.float64) ** 3).astype(np.float32)
+    hom.update(x)
+    return hom, x
+
+
+@pytest.mark.parametrize("w,pre,post", [(1.0, 1, 1), (0.8, 2, 1), (1.0, 0, 2)])
+def test_v_cycle_sweep_counts(h3d, w, pre, post):
+    """homo3d_vcycle_sweeps equals the same cycle composed from the level entry points, each of which
+    has its own oracle parity test (smooth, residual, restrict, coarse_solve, prolong)."""
+    say("v_cycle: n=32 w=%g pre=%d post=%d" % (w, pre, post))
+    hom, _ = make(h3d, 32, seed=1)
+    hom.macro_rhs()
+    zero0 = np.zeros(hom.level_shape(0), np.float32)
+    hom.set_field(0, 0, zero0)
+    hom.v_cycle(w, pre, post)
+    got = hom.get_field(0, 0)
+    # the same cycle from the level operations
+    hom.set_field(0, 0, zero0)
+    nl = len(hom.sizes)
+    for l in range(nl - 1):
+        if l > 0:
+            hom.set_field(l, 0, np.zeros(hom.level_shape(l), np.float32))
+        for _ in range(pre):
+            hom.smooth(l, w)
+        hom.residual(l)
+        hom.restrict(l + 1)
+    hom.coarse_solve()
+    for l in range(nl - 2, -1, -1):
+        hom.prolong(l)
+        for _ in range(post):
+            hom.smooth(l, w)
+    ref = hom.get_field(0, 0)
+    assert rel(got, ref) < 1e-6
+    if (w, pre, post) == (1.0, 1, 1):
+        hom.set_field(0, 0, zero0)
+        hom.vcycle(False)
+        assert rel(hom.get_field(0, 0), got) < 1e-6
+
+
+def test_sensitivity_storage_order(h3d):
+    say("sensitivity order: n=16")
+    hom, _ = make(h3d, 16, solver="pcg")
+    hom.solve(1e-5, warm=False)
+    dC = np.zeros((6, 6), np.float32)
+    dC[0, 1] = dC[1, 0] = 1.0
+    dC[3, 3] = -0.5
+    lex = hom.Sensitivity(dC, lexiOrder=True)
+    blk = hom.Sensitivity(dC, lexiOrder=False)
+    n, h = 16, 8
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    b = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)
+    sidx = b * h ** 3 + (x >> 1) + h * ((y >> 1) + h * (z >> 1))
+    assert np.array_equal(blk[sidx.ravel()], lex)
+
+
+def test_mma_per_variable_bounds(h3d, orc):
+    say("mma bounds: n=64 m=1")
+    n, m = 64, 1
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.3, 0.6, n)
+    lo = rng.uniform(0.01, 0.2, n).astype(np.float32)
+    hi = rng.uniform(0.7, 1.0, n).astype(np.float32)
+    gpu = h3d.MMAOptimizer(m, n, 1.0, 0.0, 1000.0, 1.0)
+    gpu.setBound(lo, hi)
+    st = {"x": x.copy(), "xmin": lo.astype(np.float64), "xmax": hi.astype(np.float64), "xold1": x.copy(),
+          "xold2": x.copy(), "low": np.zeros(n), "upp": np.zeros(n)}
+    w = rng.uniform(0.5, 2.0, (m, n))
+    for it in range(4):
+        xin = st["x"].astype(np.float32).astype(np.float64)
+        st["x"][:] = xin
+        df0 = (-(1.0 + np.sin(3 * xin))).astype(np.float32)
+        g = np.array([(w[0] * xin).sum() / n - 0.4], dtype=np.float32)
+        dg = (w / n).astype(np.float32)
+        xg = gpu.update(it, xin.astype(np.float32), df0, g, list(dg))
+        orc.mma_update(st, it, df0.astype(np.float64), g.astype(np.float64), dg.astype(np.float64), 1.0,
+                       np.zeros(m), np.full(m, 1000.0), np.ones(m), 1.0)
+        x64, low, upp, _ = gpu.state()
+        assert np.abs(x64 - st["x"]).max() <= 1e-9, it
+        assert np.all(xg >= lo) and np.all(xg <= hi)
+
+
+def test_nonfinite_density_reports_divergence(h3d):
+    """A NaN in the density propagates into every residual norm: the refined solve must stop with
+    HOMO3D_DIVERGED instead of running to its iteration limit and returning NOT_CONVERGED."""
+    say("divergence: n=16 NaN density")
+    hom = h3d.Homogenization(h3d.HomoConfig(reso=16, solver="pcg", max_iter=50))
+    x = (h3d.randc_density(16, 0, 0.3, "reflect6").astype(np.float64) ** 3).astype(np.float32)
+    x[123] = np.nan
+    with pytest.raises(h3d.Homo3dError) as ei:
+        hom.update(x)
+        hom.solve(1e-5, warm=False)
+    assert ei.value.code == h3d.HOMO3D_DIVERGED
+
+
+def test_warm_start_after_level0_prolongation(h3d):
+    """A warm refined solve after homo3d_prolong wrote level 0 starts from that field (the handle widens
+    it into its FP64 iterate) rather than from the stale FP64 solution of the previous solve."""
+    say("warm start after prolongation: n=32")
+    hom, _ = make(h3d, 32, solver="pcg", seed=2)
+    hom.solve(1e-5, warm=False)
+    u = hom.get_solution64()
+    st = hom.solve(1e-5, warm=True)
+    assert sum(st["iters"]) == 0  # converged iterate: nothing to do
+    rng = np.random.default_rng(0)
+    hom.set_field(1, 0, rng.standard_normal(hom.level_shape(1)).astype(np.float32))
+    hom.prolong(0)  # u[0] += P u[1]: a level-0 write that is not a field upload
+    st = hom.solve(1e-5, warm=True)
+    assert st["status"] == 0
+    assert sum(st["iters"]) > 0  # it started from the perturbed field
+    assert rel(hom.get_solution64(), u) < 1e-3
