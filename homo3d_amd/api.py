@@ -33,6 +33,12 @@ class _Config(ctypes.Structure):
                 ("fine_mfma", ctypes.c_int)]
 
 
+class _NetConfig(ctypes.Structure):
+    _fields_ = [("grid", ctypes.c_int), ("levels", ctypes.c_int), ("base_channels", ctypes.c_int),
+                ("batch", ctypes.c_int), ("target_scale", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("device", ctypes.c_int)]
+
+
 class _Stats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int * 6), ("relres", ctypes.c_double * 6),
                 ("seconds", ctypes.c_double)]
@@ -89,6 +95,24 @@ EXPORTS = {
                                         ctypes.POINTER(ctypes.c_int)]),
     "homo3d_randc_density": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_void_p]),
+    "homo3d_net_default_config": (None, [ctypes.POINTER(_NetConfig)]),
+    "homo3d_net_create": (ctypes.c_int, [ctypes.POINTER(_NetConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "homo3d_net_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_net_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "homo3d_net_num_params": (ctypes.c_long, [ctypes.c_void_p]),
+    "homo3d_net_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_net_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "homo3d_net_loss_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "homo3d_net_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
+    "homo3d_net_dataset_reserve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_net_dataset_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "homo3d_net_dataset_size": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_net_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                        ctypes.c_void_p]),
+    "homo3d_net_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "homo3d_net_warm_start": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "homo3d_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_profile_only": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "homo3d_profile_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long),
@@ -497,3 +521,113 @@ class MMAOptimizer:
         it = ctypes.c_int()
         self._chk(lib().homo3d_mma_state(self._h, _ptr(x), _ptr(lo), _ptr(up), ctypes.byref(it)), "state")
         return x, lo, up, it.value
+
+
+class WarmStartNet:
+    """The learned warm start (homo3d_net_*): a periodic 3D U-Net trained on the GPU on (density,
+    solution) pairs the solver generates; its prediction seeds the refined MG-PCG solve."""
+
+    def __init__(self, grid=64, levels=4, base_channels=16, batch=4, target_scale=8.0, seed=0, device=0):
+        c = _NetConfig()
+        lib().homo3d_net_default_config(ctypes.byref(c))
+        c.grid, c.levels, c.base_channels, c.batch = int(grid), int(levels), int(base_channels), int(batch)
+        c.target_scale, c.seed, c.device = float(target_scale), int(seed), int(device)
+        self._h = ctypes.c_void_p()
+        rc = lib().homo3d_net_create(ctypes.byref(c), ctypes.byref(self._h))
+        if rc:
+            raise Homo3dError(rc, "homo3d_net_create(grid=%d, levels=%d)" % (grid, levels))
+        self.grid, self.levels, self.base_channels, self.batch = int(grid), int(levels), int(base_channels), int(batch)
+        self.target_scale = float(target_scale)
+        self.num_params = int(lib().homo3d_net_num_params(self._h))
+
+    def close(self):
+        if self._h:
+            lib().homo3d_net_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise Homo3dError(rc, "%s: %s" % (what, lib().homo3d_net_last_error(self._h).decode()))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self):
+        out = np.zeros(self.num_params, np.float32)
+        self._chk(lib().homo3d_net_params(self._h, _ptr(out), 0), "params")
+        return out
+
+    def set_params(self, theta):
+        a = np.ascontiguousarray(theta, dtype=np.float32).ravel()
+        if a.size != self.num_params:
+            raise ValueError("expected %d parameters, got %d" % (self.num_params, a.size))
+        self._chk(lib().homo3d_net_params(self._h, _ptr(a), 1), "params")
+
+    def forward(self, rho):
+        r = np.ascontiguousarray(rho, dtype=np.float32).reshape(-1, self.grid ** 3)
+        out = np.zeros((r.shape[0], self.grid ** 3, 18), np.float32)
+        self._chk(lib().homo3d_net_forward(self._h, _ptr(r), r.shape[0], _ptr(out)), "forward")
+        return out
+
+    def loss_grad(self, rho, target):
+        r = np.ascontiguousarray(rho, dtype=np.float32).reshape(-1, self.grid ** 3)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(r.shape[0], self.grid ** 3, 18)
+        g = np.zeros(self.num_params, np.float32)
+        loss = ctypes.c_double()
+        self._chk(lib().homo3d_net_loss_grad(self._h, _ptr(r), _ptr(t), r.shape[0], ctypes.byref(loss), _ptr(g)),
+                  "loss_grad")
+        return loss.value, g
+
+    def train_step(self, rho, target, lr):
+        r = np.ascontiguousarray(rho, dtype=np.float32).reshape(-1, self.grid ** 3)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(r.shape[0], self.grid ** 3, 18)
+        loss = ctypes.c_double()
+        self._chk(lib().homo3d_net_train_step(self._h, _ptr(r), _ptr(t), r.shape[0], float(lr), ctypes.byref(loss)),
+                  "train_step")
+        return loss.value
+
+    def dataset_reserve(self, count):
+        self._chk(lib().homo3d_net_dataset_reserve(self._h, int(count)), "dataset_reserve")
+
+    def dataset_add(self, hom):
+        """Append the (density, solution) pair of a solved Homogenization."""
+        self._chk(lib().homo3d_net_dataset_add(self._h, hom.handle), "dataset_add")
+
+    def dataset_size(self):
+        return int(lib().homo3d_net_dataset_size(self._h))
+
+    def train(self, epochs, lr, seed=0):
+        losses = np.zeros(max(int(epochs), 1))
+        self._chk(lib().homo3d_net_train(self._h, int(epochs), float(lr), int(seed), _ptr(losses)), "train")
+        return losses[:int(epochs)]
+
+    def eval(self, first, count):
+        loss = ctypes.c_double()
+        self._chk(lib().homo3d_net_eval(self._h, int(first), int(count), ctypes.byref(loss)), "eval")
+        return loss.value
+
+    def warm_start(self, hom):
+        """Predict hom's solution from its current density; the next hom.solve(warm=True) starts there."""
+        rc = lib().homo3d_net_warm_start(hom.handle, self._h)
+        if rc:
+            raise Homo3dError(rc, "net_warm_start: %s" % lib().homo3d_last_error(hom.handle).decode())
+
+    def save(self, path):
+        """Parameters and configuration as an .npz (no pickled objects)."""
+        np.savez(path, params=self.get_params(), grid=self.grid, levels=self.levels,
+                 base_channels=self.base_channels, target_scale=self.target_scale)
+
+    @classmethod
+    def load(cls, path, batch=1, device=0):
+        d = np.load(path, allow_pickle=False)
+        net = cls(grid=int(d["grid"]), levels=int(d["levels"]), base_channels=int(d["base_channels"]), batch=batch,
+                  target_scale=float(d["target_scale"]), device=device)
+        net.set_params(d["params"])
+        return net

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhomo3d.so")
-SOURCES = ["kernels.hip", "api.cpp", "mma.cpp", "density.cpp"]
+SOURCES = ["kernels.hip", "net.hip", "api.cpp", "mma.cpp", "density.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [

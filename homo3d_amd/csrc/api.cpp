@@ -16,6 +16,7 @@
 
 #include "../../include/homo3d.h"
 #include "kernels.h"
+#include "net.h"
 
 using namespace homo3d;
 
@@ -1473,6 +1474,43 @@ int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch,
     HIPCHK(h, hipMemcpyAsync(rho, dn, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                              h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+// Learned warm start: the network's side is net.hip; these two entry points read or write the handle.
+int homo3d_net_dataset_add(homo3d_net* net, homo3d_handle* h) {
+    if (!net || !h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return net_set_error(net, rc, h->err.c_str());
+    if (h->cfg.solver != HOMO3D_SOLVER_PCG)
+        return net_set_error(net, HOMO3D_BAD_ARG, "training pairs come from the refined solver (HOMO3D_SOLVER_PCG)");
+    const int n = h->lv[0].n, g = net_grid(net);
+    if (n % g) return net_set_error(net, HOMO3D_NO_SUPPORT, "solve grid is not a multiple of the network grid");
+    uint16_t *in = nullptr, *tg = nullptr;
+    rc = net_dataset_push(net, &in, &tg);
+    if (rc) return rc;
+    // the pair is taken on the handle's stream (after its solve), then both streams are in step
+    launch_net_input(h->stream, n, g, h->d_x, in);
+    launch_net_target(h->stream, n, g, h->U64, net_target_scale(net), h->d_tmp, tg);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_net_warm_start(homo3d_handle* h, homo3d_net* net) {
+    if (!net || !h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    const int n = h->lv[0].n, g = net_grid(net);
+    if (n % g) return set_err(h, HOMO3D_NO_SUPPORT, "solve grid %d is not a multiple of the network grid %d", n, g);
+    hipStream_t ns = net_stream(net);
+    HIPCHK(h, hipStreamSynchronize(h->stream));  // the density update has landed
+    launch_net_input(ns, n, g, h->d_x, net_input_slot(net));
+    const float* P = nullptr;
+    rc = net_forward_one(net, &P);
+    if (rc) return set_err(h, rc, "network forward failed: %s", homo3d_net_last_error(net));
+    launch_net_warm_start(ns, n, g, P, net_pred_ld(net), net_target_scale(net), h->U64, h->X);
+    HIPCHK(h, hipStreamSynchronize(ns));
+    h->x_dirty = false;  // U64 holds the start; X is its FP32 copy
     return HOMO3D_OK;
 }
 

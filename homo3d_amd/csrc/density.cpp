@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "../../include/homo3d.h"
@@ -66,6 +67,19 @@ inline long gapply(const std::pair<int, int>& g, int n, const int p[3]) {
     return q[0] + (long)n * (q[1] + (long)n * q[2]);
 }
 
+// Run body(begin, end) over [0, count) split into contiguous ranges on up to 16 host threads.
+template <typename F>
+void parallel_ranges(long count, const F& body) {
+    const long nt = std::max(1L, std::min<long>(16, std::min<long>((long)std::thread::hardware_concurrency(), count / 4096)));
+    if (nt <= 1) {
+        body(0L, count);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (long t = 0; t < nt; t++) th.emplace_back([&, t] { body(count * t / nt, count * (t + 1) / nt); });
+    for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 extern "C" int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, float* out) {
@@ -93,8 +107,8 @@ extern "C" int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, f
         }
     const long N = (long)n * n * n;
     std::vector<float> rho1(N), rho2(N);
-#pragma omp parallel for schedule(static)
-    for (int z = 0; z < n; z++) {
+    parallel_ranges((long)n * 4096, [&](long zb, long ze) {
+    for (int z = (int)(zb / 4096); z < (int)(ze / 4096); z++) {
         double Cz[4][4][2][2];
         for (int a = 0; a < 4; a++)
             for (int b = 0; b < 4; b++)
@@ -125,35 +139,49 @@ extern "C" int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, f
             }
         }
     }
+    });
     std::vector<std::pair<int, int>> g;
     group(sym, g);
     if (g.empty()) {
         rho2 = rho1;
     } else {
-#pragma omp parallel for schedule(static)
-        for (long e = 0; e < N; e++) {
-            const int p[3] = {(int)(e % n), (int)(e / n % n), (int)(e / ((long)n * n))};
-            long best = e;
-            for (auto& ge : g) best = std::min(best, gapply(ge, n, p));
-            const int q[3] = {(int)(best % n), (int)(best / n % n), (int)(best / ((long)n * n))};
-            double s = 0;
-            for (auto& ge : g) s += (double)rho1[gapply(ge, n, q)];
-            rho2[e] = (float)(s / (double)g.size());
-        }
+        parallel_ranges(N, [&](long eb, long ee) {
+            for (long e = eb; e < ee; e++) {
+                const int p[3] = {(int)(e % n), (int)(e / n % n), (int)(e / ((long)n * n))};
+                long best = e;
+                for (auto& ge : g) best = std::min(best, gapply(ge, n, p));
+                const int q[3] = {(int)(best % n), (int)(best / n % n), (int)(best / ((long)n * n))};
+                double s = 0;
+                for (auto& ge : g) s += (double)rho1[gapply(ge, n, q)];
+                rho2[e] = (float)(s / (double)g.size());
+            }
+        });
     }
+    // mean(clamp(s rho2)) over the cells for each bisection step: 64 fixed ranges summed in order (the
+    // same result for any thread count)
+    constexpr int kParts = 64;
     double lo = 0, hi = 1e3;
     for (int it = 0; it < 100; it++) {
         const double sc = 0.5 * (lo + hi);
+        double part[kParts];
+        parallel_ranges(kParts * 4096L, [&](long pb, long pe) {
+            for (long k = pb / 4096; k < pe / 4096; k++) {
+                double s = 0;
+                for (long e = N * k / kParts; e < N * (k + 1) / kParts; e++)
+                    s += std::min(1.0, std::max(1e-3, sc * (double)rho2[e]));
+                part[k] = s;
+            }
+        });
         double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-        for (long e = 0; e < N; e++) s += std::min(1.0, std::max(1e-3, sc * (double)rho2[e]));
+        for (int k = 0; k < kParts; k++) s += part[k];
         if (s / (double)N > vol)
             hi = sc;
         else
             lo = sc;
     }
     const double sc = 0.5 * (lo + hi);
-#pragma omp parallel for schedule(static)
-    for (long e = 0; e < N; e++) out[e] = (float)std::min(1.0, std::max(1e-3, sc * (double)rho2[e]));
+    parallel_ranges(N, [&](long eb, long ee) {
+        for (long e = eb; e < ee; e++) out[e] = (float)std::min(1.0, std::max(1e-3, sc * (double)rho2[e]));
+    });
     return HOMO3D_OK;
 }

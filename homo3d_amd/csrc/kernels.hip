@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "net.h"
 
 namespace homo3d {
 
@@ -2922,6 +2923,112 @@ void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* parti
 void launch_set_weights(hipStream_t s, int n, int pitch, double emin, const float* x, float* w, float* xc) {
     const size_t N = (size_t)n * n * n;
     hipLaunchKernelGGL(k_set_weights, dim3(nblk(N)), dim3(256), 0, s, make_lay(n, 0), pitch, emin, x, w, xc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Bridges between the solver's fields and the learned warm-start network (net.hip): the network runs
+// on a g^3 grid (g | n) with bf16 channels-last tensors.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// Network input: channel 0 of cell (i, j, k) = mean of rho = x^(1/3) over the f^3 fine cells it covers
+// (f = n / g), channels 1..7 = 0.  x holds the raw x = rho^3 per cell in level storage order.
+__global__ __launch_bounds__(256) void k_net_input(Lay L, int g, const float* __restrict__ x, uint16_t* __restrict__ in) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long g3 = (long)g * g * g;
+    if (v >= g3) return;
+    const int f = L.n / g;
+    const int i = (int)(v % g), j = (int)((v / g) % g), k = (int)(v / ((long)g * g));
+    float s = 0.f;
+    for (int c = 0; c < f * f * f; c++) {
+        const int xx = i * f + c % f, yy = j * f + (c / f) % f, zz = k * f + c / (f * f);
+        s += cbrtf(fmaxf(x[L.idx(xx, yy, zz)], 0.f));
+    }
+    *reinterpret_cast<uint4*>(in + v * 8) = make_uint4(f2bf(s / (float)(f * f * f)), 0u, 0u, 0u);
+}
+
+// Per-channel mean of the scaled solution sampled at the coarse vertices (i f, j f, k f).
+__global__ __launch_bounds__(256) void k_net_target_mean(Lay L, int g, const double* __restrict__ u, double* __restrict__ mean) {
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
+    const long g3 = (long)g * g * g;
+    const size_t N = (size_t)L.n * L.n * L.n;
+    const int f = L.n / g;
+    double s = 0.0;
+    for (long v = threadIdx.x; v < g3; v += blockDim.x) {
+        const int i = (int)(v % g), j = (int)((v / g) % g), k = (int)(v / ((long)g * g));
+        s += u[c * N + L.idx(i * f, j * f, k * f)];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int t = 128; t > 0; t >>= 1) {
+        if (threadIdx.x < t) sh[threadIdx.x] += sh[threadIdx.x + t];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mean[c] = sh[0] / (double)g3;
+}
+
+// Network target: channel 3 s + k of coarse vertex v = scale (u_sk(i f, j f, k f) - mean) / n.
+__global__ __launch_bounds__(256) void k_net_target(Lay L, int g, const double* __restrict__ u,
+                                                    const double* __restrict__ mean, double scale,
+                                                    uint16_t* __restrict__ tg) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long g3 = (long)g * g * g;
+    if (v >= g3) return;
+    const size_t N = (size_t)L.n * L.n * L.n;
+    const int f = L.n / g;
+    const int i = (int)(v % g), j = (int)((v / g) % g), k = (int)(v / ((long)g * g));
+    const uint32_t e = L.idx(i * f, j * f, k * f);
+    const double sc = scale / (double)L.n;
+    for (int c = 0; c < 18; c++) tg[v * 18 + c] = f2bf((float)(sc * (u[c * N + e] - mean[c])));
+}
+
+// Warm start from the prediction P ([g^3][ldp], channel 3 s + k = scale u_sk / n): periodic trilinear
+// interpolation to the n^3 vertices, minus the value at vertex 0 (the pinned vertex keeps u = 0),
+// times n / scale, into the FP64 iterate U64 and its FP32 copy X (level storage order).
+__global__ __launch_bounds__(256) void k_net_warm_start(Lay L, int g, const float* __restrict__ P, int ldp, double inv,
+                                                        double* __restrict__ U64, float* __restrict__ X) {
+    const size_t N = (size_t)L.n * L.n * L.n;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    int x, y, z;
+    L.pos((uint32_t)t, x, y, z);
+    const int f = L.n / g;
+    const int i0 = x / f, j0 = y / f, k0 = z / f;
+    const float tx = (float)(x - i0 * f) / f, ty = (float)(y - j0 * f) / f, tz = (float)(z - k0 * f) / f;
+    long cv[8];
+    float wt[8];
+    for (int c = 0; c < 8; c++) {
+        const int i = (i0 + (c & 1)) % g, j = (j0 + ((c >> 1) & 1)) % g, k = (k0 + (c >> 2)) % g;
+        cv[c] = i + (long)g * (j + (long)g * k);
+        wt[c] = ((c & 1) ? tx : 1.f - tx) * (((c >> 1) & 1) ? ty : 1.f - ty) * ((c >> 2) ? tz : 1.f - tz);
+    }
+    for (int ch = 0; ch < 18; ch++) {
+        float s = 0.f;
+        for (int c = 0; c < 8; c++) s += wt[c] * P[cv[c] * ldp + ch];
+        const double u = ((double)s - (double)P[ch]) * inv;
+        U64[ch * N + t] = u;
+        X[ch * N + t] = (float)u;
+    }
+}
+
+}  // namespace
+
+void launch_net_input(hipStream_t s, int n, int g, const float* x, uint16_t* in) {
+    const long g3 = (long)g * g * g;
+    hipLaunchKernelGGL(k_net_input, dim3(nblk(g3)), dim3(256), 0, s, make_lay(n, 0), g, x, in);
+}
+
+void launch_net_target(hipStream_t s, int n, int g, const double* u64, double scale, double* mean18, uint16_t* tg) {
+    const long g3 = (long)g * g * g;
+    hipLaunchKernelGGL(k_net_target_mean, dim3(18), dim3(256), 0, s, make_lay(n, 0), g, u64, mean18);
+    hipLaunchKernelGGL(k_net_target, dim3(nblk(g3)), dim3(256), 0, s, make_lay(n, 0), g, u64, mean18, scale, tg);
+}
+
+void launch_net_warm_start(hipStream_t s, int n, int g, const float* P, int ldp, double scale, double* u64, float* x) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_net_warm_start, dim3(nblk(N)), dim3(256), 0, s, make_lay(n, 0), g, P, ldp, (double)n / scale,
+                       u64, x);
 }
 
 void launch_copy_pitched(hipStream_t s, int n, int pitch_in, int pitch_out, const float* in, float* out) {

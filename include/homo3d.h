@@ -184,6 +184,51 @@ int homo3d_mma_state(homo3d_mma* h, double* x, double* low, double* upp, int* ne
 /* Seeded randc density (project generator), written to host memory, n^3 floats, pitch n. */
 int homo3d_randc_density(int n, uint64_t seed, double vol, int sym, float* out);
 
+/* Learned warm start (no counterpart in the reference, whose warm start is the previous design's stored
+ * solutions, homogenization.cu:471-477 / grid.cpp:459-462).  A periodic 3D U-Net (levels, channels
+ * base_channels * 2^l, bf16 weights and activations, FP32 accumulation on the matrix cores) maps the
+ * density on a grid^3 grid to the six load cases' displacements, trained on the GPU on (density, solution)
+ * pairs the solver generates.  The prediction, upsampled to the solve grid, becomes the FP64 iterate of
+ * the next warm refined solve, which stays the solver and the judge of its error.
+ * Parameter layout (FP32): per layer W[cout][taps][cin] then b[cout], in the order enc0.c1, enc0.c2, ...,
+ * enc{L-1}.c2, dec{L-2}.c1, dec{L-2}.c2, ..., dec0.c2, head (1x1, 18 outputs). */
+typedef struct homo3d_net homo3d_net;
+typedef struct {
+    int grid;            /* network grid g (power of two, g >= 8 * 2^(levels-1)); default 64 */
+    int levels;          /* U-Net levels (4) */
+    int base_channels;   /* channels of level 0 (16; multiple of 8) */
+    int batch;           /* largest batch of a forward or training step (4) */
+    double target_scale; /* the network predicts target_scale * u / n, mean-free per channel (8) */
+    uint64_t seed;       /* initial weights (He-normal) */
+    int device;
+} homo3d_net_config;
+void homo3d_net_default_config(homo3d_net_config* cfg);
+int homo3d_net_create(const homo3d_net_config* cfg, homo3d_net** out);
+int homo3d_net_destroy(homo3d_net* net);
+const char* homo3d_net_last_error(const homo3d_net* net);
+long homo3d_net_num_params(const homo3d_net* net);
+/* FP32 master parameters to (to_device = 1; resets the Adam state) or from the network. */
+int homo3d_net_params(homo3d_net* net, float* host, int to_device);
+/* rho: batch x grid^3 densities (host, x fastest); out: batch x grid^3 x 18 predictions (host). */
+int homo3d_net_forward(homo3d_net* net, const float* rho, int batch, float* out);
+/* Loss (mean over batch, voxels and channels of (P - mean_v P - target)^2, target mean-free) and its
+ * gradient (num_params floats, may be NULL); target: batch x grid^3 x 18 (host). */
+int homo3d_net_loss_grad(homo3d_net* net, const float* rho, const float* target, int batch, double* loss, float* grad);
+/* One Adam step (lr, betas 0.9 / 0.999) on a host batch. */
+int homo3d_net_train_step(homo3d_net* net, const float* rho, const float* target, int batch, double lr, double* loss);
+/* Device-resident training set: reserve room for `count` pairs; homo3d_net_dataset_add appends the pair of a
+ * solved handle (its density pooled to the network grid and its FP64 solution sampled there; n % grid == 0). */
+int homo3d_net_dataset_reserve(homo3d_net* net, int count);
+int homo3d_net_dataset_add(homo3d_net* net, homo3d_handle* h);
+int homo3d_net_dataset_size(const homo3d_net* net);
+/* Adam epochs over the training set in seeded random order; losses[e] = mean batch loss of epoch e. */
+int homo3d_net_train(homo3d_net* net, int epochs, double lr, uint64_t seed, double* losses);
+/* Mean loss over pairs [first, first + count) of the training set, without updating. */
+int homo3d_net_eval(homo3d_net* net, int first, int count, double* loss);
+/* Predict the solution of the handle's current density and make it the FP64 iterate that the next warm
+ * solve (homo3d_solve with warm = 1) starts from.  n % grid == 0. */
+int homo3d_net_warm_start(homo3d_handle* h, homo3d_net* net);
+
 /* Kernel timing: enable event pairs around the finest-level kernel launches, then read totals per
  * kernel: "fine_gs" (full GS colour passes), "fine_gs_zero" (colour passes of sweeps that start from
  * zero), "fine_apply" (the CG operator pass: q = K z + beta q_old with the fused p, x updates and p.q;

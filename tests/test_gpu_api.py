@@ -131,7 +131,7 @@ def test_warm_start_after_level0_prolongation(h3d):
     say("warm start after prolongation: n=32")
     hom, _ = make(h3d, 32, solver="pcg", seed=2)
     hom.solve(1e-5, warm=False)
-    u = hom.get_solution64()
+    C0 = hom.elasticMatrix(solve=False)
     st = hom.solve(1e-5, warm=True)
     assert sum(st["iters"]) == 0  # converged iterate: nothing to do
     rng = np.random.default_rng(0)
@@ -140,4 +140,5 @@ def test_warm_start_after_level0_prolongation(h3d):
     st = hom.solve(1e-5, warm=True)
     assert st["status"] == 0
     assert sum(st["iters"]) > 0  # it started from the perturbed field
-    assert rel(hom.get_solution64(), u) < 1e-3
+    # u itself is barely determined in the near-void cells (stiffness ~1e-9), C^H is not
+    assert rel(hom.elasticMatrix(solve=False), C0) < 1e-4
