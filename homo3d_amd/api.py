@@ -54,6 +54,7 @@ EXPORTS = {
     "homo3d_update_density": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "homo3d_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_Stats)]),
     "homo3d_elastic_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "homo3d_residual_norms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_sensitivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                           ctypes.c_int, ctypes.c_int]),
     "homo3d_sensitivity_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
@@ -109,8 +110,8 @@ EXPORTS = {
     "homo3d_net_dataset_reserve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_net_dataset_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "homo3d_net_dataset_size": (ctypes.c_int, [ctypes.c_void_p]),
-    "homo3d_net_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
-                                        ctypes.c_void_p]),
+    "homo3d_net_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_uint64, ctypes.c_void_p]),
     "homo3d_net_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_net_warm_start": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "homo3d_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -338,6 +339,12 @@ class Homogenization:
         C = (ctypes.c_double * 36)()
         self._chk(lib().homo3d_elastic_matrix(self._h, C), "elasticMatrix")
         return np.array(C[:]).reshape(6, 6)
+
+    def residual_norms(self):
+        """FP64 relres per load case of the current FP64 iterate (no solve)."""
+        r = (ctypes.c_double * 6)()
+        self._chk(lib().homo3d_residual_norms(self._h, r), "residual_norms")
+        return list(r)
 
     def Sensitivity(self, dC, pitchT=-1, lexiOrder=True):
         """Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder): returns dJ/dx per cell, lexicographic
@@ -603,9 +610,11 @@ class WarmStartNet:
     def dataset_size(self):
         return int(lib().homo3d_net_dataset_size(self._h))
 
-    def train(self, epochs, lr, seed=0):
+    def train(self, epochs, lr, seed=0, count=0):
+        """Adam epochs over the first `count` training pairs (all when 0); returns the epoch losses."""
         losses = np.zeros(max(int(epochs), 1))
-        self._chk(lib().homo3d_net_train(self._h, int(epochs), float(lr), int(seed), _ptr(losses)), "train")
+        self._chk(lib().homo3d_net_train(self._h, int(epochs), int(count), float(lr), int(seed), _ptr(losses)),
+                  "train")
         return losses[:int(epochs)]
 
     def eval(self, first, count):

@@ -990,6 +990,7 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
         }
         h->x_dirty = false;
         double rel[kRhs];
+        bool warm_checked = false;
         for (int outer = 0;; outer++) {
             double rn2[kRhs];
             launch_fine_residual64(h->stream, L0.n, h->d_ke64, h->d_kchi64, h->d_x, h->cfg.emin, h->U64, h->Rr,
@@ -997,6 +998,21 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             HIPCHK(h, hipMemcpyAsync(rn2, h->d_tmp, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(h, hipStreamSynchronize(h->stream));
             if (outer == 0 && u_zero) std::copy(rn2, rn2 + kRhs, fn2);
+            // A warm start whose FP64 residual is not below ||f|| is worse than none: that load case
+            // restarts from zero (the residual is formed once more with its slice of U64 cleared).
+            if (outer == 0 && warm && !warm_checked) {
+                warm_checked = true;
+                bool reset = false;
+                for (int r = 0; r < kRhs; r++)
+                    if (fn2[r] > 0 && !(rn2[r] < fn2[r])) {
+                        HIPCHK(h, hipMemsetAsync(h->U64 + r * M, 0, sizeof(double) * M, h->stream));
+                        reset = true;
+                    }
+                if (reset) {
+                    outer = -1;
+                    continue;
+                }
+            }
             bool all = true, finite = true;
             for (int r = 0; r < kRhs; r++) {
                 rel[r] = fn2[r] > 0 ? std::sqrt(rn2[r] / fn2[r]) : 0.0;
@@ -1063,6 +1079,29 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
     }
     if (status == HOMO3D_DIVERGED) set_err(h, status, "MG numerical explode");
     return status;
+}
+
+// FP64 relres ||f - K u|| / ||f|| per load case of the current FP64 iterate (the caller's level-0 u when
+// that was written last), without solving: the measure a warm start is judged by.
+int homo3d_residual_norms(homo3d_handle* h, double relres[6]) {
+    if (!h || !relres) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    if (h->cfg.solver != HOMO3D_SOLVER_PCG) return set_err(h, HOMO3D_BAD_ARG, "needs the refined solver's FP64 iterate");
+    const Level& L0 = h->lv[0];
+    const size_t M6 = (size_t)kRhs * 3 * L0.N;
+    double fn2[kRhs], rn2[kRhs];
+    launch_rhs_norm64(h->stream, L0.n, h->d_kchi64, h->d_x, h->cfg.emin, h->d_partial, h->d_tmp);
+    HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
+    if (h->x_dirty) launch_widen_f64(h->stream, M6, h->X, h->U64);
+    launch_zero_pin64(h->stream, L0.N, h->U64);
+    h->x_dirty = false;
+    launch_fine_residual64(h->stream, L0.n, h->d_ke64, h->d_kchi64, h->d_x, h->cfg.emin, h->U64, h->Rr, h->d_partial,
+                           h->d_tmp, 1);
+    HIPCHK(h, hipMemcpyAsync(rn2, h->d_tmp, sizeof(rn2), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int r = 0; r < kRhs; r++) relres[r] = fn2[r] > 0 ? std::sqrt(rn2[r] / fn2[r]) : 0.0;
+    return HOMO3D_OK;
 }
 
 // C^H from the current solutions (Homogenization::elasticMatrix, homogenization.cu:478-575).
@@ -1491,7 +1530,7 @@ int homo3d_net_dataset_add(homo3d_net* net, homo3d_handle* h) {
     if (rc) return rc;
     // the pair is taken on the handle's stream (after its solve), then both streams are in step
     launch_net_input(h->stream, n, g, h->d_x, in);
-    launch_net_target(h->stream, n, g, h->U64, net_target_scale(net), h->d_tmp, tg);
+    launch_net_target(h->stream, n, g, h->U64, in, net_target_scale(net), h->d_tmp, tg);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }

@@ -151,10 +151,11 @@ void launch_f32_to_f64(hipStream_t s, long n, const float* a, double* b);
 void launch_f64_to_f32(hipStream_t s, long n, const double* a, float* b);
 // Bridges to the learned warm-start network (g | n; network tensors bf16 channels-last, see net.hip):
 // input channel 0 = mean of x^(1/3) over each coarse cell; target = scale (u - mean) / n sampled at the
-// coarse vertices (mean18: 18 doubles of device scratch); warm start = periodic trilinear upsampling of
+// coarse vertices, the mean weighted by the input's rho^3 (mean18: 18 doubles of device scratch); warm start = periodic trilinear upsampling of
 // the prediction P ([g^3][ldp]) minus its value at vertex 0, times n / scale, into U64 and x.
 void launch_net_input(hipStream_t s, int n, int g, const float* x, uint16_t* in);
-void launch_net_target(hipStream_t s, int n, int g, const double* u64, double scale, double* mean18, uint16_t* tg);
+void launch_net_target(hipStream_t s, int n, int g, const double* u64, const uint16_t* in, double scale, double* mean18,
+                       uint16_t* tg);
 void launch_net_warm_start(hipStream_t s, int n, int g, const float* P, int ldp, double scale, double* u64, float* x);
 
 }  // namespace homo3d

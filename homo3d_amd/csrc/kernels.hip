@@ -14,8 +14,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
+#include "../../include/homo3d.h"
 #include "kernels.h"
 #include "net.h"
 
@@ -2195,8 +2197,21 @@ __global__ __launch_bounds__(256) void k_oc_filter(int n, int pitch, int h, int 
     out[me] = (float)(sum / (ws * (double)rho[me]));
 }
 
-// OC trial update for multiplier g (update_kernel, OCOptimizer.cu:12-34), FP64 arithmetic rounded
-// to FP32; per-block FP64 partial sums of the new density.
+// OC update of one cell for multiplier g (update_kernel, OCOptimizer.cu:12-34), FP64 arithmetic rounded
+// to FP32.
+__device__ __forceinline__ float oc_cell(double r0, double sv, double g, double min_rho, double step, double damp) {
+    double B = -sv / g;
+    if (B < 0) B = 0.01;
+    double xv = (damp == 0.5 ? sqrt(B) : pow(B, damp)) * r0;
+    if (xv - r0 < -step) xv = r0 - step;
+    if (xv - r0 > step) xv = r0 + step;
+    if (xv < min_rho) xv = min_rho;
+    if (xv > 1.0) xv = 1.0;
+    return (float)xv;
+}
+
+// OC trial for multiplier g: per-block FP64 partial sums of the new density; the new density is written
+// to out unless out is null (sum only).
 __global__ __launch_bounds__(256) void k_oc_trial(int n, int pitch, double g, double min_rho, double step,
                                                   double damp, const float* __restrict__ sens,
                                                   const float* __restrict__ rho, float* __restrict__ out,
@@ -2207,20 +2222,23 @@ __global__ __launch_bounds__(256) void k_oc_trial(int n, int pitch, double g, do
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
         const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
         const size_t id = x + ((size_t)y + (size_t)z * n) * pitch;
-        const double r0 = rho[id];
-        double B = -(double)sens[id] / g;
-        if (B < 0) B = 0.01;
-        double xv = (damp == 0.5 ? sqrt(B) : pow(B, damp)) * r0;
-        if (xv - r0 < -step) xv = r0 - step;
-        if (xv - r0 > step) xv = r0 + step;
-        if (xv < min_rho) xv = min_rho;
-        if (xv > 1.0) xv = 1.0;
-        const float o = (float)xv;
-        out[id] = o;
+        const float o = oc_cell(rho[id], sens[id], g, min_rho, step, damp);
+        if (out) out[id] = o;
         s += (double)o;
     }
     s = block_sum<256>(s, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// The OC update for the final multiplier, in place (rho is read and written by the same thread).
+__global__ __launch_bounds__(256) void k_oc_apply(int n, int pitch, double g, double min_rho, double step, double damp,
+                                                  const float* __restrict__ sens, float* rho) {
+    const long N = (long)n * n * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+        const size_t id = x + ((size_t)y + (size_t)z * n) * pitch;
+        rho[id] = oc_cell(rho[id], sens[id], g, min_rho, step, damp);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_absmax(long N, int n, int pitch, const float* __restrict__ a,
@@ -2947,25 +2965,33 @@ __global__ __launch_bounds__(256) void k_net_input(Lay L, int g, const float* __
     *reinterpret_cast<uint4*>(in + v * 8) = make_uint4(f2bf(s / (float)(f * f * f)), 0u, 0u, 0u);
 }
 
-// Per-channel mean of the scaled solution sampled at the coarse vertices (i f, j f, k f).
-__global__ __launch_bounds__(256) void k_net_target_mean(Lay L, int g, const double* __restrict__ u, double* __restrict__ mean) {
-    __shared__ double sh[256];
+// Per-channel mean of the solution sampled at the coarse vertices (i f, j f, k f), weighted by the
+// stiffness rho^3 of the network input at the same index (the network's loss weights).
+__global__ __launch_bounds__(256) void k_net_target_mean(Lay L, int g, const double* __restrict__ u,
+                                                         const uint16_t* __restrict__ in, double* __restrict__ mean) {
+    __shared__ double sh[2][256];
     const int c = blockIdx.x;
     const long g3 = (long)g * g * g;
     const size_t N = (size_t)L.n * L.n * L.n;
     const int f = L.n / g;
-    double s = 0.0;
+    double s = 0.0, ws = 0.0;
     for (long v = threadIdx.x; v < g3; v += blockDim.x) {
         const int i = (int)(v % g), j = (int)((v / g) % g), k = (int)(v / ((long)g * g));
-        s += u[c * N + L.idx(i * f, j * f, k * f)];
+        const double r = bf2f(in[v * 8]);
+        s += r * r * r * u[c * N + L.idx(i * f, j * f, k * f)];
+        ws += r * r * r;
     }
-    sh[threadIdx.x] = s;
+    sh[0][threadIdx.x] = s;
+    sh[1][threadIdx.x] = ws;
     __syncthreads();
     for (int t = 128; t > 0; t >>= 1) {
-        if (threadIdx.x < t) sh[threadIdx.x] += sh[threadIdx.x + t];
+        if (threadIdx.x < t) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + t];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + t];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) mean[c] = sh[0] / (double)g3;
+    if (threadIdx.x == 0) mean[c] = sh[1][0] > 0 ? sh[0][0] / sh[1][0] : 0.0;
 }
 
 // Network target: channel 3 s + k of coarse vertex v = scale (u_sk(i f, j f, k f) - mean) / n.
@@ -3019,9 +3045,10 @@ void launch_net_input(hipStream_t s, int n, int g, const float* x, uint16_t* in)
     hipLaunchKernelGGL(k_net_input, dim3(nblk(g3)), dim3(256), 0, s, make_lay(n, 0), g, x, in);
 }
 
-void launch_net_target(hipStream_t s, int n, int g, const double* u64, double scale, double* mean18, uint16_t* tg) {
+void launch_net_target(hipStream_t s, int n, int g, const double* u64, const uint16_t* in, double scale, double* mean18,
+                       uint16_t* tg) {
     const long g3 = (long)g * g * g;
-    hipLaunchKernelGGL(k_net_target_mean, dim3(18), dim3(256), 0, s, make_lay(n, 0), g, u64, mean18);
+    hipLaunchKernelGGL(k_net_target_mean, dim3(18), dim3(256), 0, s, make_lay(n, 0), g, u64, in, mean18);
     hipLaunchKernelGGL(k_net_target, dim3(nblk(g3)), dim3(256), 0, s, make_lay(n, 0), g, u64, mean18, scale, tg);
 }
 
@@ -3342,3 +3369,4 @@ void launch_f64_to_f32(hipStream_t s, long n, const double* a, float* b) {
 }
 
 }  // namespace homo3d
+

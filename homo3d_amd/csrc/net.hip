@@ -325,24 +325,46 @@ __global__ __launch_bounds__(256) void k_skip_grad(const uint16_t* __restrict__ 
     out[i] = f2bf(bf2f(dcat[v * ld + cu + c]) + 0.125f * bf2f(din[vc * C + c]));
 }
 
-// Per (sample, channel) mean of the prediction P [v][kPld] over the sample's g^3 voxels.
-__global__ __launch_bounds__(256) void k_chan_mean(const float* __restrict__ P, long nsv, float* __restrict__ mean) {
-    __shared__ float sh[256];
-    const int b = blockIdx.x, c = blockIdx.y;
-    float s = 0.f;
-    for (long v = threadIdx.x; v < nsv; v += blockDim.x) s += P[((long)b * nsv + v) * kPld + c];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) mean[b * kOut + c] = sh[0] / (float)nsv;
+// Stiffness weight of voxel v: w = rho^3 of the network input (the cell stiffness of the SIMP law).
+__device__ __forceinline__ float net_weight(const uint16_t* in, long v) {
+    const float r = bf2f(in[v * kInCh]);
+    return r * r * r;
 }
 
-// Loss terms: d = P - mean - T; dP = 2 d / count (bf16, [v][kDpLd], channels >= 18 stay 0); per-block
-// FP64 partial sums of d^2.
+// Per (sample, channel) stiffness-weighted mean of the prediction P [v][kPld] over the sample's g^3
+// voxels, and (channel 0 blocks) the sample's mean weight.
+__global__ __launch_bounds__(256) void k_chan_mean(const float* __restrict__ P, const uint16_t* __restrict__ in, long nsv,
+                                                   float* __restrict__ mean, float* __restrict__ wbar) {
+    __shared__ float sh[2][256];
+    const int b = blockIdx.x, c = blockIdx.y;
+    float s = 0.f, ws = 0.f;
+    for (long v = threadIdx.x; v < nsv; v += blockDim.x) {
+        const long vv = (long)b * nsv + v;
+        const float w = net_weight(in, vv);
+        s += w * P[vv * kPld + c];
+        ws += w;
+    }
+    sh[0][threadIdx.x] = s;
+    sh[1][threadIdx.x] = ws;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + k];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        mean[b * kOut + c] = sh[1][0] > 0.f ? sh[0][0] / sh[1][0] : 0.f;
+        if (c == 0) wbar[b] = sh[1][0] / (float)nsv;
+    }
+}
+
+// Loss terms with stiffness weights (w / mean w per sample): d = P - mean_w P - T;
+// dP = 2 (w / wbar) d / count (bf16, [v][kDpLd], channels >= 18 stay 0); per-block FP64 partial sums of
+// (w / wbar) d^2.  T is mean-free under the same weights, so the weighted mean needs no gradient term.
 __global__ __launch_bounds__(256) void k_head_loss(const float* __restrict__ P, const float* __restrict__ mean,
+                                                   const float* __restrict__ wbar, const uint16_t* __restrict__ in,
                                                    const uint16_t* __restrict__ T, long nvox, int lgs, double inv_count,
                                                    uint16_t* __restrict__ dP, double* __restrict__ partial) {
     __shared__ double sh[256];
@@ -351,9 +373,11 @@ __global__ __launch_bounds__(256) void k_head_loss(const float* __restrict__ P, 
         const long v = i / kOut;
         const int c = (int)(i - v * kOut);
         const long b = v >> (3 * lgs);
+        const float wb = wbar[b];
+        const float wt = wb > 0.f ? net_weight(in, v) / wb : 0.f;
         const float d = P[v * kPld + c] - mean[b * kOut + c] - bf2f(T[i]);
-        dP[v * kDpLd + c] = f2bf((float)(2.0 * inv_count) * d);
-        s += (double)d * d;
+        dP[v * kDpLd + c] = f2bf((float)(2.0 * inv_count) * wt * d);
+        s += (double)wt * d * d;
     }
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -464,6 +488,7 @@ struct homo3d_net {
     uint16_t* T = nullptr;       // targets [v][18]
     uint16_t* dP = nullptr;      // [v][kDpLd]
     float* mean = nullptr;       // [batch][18]
+    float* wbar = nullptr;       // [batch] mean stiffness weight
     double* lpart = nullptr;     // loss partial sums
     int lblocks = 1024;
     float* rho = nullptr;        // host-staged densities [batch][g^3]
@@ -627,10 +652,10 @@ int loss_backward(homo3d_net* n, int batch, double* loss) {
     const NetLevel& V0 = n->lv[0];
     const long nv0 = (long)batch << (3 * V0.lg);
     const long nsv = 1L << (3 * V0.lg);
-    hipLaunchKernelGGL(k_chan_mean, dim3(batch, kOut), dim3(256), 0, n->stream, n->P, nsv, n->mean);
+    hipLaunchKernelGGL(k_chan_mean, dim3(batch, kOut), dim3(256), 0, n->stream, n->P, V0.in, nsv, n->mean, n->wbar);
     const double inv = 1.0 / ((double)nv0 * kOut);
-    hipLaunchKernelGGL(k_head_loss, dim3(n->lblocks), dim3(256), 0, n->stream, n->P, n->mean, n->T, nv0, V0.lg, inv,
-                       n->dP, n->lpart);
+    hipLaunchKernelGGL(k_head_loss, dim3(n->lblocks), dim3(256), 0, n->stream, n->P, n->mean, n->wbar, V0.in, n->T, nv0,
+                       V0.lg, inv, n->dP, n->lpart);
     NETCHK(n, hipMemsetAsync(n->g, 0, sizeof(float) * n->nparam, n->stream));
     int li = (int)n->layers.size() - 1;
     int rc;
@@ -733,7 +758,7 @@ void destroy(homo3d_net* n) {
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
     }
-    void* ptrs[] = {n->p, n->g, n->m1, n->m2, n->P, n->T, n->dP, n->mean, n->lpart, n->rho, n->tstage, n->zc, n->xc,
+    void* ptrs[] = {n->p, n->g, n->m1, n->m2, n->P, n->T, n->dP, n->mean, n->wbar, n->lpart, n->rho, n->tstage, n->zc, n->xc,
                     n->partial, n->ds_in, n->ds_tg};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -876,7 +901,7 @@ int homo3d_net_create(const homo3d_net_config* cfg, homo3d_net** out) {
     if (hipMalloc(&n->p, sizeof(float) * n->nparam) || hipMalloc(&n->g, sizeof(float) * n->nparam) ||
         hipMalloc(&n->m1, sizeof(float) * n->nparam) || hipMalloc(&n->m2, sizeof(float) * n->nparam) ||
         hipMalloc(&n->P, sizeof(float) * nv0 * kPld) || hipMalloc(&n->T, sizeof(uint16_t) * nv0 * kOut) ||
-        hipMalloc(&n->dP, sizeof(uint16_t) * nv0 * kDpLd) || hipMalloc(&n->mean, sizeof(float) * B * kOut) ||
+        hipMalloc(&n->dP, sizeof(uint16_t) * nv0 * kDpLd) || hipMalloc(&n->mean, sizeof(float) * B * kOut) || hipMalloc(&n->wbar, sizeof(float) * B) ||
         hipMalloc(&n->lpart, sizeof(double) * n->lblocks) || hipMalloc(&n->rho, sizeof(float) * nv0) ||
         hipMalloc(&n->tstage, sizeof(float) * nv0 * kOut) ||
         hipMalloc(&n->zc, sizeof(uint16_t) * max_zc) || hipMalloc(&n->xc, sizeof(uint16_t) * max_xc) ||
@@ -980,21 +1005,23 @@ int homo3d_net_dataset_reserve(homo3d_net* n, int count) {
 
 int homo3d_net_dataset_size(const homo3d_net* n) { return n ? n->ds_count : -1; }
 
-// Adam epochs over the device-resident dataset: each epoch visits the pairs in a seeded random order in
-// batches of the configured size (the last partial batch is dropped).  losses[e] = mean batch loss.
-int homo3d_net_train(homo3d_net* n, int epochs, double lr, uint64_t seed, double* losses) {
+// Adam epochs over the first `count` pairs of the device-resident dataset (all when count <= 0): each epoch
+// visits them in a seeded random order in batches of the configured size (the last partial batch is
+// dropped).  losses[e] = mean batch loss of epoch e.
+int homo3d_net_train(homo3d_net* n, int epochs, int count, double lr, uint64_t seed, double* losses) {
     if (!n || epochs < 0) return HOMO3D_BAD_ARG;
     const int B = n->cfg.batch;
-    if (n->ds_count < B) return nerr(n, HOMO3D_BAD_ARG, "dataset holds %d pairs, fewer than one batch", n->ds_count);
+    if (count <= 0 || count > n->ds_count) count = n->ds_count;
+    if (count < B) return nerr(n, HOMO3D_BAD_ARG, "training on %d pairs, fewer than one batch", count);
     const long nsv = (long)n->cfg.grid * n->cfg.grid * n->cfg.grid;
     std::mt19937_64 rng(seed);
-    std::vector<int> order(n->ds_count);
-    for (int i = 0; i < n->ds_count; i++) order[i] = i;
+    std::vector<int> order(count);
+    for (int i = 0; i < count; i++) order[i] = i;
     for (int e = 0; e < epochs; e++) {
         std::shuffle(order.begin(), order.end(), rng);
         double sum = 0;
         int steps = 0;
-        for (int s = 0; s + B <= n->ds_count; s += B) {
+        for (int s = 0; s + B <= count; s += B) {
             for (int b = 0; b < B; b++) {
                 const long k = order[s + b];
                 NETCHK(n, hipMemcpyAsync(n->lv[0].in + b * nsv * kInCh, n->ds_in + k * nsv * kInCh,

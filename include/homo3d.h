@@ -118,6 +118,9 @@ void* homo3d_stream(homo3d_handle* h);
 int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_device);
 int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* stats);
 int homo3d_elastic_matrix(homo3d_handle* h, double C[36]);
+/* FP64 relres ||f - K u|| / ||f|| per load case of the refined solver's current iterate, without solving
+ * (Grid::update_residual with the norms of MG::solveEquation, MG.cpp:113-203, in FP64). */
+int homo3d_residual_norms(homo3d_handle* h, double relres[6]);
 int homo3d_sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pitch, int on_device);
 /* Homogenization::Sensitivity(dC, sens, pitchT, lexiOrder) (homogenization.h:38): lexi_order = 1 is
  * homo3d_sensitivity; 0 writes the library's own cell order (parity-blocked: eight (n/2)^3 blocks,
@@ -221,8 +224,9 @@ int homo3d_net_train_step(homo3d_net* net, const float* rho, const float* target
 int homo3d_net_dataset_reserve(homo3d_net* net, int count);
 int homo3d_net_dataset_add(homo3d_net* net, homo3d_handle* h);
 int homo3d_net_dataset_size(const homo3d_net* net);
-/* Adam epochs over the training set in seeded random order; losses[e] = mean batch loss of epoch e. */
-int homo3d_net_train(homo3d_net* net, int epochs, double lr, uint64_t seed, double* losses);
+/* Adam epochs over the first `count` pairs of the training set (all when count <= 0) in seeded random order;
+ * losses[e] = mean batch loss of epoch e. */
+int homo3d_net_train(homo3d_net* net, int epochs, int count, double lr, uint64_t seed, double* losses);
 /* Mean loss over pairs [first, first + count) of the training set, without updating. */
 int homo3d_net_eval(homo3d_net* net, int first, int count, double* loss);
 /* Predict the solution of the handle's current density and make it the FP64 iterate that the next warm

@@ -19,7 +19,10 @@ g^3 vertices, scaled by 1/n and mean-free per channel:
 conv3 is a 3x3x3 periodic correlation, tap t = (dx+1) + 3 (dy+1) + 9 (dz+1) reads X[v + (dx,dy,dz)];
 weights W[co][t][ci].  up2 is nearest-neighbour.  Tensors are channels-last [B, z, y, x, C].
 
-Loss: mean over (B, g^3, 18) of (P - mean_v P - T)^2 with T mean-free per (sample, channel).
+Loss: mean over (B, g^3, 18) of (w / mean_v w) (P - m_w(P) - T)^2 with the stiffness weights
+w = rho^3 of the input, m_w the w-weighted mean over a sample's voxels, and T w-mean-free per
+(sample, channel): the displacement of near-void cells (stiffness ~1e-9) is barely determined by the
+solve and carries no weight, as in the energy norm.
 
 Pins (tests/test_unet_oracle.py): central finite differences of the loss against the analytic
 backward on every layer kind; translation invariance of the loss; periodic equivariance of the
@@ -175,17 +178,32 @@ def forward(theta, rho, levels=4, c0=16, bf16=False, keep=False):
     return P
 
 
-def loss(P, T):
-    Pm = P - P.mean(axis=(1, 2, 3), keepdims=True)
-    return float(np.mean((Pm - T) ** 2))
+def weights(rho, bf16=False):
+    """Stiffness weights w = rho^3 / mean(rho^3) per sample, [B, g, g, g, 1]."""
+    r = bf16_round(rho) if bf16 else np.asarray(rho, np.float64)
+    w = r ** 3
+    return (w / w.mean(axis=(1, 2, 3), keepdims=True))[..., None]
+
+
+def center(P, w):
+    """P minus its w-weighted mean over each sample's voxels, per channel."""
+    return P - (w * P).sum(axis=(1, 2, 3), keepdims=True) / w.sum(axis=(1, 2, 3), keepdims=True)
+
+
+def loss(P, T, rho, bf16=False):
+    w = weights(rho, bf16)
+    return float(np.mean(w * (center(P, w) - T) ** 2))
 
 
 def loss_and_grad(theta, rho, T, levels=4, c0=16, bf16=False):
     """Loss and its gradient with respect to the flat parameter vector (same layout as theta)."""
     P, acts, P_ = forward(theta, rho, levels, c0, bf16, keep=True)
-    Pm = P - P.mean(axis=(1, 2, 3), keepdims=True)
-    L = float(np.mean((Pm - T) ** 2))
-    dP = 2.0 * (Pm - T) / Pm.size
+    w = weights(rho, bf16)
+    d = center(P, w) - T
+    L = float(np.mean(w * d ** 2))
+    # d/dP of the weighted mean term vanishes when sum_v w d = 0 (T w-mean-free); keep it general:
+    wd = w * d
+    dP = 2.0 * (wd - w * wd.sum(axis=(1, 2, 3), keepdims=True) / w.sum(axis=(1, 2, 3), keepdims=True)) / d.size
     grads = {}
     W, b = P_["head"]
     dD, dW, db = conv_backward(acts["dec0.c2"] if levels > 1 else acts["enc0.c2"], W, dP)

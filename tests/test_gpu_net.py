@@ -1,6 +1,8 @@
 """GPU tests of the learned warm start's network (net.hip) against the FP64 oracle
 (oracle/unet_oracle.py, itself pinned by finite differences in tests/test_unet_oracle.py).
 
+The loss weighs each voxel by its stiffness rho^3 (see oracle/unet_oracle.py).
+
 Tolerances: the GPU stores weights, activations and activation gradients in bf16 (8 bits of mantissa)
 and accumulates in FP32.  The oracle with bf16=True rounds weights and stored activations the same
 way, so the forward pass agrees to a few bf16 ulps (2e-2 of the output scale); gradients pass through
@@ -70,6 +72,7 @@ def test_loss_and_gradient_match_oracle(h3d):
     theta += (0.02 * rng.standard_normal(theta.size)).astype(np.float32)
     net.set_params(theta)
     rho, T = _inputs(g, B, 5)
+    T = U.center(T, U.weights(rho, bf16=True))  # targets are mean-free under the loss weights
     loss, grad = net.loss_grad(rho.reshape(B, -1), T.reshape(B, -1, 18))
     Lr, gr = U.loss_and_grad(theta.astype(np.float64), rho, U.bf16_round(T), levels, 16, bf16=True)
     say("  loss gpu %.6e oracle %.6e" % (loss, Lr))
@@ -95,6 +98,7 @@ def test_training_reduces_loss(h3d):
     rho, _ = _inputs(g, B, 8)
     T = np.repeat(((rho - rho.mean(axis=(1, 2, 3), keepdims=True)) * 2.0)[..., None], 18, axis=-1)
     T *= np.linspace(-1, 1, 18)
+    T = U.center(T, U.weights(rho, bf16=True))
     losses = [net.train_step(rho.reshape(B, -1), T.reshape(B, -1, 18), 2e-3) for _ in range(60)]
     say("  loss %.4e -> %.4e" % (losses[0], losses[-1]))
     assert losses[-1] < 0.2 * losses[0]

@@ -21,6 +21,24 @@ def test_param_count_closed_form():
     assert U.num_params(4, 16) == n
 
 
+def test_weighted_loss_ignores_void_cells():
+    """The loss weights are the stiffness rho^3: errors where rho = 1e-3 (weight 1e-9 of a solid cell)
+    change it by nothing measurable, errors in solid cells by their full square."""
+    g, B = 8, 1
+    rho = np.full((B, g, g, g), 1e-3)
+    rho[:, :, :, :4] = 1.0  # half solid
+    w = U.weights(rho)
+    T = U.center(np.zeros((B, g, g, g, U.NOUT)), w)
+    P = np.zeros_like(T)
+    assert U.loss(P, T, rho) == 0.0
+    Pv = P.copy()
+    Pv[:, :, :, 6, :] += 1.0  # a void plane
+    Ps = P.copy()
+    Ps[:, :, :, 1, :] += 1.0  # a solid plane
+    assert U.loss(Pv, T, rho) < 1e-8 * U.loss(Ps, T, rho)
+    assert U.loss(Ps, T, rho) > 0.05
+
+
 def test_bf16_round_known_values():
     vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 1e-40, 3.0e38])
     r = U.bf16_round(vals)
@@ -38,7 +56,7 @@ def _setup(levels=3, c0=4, g=8, B=2, seed=0):
     theta += 0.05 * rng.standard_normal(theta.size)  # non-zero biases
     rho = rng.uniform(0.05, 1.0, (B, g, g, g))
     T = rng.standard_normal((B, g, g, g, U.NOUT)) * 0.1
-    T -= T.mean(axis=(1, 2, 3), keepdims=True)
+    T = U.center(T, U.weights(rho))
     return theta, rho, T
 
 
@@ -61,7 +79,7 @@ def test_backward_matches_finite_differences():
         tp, tm = theta.copy(), theta.copy()
         tp[i] += h
         tm[i] -= h
-        fd = (U.loss(U.forward(tp, rho, levels, c0), T) - U.loss(U.forward(tm, rho, levels, c0), T)) / (2 * h)
+        fd = (U.loss(U.forward(tp, rho, levels, c0), T, rho) - U.loss(U.forward(tm, rho, levels, c0), T, rho)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, np.abs(g).max()) + 1e-4 * abs(fd), (i, fd, g[i])
 
 
@@ -73,8 +91,8 @@ def test_forward_is_periodic_equivariant_and_loss_translation_invariant():
     Ps = U.forward(theta, np.roll(rho, 4, axis=3), levels, c0)
     assert np.allclose(Ps, np.roll(P, 4, axis=3), atol=1e-12)
     # adding a constant per channel to the prediction leaves the loss unchanged
-    L0 = U.loss(P, T)
-    assert abs(U.loss(P + np.arange(18.0), T) - L0) < 1e-12
+    L0 = U.loss(P, T, rho)
+    assert abs(U.loss(P + np.arange(18.0), T, rho) - L0) < 1e-12
 
 
 def test_bf16_forward_close_to_fp64():
