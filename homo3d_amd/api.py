@@ -83,6 +83,9 @@ EXPORTS = {
     "homo3d_oc_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int]),
+    "homo3d_oc_update_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_void_p]),
     "homo3d_mma_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_void_p)]),

@@ -3370,3 +3370,78 @@ void launch_f64_to_f32(hipStream_t s, long n, const double* a, float* b) {
 
 }  // namespace homo3d
 
+// =============================================================================================
+// OC update without allocation: homo3d_oc_update_noalloc
+// =============================================================================================
+// The bisection's reduction scratch (|s| maxima of kOcBlocks workgroups, kOcBlocks FP64 partial volume
+// sums and their total) lives in this file-scope device array: every GPU the library runs on holds one
+// copy until the process ends.  Its device address is looked up once per device and used for every
+// launch; every launch rewrites what it reads, so nothing needs clearing between calls.  Calls that
+// overlap on one GPU share the array and race.
+namespace {
+constexpr int kOcBlocks = 1024;
+constexpr int kOcMaxReso = 1024;  // largest grid accepted (as the density-only handle)
+constexpr int kOcScratchDoubles = kOcBlocks + kOcBlocks / 2 + 8;  // partials, float maxima, total
+constexpr int kOcMaxDevices = 64;
+static __device__ __align__(256) double scratch[kOcScratchDoubles];
+double* g_scratch[kOcMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kOcMaxDevices) return HOMO3D_HIP_ERROR;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HOMO3D_HIP_ERROR;
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
+    return HOMO3D_OK;
+}
+}  // namespace
+
+extern "C" int homo3d_oc_update_noalloc(const float* sens, float* rho, int n, int pitch, double vol, double min_rho,
+                                        double step, double damp, void* stream) {
+    using namespace homo3d;
+    if (!sens || !rho || n < 1 || pitch < n) return HOMO3D_BAD_ARG;
+    if (n > kOcMaxReso) return HOMO3D_BAD_ARG;  // beyond the stated maximum grid
+    double* sc = nullptr;
+    int rc = scratch_ptr(&sc);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* partial = sc;
+    float* pmax = reinterpret_cast<float*>(sc + kOcBlocks);
+    double* total = sc + kOcBlocks + kOcBlocks / 2;
+    const long N = (long)n * n * n;
+    hipLaunchKernelGGL(k_absmax, dim3(kOcBlocks), dim3(256), 0, s, N, n, pitch, sens, pmax);
+    std::vector<float> pm(kOcBlocks);
+    if (hipMemcpyAsync(pm.data(), pmax, sizeof(float) * kOcBlocks, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return HOMO3D_HIP_ERROR;
+    double hi = 0, lo = 0, g = 0;
+    for (float v : pm) hi = std::max(hi, (double)v);
+    const double Ne = (double)N;
+    // the same bisection as homo3d_oc_update (OCOptimizer.cu:218-247): the trials only sum, and the
+    // last trial's multiplier is applied in place at the end
+    for (int it = 0; it < 20; it++) {
+        g = (hi + lo) / 2;
+        double sum = 0;
+        hipLaunchKernelGGL(k_oc_trial, dim3(kOcBlocks), dim3(256), 0, s, n, pitch, g, min_rho, step, damp, sens,
+                           (const float*)rho, (float*)nullptr, partial);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kSumThreads), 0, s, kOcBlocks, (const double*)partial, total);
+        if (hipMemcpyAsync(&sum, total, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return HOMO3D_HIP_ERROR;
+        const double cur = sum / Ne;
+        if (cur < vol - 1e-4)
+            hi = g;
+        else if (cur > vol + 1e-4)
+            lo = g;
+        else
+            break;
+    }
+    hipLaunchKernelGGL(k_oc_apply, dim3(kOcBlocks), dim3(256), 0, s, n, pitch, g, min_rho, step, damp, sens, rho);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return HOMO3D_HIP_ERROR;
+    return HOMO3D_OK;
+}

@@ -168,6 +168,16 @@ int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch,
 int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch, double vol, double min_rho,
                      double step, double damp, int on_device);
 
+/* homo3d_oc_update without any allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  sens and rho are device pointers (lexicographic, row pitch `pitch`, rho updated in
+ * place); stream is a hipStream_t or NULL for the default stream; 1 <= n <= 1024 (HOMO3D_BAD_ARG
+ * beyond).  The bisection's reduction scratch is a fixed 20 KB device array inside the library (one per
+ * GPU, held until the process ends); calls that overlap on one GPU share it and race.  The result is
+ * bit-identical to homo3d_oc_update's. */
+int homo3d_oc_update_noalloc(const float* sens, float* rho, int n, int pitch, double vol, double min_rho, double step,
+                             double damp, void* stream);
+
 /* MMA (homo::MMAOptimizer, optimization/mmaOptimizer.h:12-34; mmasubDevice, mmaOpt.h:17-19).
  * FP64 state on the GPU; m <= 4 constraints; constants of mmasub_ker (mmaOpt.cu:846-851), move = 1. */
 typedef struct homo3d_mma homo3d_mma;
