@@ -34,6 +34,8 @@ PROF_KERNEL = {"fine_gs": ("k_fine_gs2<6, true, -1", "k_fine_gs2ILi6ELb1ELin1E",
                               "k_fine_apply_efILi0ELb1ELb1ELi2ELi2E", "k_fine_apply<6, 0, true, half2, true, 2>"),
                "fine_residual": ("k_fine_apply<6, 1, false", "k_fine_applyILi6ELi1ELb0E")}
 DOMINANT = "fine_apply"  # the kernel with the largest total time per step
+NET_WEIGHTS = os.path.join(ROOT, "homo3d_amd", "data", "warmstart_unet64.npz")
+CH_FIXTURE = os.path.join(ROOT, "tests", "golden", "ch_256_rotate3_seed0.json")
 METRIC = "solves/s to relres 1e-5 @256^3, learned-warm-start MG vs plain MG; stencil GB/s"
 
 
@@ -115,7 +117,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-5)
-    ap.add_argument("--cpu-n", type=int, default=None, help="oracle grid for the CPU baseline (default: --n)")
+    ap.add_argument("--cpu-n", type=int, default=96,
+                    help="oracle grid of the CPU baseline's bounded sample (a full solve there, scaled to --n)")
+    ap.add_argument("--net", default=NET_WEIGHTS, help="trained warm-start network ('' to skip the learned leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--omega", type=float, default=None, help="smoother relaxation factor (default: the library's)")
     ap.add_argument("--mfma", type=int, default=None,
@@ -213,13 +217,62 @@ def main():
                    "worst_relres_all_steps": float("%.3e" % worst)},
         "roofline": roofline,
     }
+    if n == 256 and os.path.exists(CH_FIXTURE):
+        # the FP64 oracle's C^H of this design (tests/golden/make_ch256.py)
+        ref = json.load(open(CH_FIXTURE))
+        out["config"]["ch_vs_fp64_oracle"] = ch_errors(C, np.array(ref["C"]).reshape(6, 6))
+        out["config"]["ch_vs_fp64_oracle"]["oracle_solve_s"] = round(ref["setup_s"] + ref["solve_s"], 1)
+        out["config"]["ch_vs_fp64_oracle"]["oracle_threads"] = ref["threads"]
+    if args.net and os.path.exists(args.net):
+        out["learned"] = learned_leg(homo3d_amd, hom, x, n, args, sync, check, log)
+        out["plain"] = {"value": out["value"], "ms_per_step": out["ms_per_step"]}
+        out["learned_vs_plain"] = round(out["learned"]["value"] / out["value"], 4)
     if not args.no_cpu:
         cpu_n = args.cpu_n or n
         log("cpu baseline: oracle at %d^3" % cpu_n)
         out["cpu_baseline"], Cref = cpu_baseline(cpu_n, n, args.tol)
-        if cpu_n == n:
-            out["config"]["ch_vs_fp64_oracle"] = ch_errors(C, Cref)
     print(json.dumps(out), flush=True)
+
+
+def learned_leg(homo3d_amd, hom, x, n, args, sync, check, log):
+    """The same steps with the learned warm start: update, network prediction as the FP64 start of the
+    refined solve (a load case whose predicted start has relres >= 1 restarts from zero), solve, C^H."""
+    net = homo3d_amd.WarmStartNet.load(args.net)
+
+    def lstep():
+        hom.update(x.value, n, on_device=True)
+        net.warm_start(hom)
+        st = hom.solve(args.tol, warm=True)
+        C = hom.elasticMatrix(solve=False)
+        return st, C
+
+    for i in range(args.warmup):
+        st, _ = lstep()
+        check(st, "learned warmup %d" % i)
+    sync()
+    t0 = time.perf_counter()
+    stats = []
+    for i in range(args.steps):
+        st, C = lstep()
+        stats.append(st)
+    sync()
+    dt = time.perf_counter() - t0
+    for i, st in enumerate(stats):
+        check(st, "learned step %d" % i)
+        log("learned step %d: iters %s relres %.2e" % (i, st["iters"], max(st["relres"])))
+    # untimed: the prediction's own FP64 relres per load case
+    hom.update(x.value, n, on_device=True)
+    net.warm_start(hom)
+    start = hom.residual_norms()
+    rec = {"value": round(args.steps / dt, 5), "ms_per_step": round(dt / args.steps * 1e3, 3),
+           "iters": stats[-1]["iters"], "relres": [float("%.3e" % r) for r in stats[-1]["relres"]],
+           "start_relres": [float("%.3e" % r) for r in start],
+           "network": {"grid": net.grid, "levels": net.levels, "channels": net.base_channels,
+                       "params": net.num_params}}
+    if n == 256 and os.path.exists(CH_FIXTURE):
+        ref = json.load(open(CH_FIXTURE))
+        rec["ch_vs_fp64_oracle"] = ch_errors(C, np.array(ref["C"]).reshape(6, 6))
+    return rec
 
 
 if __name__ == "__main__":

@@ -83,6 +83,10 @@ EXPORTS = {
     "homo3d_oc_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int]),
+    "homo3d_density_filter": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "homo3d_projection": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "homo3d_oc_update_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_void_p]),
@@ -439,6 +443,24 @@ class Homogenization:
         self._chk(lib().homo3d_symmetrize(self._h, _ptr(a), self.n, SYM[sym] if isinstance(sym, str) else int(sym),
                                           0), "symmetrize")
         return a
+
+    def density_filter(self, field, radius=2.0, kernel="spline4", periodic=False, backward=False):
+        """conv with radial_convker_t<Spline4|Linear>(radius) on an n^3 field (or its adjoint)."""
+        a = np.ascontiguousarray(field, dtype=np.float32).ravel()
+        out = np.zeros_like(a)
+        self._chk(lib().homo3d_density_filter(self._h, _ptr(a), _ptr(out), self.n, float(radius),
+                                              0 if kernel == "spline4" else 1, int(periodic), int(backward), 0),
+                  "density_filter")
+        return out
+
+    def projection(self, field, k=30.0, c=0.5, y=None):
+        """sgm(k, c) of an n^3 field; with y (the forward output) the backward pass of dJ/dy = field."""
+        a = np.ascontiguousarray(field, dtype=np.float32).ravel()
+        out = np.zeros_like(a)
+        yy = None if y is None else np.ascontiguousarray(y, dtype=np.float32).ravel()
+        self._chk(lib().homo3d_projection(self._h, _ptr(a), None if yy is None else _ptr(yy), _ptr(out), self.n,
+                                          float(k), float(c), int(y is not None), 0), "projection")
+        return out
 
     def profile_enable(self, on=True, only=None):
         """Event timing of the finest-level kernels; `only` restricts it to one kind (profile_get names)."""

@@ -1473,6 +1473,76 @@ int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch,
 }
 
 // OC update of rho in place (OCOptimizer::update, OCOptimizer.cu:218-247).
+// Density filter conv(rho, radial_convker_t<Spline4 | Linear>(radius)) (TensorExpression.h:1007-1046,
+// TensorExpression.cuh:288-368), forward (backward = 0) or its adjoint (backward = 1: in holds dJ/dout,
+// out receives dJ/din).  kernel: 0 = Spline4, 1 = Linear.  in and out may be the same array.
+int homo3d_density_filter(homo3d_handle* h, const float* in, float* out, int pitch, double radius, int kernel,
+                          int periodic, int backward, int on_device) {
+    if (!h || !in || !out || !(radius > 0) || kernel < 0 || kernel > 1) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(out, on_device);
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const int hw = (int)(radius + 0.5);
+    const int side = 2 * hw + 1;
+    if (side * side * side > 4096) return set_err(h, HOMO3D_NO_SUPPORT, "filter radius too large");
+    if (2 * hw + 1 > n && periodic) return set_err(h, HOMO3D_NO_SUPPORT, "periodic filter wider than the grid");
+    std::vector<double> wd(side * side * side);
+    double W = 0;
+    for (int i = 0; i < side * side * side; i++) {
+        const int o[3] = {i % side - hw, (i / side) % side - hw, i / (side * side) - hw};
+        const double q = (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) / (radius * radius);
+        wd[i] = q < 1 ? (kernel == 0 ? 1 - 6 * q + 8 * q * std::sqrt(q) - 3 * q * q : 1 - std::sqrt(q)) : 0.0;
+        W += wd[i];
+    }
+    std::vector<float> wk(wd.size());
+    for (size_t i = 0; i < wk.size(); i++) wk[i] = (float)(wd[i] / W);
+    HIPCHK(h, hipMemcpyAsync(h->d_ftmp, wk.data(), sizeof(float) * wk.size(), hipMemcpyHostToDevice, h->stream));
+    const size_t elems = (size_t)pitch * n * n;
+    int rc = ensure_stage(h, 3 * elems);
+    if (rc) return rc;
+    float* din = h->d_stage;
+    float* dws = h->d_stage + elems;
+    float* dres = h->d_stage + 2 * elems;
+    HIPCHK(h, hipMemcpyAsync(din, in, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             h->stream));
+    if (!backward) {
+        launch_conv_filter(h->stream, n, pitch, hw, periodic, h->d_ftmp, din, dres, nullptr);
+    } else {
+        launch_conv_filter(h->stream, n, pitch, hw, periodic, h->d_ftmp, din, dres, dws);  // weight sums W
+        launch_conv_filter_back(h->stream, n, pitch, hw, periodic, h->d_ftmp, din, dws, dres);
+    }
+    HIPCHK(h, hipMemcpyAsync(out, dres, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+// sgm(k, c) projection (sgm_exp_t, AutoDiff.h:808-823): out = 1 / (1 + exp(-k (in - c))); with
+// backward = 1, y holds the forward output and out = in * k y (1 - y) (in = dJ/dy).
+int homo3d_projection(homo3d_handle* h, const float* in, const float* y, float* out, int pitch, double k, double c,
+                      int backward, int on_device) {
+    if (!h || !in || !out || (backward && !y)) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(out, on_device);
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const size_t elems = (size_t)pitch * n * n;
+    int rc = ensure_stage(h, 3 * elems);
+    if (rc) return rc;
+    float *din = h->d_stage, *dy = h->d_stage + elems, *dres = h->d_stage + 2 * elems;
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPCHK(h, hipMemcpyAsync(din, in, sizeof(float) * elems, kin, h->stream));
+    if (backward) {
+        HIPCHK(h, hipMemcpyAsync(dy, y, sizeof(float) * elems, kin, h->stream));
+        launch_sgm_back(h->stream, n, pitch, k, dy, din, dres);
+    } else {
+        launch_sgm(h->stream, n, pitch, k, c, din, dres);
+    }
+    HIPCHK(h, hipMemcpyAsync(out, dres, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
 int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch, double vol, double min_rho,
                      double step, double damp, int on_device) {
     if (!h || !sens || !rho) return HOMO3D_BAD_ARG;

@@ -135,6 +135,18 @@ struct DensityOps {
         int rc = homo3d_symmetrize(h, f.data(), n, sym_code(s), 0);
         if (rc) throw Homo3dError(rc, std::string("symmetrize: ") + homo3d_last_error(h));
     }
+    // conv(radial_convker_t<Spline4>(radius)) (non-periodic, normalised) or its adjoint
+    void spline4_filter(const std::vector<float>& in, std::vector<float>& out, double radius, bool backward) {
+        out.resize(in.size());
+        int rc = homo3d_density_filter(h, in.data(), out.data(), n, radius, 0, 0, backward ? 1 : 0, 0);
+        if (rc) throw Homo3dError(rc, std::string("density filter: ") + homo3d_last_error(h));
+    }
+    // sgm(k = 30, c = 0.5) or its backward (y the forward output, in = dJ/dy)
+    void sgm(const std::vector<float>& in, const std::vector<float>* y, std::vector<float>& out) {
+        out.resize(in.size());
+        int rc = homo3d_projection(h, in.data(), y ? y->data() : nullptr, out.data(), n, 30.0, 0.5, y ? 1 : 0, 0);
+        if (rc) throw Homo3dError(rc, std::string("projection: ") + homo3d_last_error(h));
+    }
 };
 
 // ConvergeChecker (Framework.cuh:312-343): converged when the relative change stays below the
@@ -376,6 +388,8 @@ std::vector<double> run_mma(cfg::HomoConfig config, const ObjectiveFn& objective
     DensityOps ops(config.reso[0]);
     std::vector<float> rho, grad;
     initDensity(rho, config, ops);
+    std::ofstream(out_path(config, "initRho"), std::ios::binary)
+        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
     Evaluator ev(hom);
     const int ne = (int)rho.size();
     double C[6][6];
@@ -450,10 +464,72 @@ MmaConstraint shear_isotropy_constraint(const double C[6][6]) {
     return k;
 }
 
+// example_opti_shear_isotropy (Framework.cu:282-356): maximise the shear modulus by MMA with a volume
+// constraint and an isotropy (Zener ratio) constraint on the density rhop = sgm(conv(rho, Spline4(r)))^3.
+// As there: the volume is that of sgm(conv(rho)); the constraint values carry a factor 1000 their
+// gradients do not (the reference back-propagates the unscaled expressions); the objective and isotropy
+// gradients are symmetrised, the volume gradient is not.
 std::vector<double> example_opti_shear_isotropy(cfg::HomoConfig config) {
-    auto obj = [](int, const double C[6][6], float dC[6][6]) { return shear_objective(C, dC); };
-    auto aniso = [](const double C[6][6]) { return std::vector<MmaConstraint>{shear_isotropy_constraint(C)}; };
-    return run_mma(config, obj, aniso);
+    Homogenization hom(config);
+    DensityOps ops(config.reso[0]);
+    std::vector<float> rho;
+    initDensity(rho, config, ops);
+    std::ofstream(out_path(config, "initRho"), std::ios::binary)
+        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
+    const int ne = (int)rho.size();
+    MMAOptimizer mma(2, ne, 1, 0, 1000, 1);
+    mma.setBound(0.001f, 1.f);
+    ConvergeChecker criteria(config.finthres);
+    std::vector<double> objlist;
+    std::vector<float> c, rhat, x(ne), sens(ne), t(ne), t2, objGrad, gGrad, volGrad;
+    // dJ/drho for dJ/dC = dC through x = rhat^3, rhat = sgm(c), c = conv(rho)
+    auto backward = [&](float dC[6][6], std::vector<float>& grad) {
+        hom.Sensitivity(dC, sens.data(), hom.grid->cellReso[0], true);
+        for (int i = 0; i < ne; i++) t[i] = 3.f * rhat[i] * rhat[i] * sens[i];
+        ops.sgm(t, &rhat, t2);
+        ops.spline4_filter(t2, grad, config.filterRadius, true);
+    };
+    double C[6][6];
+    for (int iter = 0; iter < config.max_iter; iter++) {
+        ops.spline4_filter(rho, c, config.filterRadius, false);
+        ops.sgm(c, nullptr, rhat);
+        for (int i = 0; i < ne; i++) x[i] = rhat[i] * rhat[i] * rhat[i];
+        hom.update(x.data(), hom.grid->cellReso[0]);
+        hom.elasticMatrix(C);
+        float dC[6][6] = {};
+        const double val = shear_objective(C, dC);
+        objlist.push_back(val);
+        backward(dC, objGrad);
+        std::printf(" * Iter %d   obj = %.4e   (solve %.3fs)\n", iter, val, hom.last_stats.seconds);
+        if (criteria.is_converge(iter, val)) {
+            std::printf("= converged\n");
+            break;
+        }
+        ops.symmetrize(objGrad, config.sym, config.usesym);
+        MmaConstraint k = shear_isotropy_constraint(C);
+        backward(k.dC, gGrad);
+        ops.symmetrize(gGrad, config.sym, config.usesym);
+        double vsum = 0;
+        for (float r : rhat) vsum += r;
+        const double vol_ratio = vsum / ne;
+        float gval[2] = {(float)((vol_ratio - config.volRatio) * 1000.0), (float)k.value};
+        std::fill(t.begin(), t.end(), 1.f / ne);
+        ops.sgm(t, &rhat, t2);
+        ops.spline4_filter(t2, volGrad, config.filterRadius, true);
+        std::printf("zener ratio = %4.2e ; obj = %4.2e ; vol = %4.2e\n", std::sqrt(std::max(k.value + 0.1, 0.0) / 1e6) + 1,
+                    val, vol_ratio);
+        float* dgdx[2] = {volGrad.data(), gGrad.data()};
+        mma.update(iter, rho.data(), objGrad.data(), gval, dgdx);
+        logIter(iter, config, rho, C, val);
+    }
+    std::ofstream(out_path(config, "rho"), std::ios::binary)
+        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
+    float Cf[36];
+    for (int i = 0; i < 36; i++) Cf[i] = (float)C[i / 6][i % 6];
+    std::ofstream(out_path(config, "C"), std::ios::binary).write(reinterpret_cast<const char*>(Cf), sizeof(Cf));
+    std::ofstream ol(out_path(config, "objlist"));
+    for (double v : objlist) ol << v << "\n";
+    return objlist;
 }
 
 // Negative Poisson ratio by MMA with a volume constraint (the 256^3 MMA workload): the objective of

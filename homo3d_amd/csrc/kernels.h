@@ -149,6 +149,15 @@ void launch_mma_init(hipStream_t s, long n, const double* alfa, const double* be
                      double* eta);
 void launch_f32_to_f64(hipStream_t s, long n, const float* a, double* b);
 void launch_f64_to_f32(hipStream_t s, long n, const double* a, float* b);
+// Density filter with Spline4/linear radial weights (wk: normalised (2h+1)^3 box) and its adjoint (wsum: the
+// forward pass's in-grid weight sums); sgm projection and its backward.  Pitched lexicographic arrays.
+void launch_conv_filter(hipStream_t s, int n, int pitch, int h, int periodic, const float* wk, const float* in,
+                        float* out, float* wsum);
+void launch_conv_filter_back(hipStream_t s, int n, int pitch, int h, int periodic, const float* wk, const float* dout,
+                             const float* wsum, float* din);
+void launch_sgm(hipStream_t s, int n, int pitch, double k, double c, const float* x, float* y);
+void launch_sgm_back(hipStream_t s, int n, int pitch, double k, const float* y, const float* gy, float* gx);
+
 // Bridges to the learned warm-start network (g | n; network tensors bf16 channels-last, see net.hip):
 // input channel 0 = mean of x^(1/3) over each coarse cell; target = scale (u - mean) / n sampled at the
 // coarse vertices, the mean weighted by the input's rho^3 (mean18: 18 doubles of device scratch); warm start = periodic trilinear upsampling of

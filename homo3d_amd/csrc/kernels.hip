@@ -2197,6 +2197,79 @@ __global__ __launch_bounds__(256) void k_oc_filter(int n, int pitch, int h, int 
     out[me] = (float)(sum / (ws * (double)rho[me]));
 }
 
+// Density filter conv with radial_convker_t (TensorExpression.cuh:288-368): out(i) = sum_off w(off) in(i + off)
+// / W(i), W(i) = sum of w over the neighbours inside the grid (every neighbour when periodic); wk holds the
+// normalised weights of the (2h+1)^3 box.  FP64 sums.  W is written when wsum is not null.
+__global__ __launch_bounds__(256) void k_conv_filter(int n, int pitch, int h, int periodic, const float* __restrict__ wk,
+                                                     const float* __restrict__ in, float* __restrict__ out,
+                                                     float* __restrict__ wsum) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const int p[3] = {(int)(e % n), (int)((e / n) % n), (int)(e / ((long)n * n))};
+    const int side = 2 * h + 1;
+    double sum = 0.0, ws = 0.0;
+    for (int i = 0; i < side * side * side; i++) {
+        int q[3] = {p[0] + i % side - h, p[1] + (i / side) % side - h, p[2] + i / (side * side) - h};
+        bool inb = true;
+        for (int k = 0; k < 3; k++) {
+            if (periodic) q[k] = ((q[k] % n) + n) % n;
+            if (q[k] < 0 || q[k] >= n) inb = false;
+        }
+        if (!inb) continue;
+        sum += (double)wk[i] * (double)in[q[0] + ((size_t)q[1] + (size_t)q[2] * n) * pitch];
+        ws += (double)wk[i];
+    }
+    const size_t me = p[0] + ((size_t)p[1] + (size_t)p[2] * n) * pitch;
+    out[me] = (float)(sum / ws);
+    if (wsum) wsum[me] = (float)ws;
+}
+
+// Its adjoint (conv_backward_kernel, TensorExpression.cuh:331-366): din(j) = sum_off w(off) dout(j - off) / W(j - off)
+// over the in-grid (or wrapped) j - off; wsum from k_conv_filter.
+__global__ __launch_bounds__(256) void k_conv_filter_back(int n, int pitch, int h, int periodic,
+                                                          const float* __restrict__ wk, const float* __restrict__ dout,
+                                                          const float* __restrict__ wsum, float* __restrict__ din) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const int p[3] = {(int)(e % n), (int)((e / n) % n), (int)(e / ((long)n * n))};
+    const int side = 2 * h + 1;
+    double sum = 0.0;
+    for (int i = 0; i < side * side * side; i++) {
+        int q[3] = {p[0] - (i % side - h), p[1] - ((i / side) % side - h), p[2] - (i / (side * side) - h)};
+        bool inb = true;
+        for (int k = 0; k < 3; k++) {
+            if (periodic) q[k] = ((q[k] % n) + n) % n;
+            if (q[k] < 0 || q[k] >= n) inb = false;
+        }
+        if (!inb) continue;
+        const size_t id = q[0] + ((size_t)q[1] + (size_t)q[2] * n) * pitch;
+        sum += (double)wk[i] * (double)dout[id] / (double)wsum[id];
+    }
+    din[p[0] + ((size_t)p[1] + (size_t)p[2] * n) * pitch] = (float)sum;
+}
+
+// sgm projection (sgm_exp_t, AutoDiff.h:808-823): y = 1 / (1 + exp(-k (x - c))); backward gx = gy k y (1 - y).
+__global__ __launch_bounds__(256) void k_sgm(int n, int pitch, double k, double c, const float* __restrict__ x,
+                                             float* __restrict__ y) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const size_t id = (e % n) + (size_t)(e / n) * pitch;
+    y[id] = (float)(1.0 / (1.0 + exp(-k * ((double)x[id] - c))));
+}
+
+__global__ __launch_bounds__(256) void k_sgm_back(int n, int pitch, double k, const float* __restrict__ y,
+                                                  const float* __restrict__ gy, float* __restrict__ gx) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)n * n * n;
+    if (e >= N) return;
+    const size_t id = (e % n) + (size_t)(e / n) * pitch;
+    const double yv = y[id];
+    gx[id] = (float)((double)gy[id] * k * yv * (1.0 - yv));
+}
+
 // OC update of one cell for multiplier g (update_kernel, OCOptimizer.cu:12-34), FP64 arithmetic rounded
 // to FP32.
 __device__ __forceinline__ float oc_cell(double r0, double sv, double g, double min_rho, double step, double damp) {
@@ -2924,6 +2997,28 @@ void launch_oc_filter(hipStream_t s, int n, int pitch, int h, int periodic, cons
                       const float* rho, float* out) {
     const size_t N = (size_t)n * n * n;
     hipLaunchKernelGGL(k_oc_filter, dim3(nblk(N)), dim3(256), 0, s, n, pitch, h, periodic, wk, sens, rho, out);
+}
+
+void launch_conv_filter(hipStream_t s, int n, int pitch, int h, int periodic, const float* wk, const float* in,
+                        float* out, float* wsum) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_conv_filter, dim3(nblk(N)), dim3(256), 0, s, n, pitch, h, periodic, wk, in, out, wsum);
+}
+
+void launch_conv_filter_back(hipStream_t s, int n, int pitch, int h, int periodic, const float* wk, const float* dout,
+                             const float* wsum, float* din) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_conv_filter_back, dim3(nblk(N)), dim3(256), 0, s, n, pitch, h, periodic, wk, dout, wsum, din);
+}
+
+void launch_sgm(hipStream_t s, int n, int pitch, double k, double c, const float* x, float* y) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_sgm, dim3(nblk(N)), dim3(256), 0, s, n, pitch, k, c, x, y);
+}
+
+void launch_sgm_back(hipStream_t s, int n, int pitch, double k, const float* y, const float* gy, float* gx) {
+    const size_t N = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_sgm_back, dim3(nblk(N)), dim3(256), 0, s, n, pitch, k, y, gy, gx);
 }
 
 void launch_oc_trial(hipStream_t s, int n, int pitch, double g, double min_rho, double step, double damp,

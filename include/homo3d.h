@@ -168,6 +168,17 @@ int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch,
 int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch, double vol, double min_rho,
                      double step, double damp, int on_device);
 
+/* Density filter of the reference's expression templates, rho.conv(radial_convker_t<Spline4 | Linear>(radius))
+ * (AutoDiff/TensorExpression.h:1007-1046, TensorExpression.cuh:288-368): normalised radial weights on the
+ * (2 int(radius + 0.5) + 1)^3 box, renormalised over the in-grid neighbours unless periodic.  kernel 0 =
+ * Spline4, 1 = Linear.  backward = 1 applies the adjoint (in = dJ/dout, out = dJ/din).  in may equal out. */
+int homo3d_density_filter(homo3d_handle* h, const float* in, float* out, int pitch, double radius, int kernel,
+                          int periodic, int backward, int on_device);
+/* Tensor::sgm(k, c) (AutoDiff.h:808-823): out = 1 / (1 + exp(-k (in - c))); backward = 1: y = forward output,
+ * in = dJ/dy, out = dJ/dx = in k y (1 - y). */
+int homo3d_projection(homo3d_handle* h, const float* in, const float* y, float* out, int pitch, double k, double c,
+                      int backward, int on_device);
+
 /* homo3d_oc_update without any allocation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  sens and rho are device pointers (lexicographic, row pitch `pitch`, rho updated in

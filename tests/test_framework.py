@@ -57,7 +57,7 @@ def test_example_opti_bulk_oc(tmp_path, orc):
 
 
 @pytest.mark.gpu
-def test_example_opti_npr_mma_and_shear(tmp_path):
+def test_example_opti_npr_mma_and_shear(tmp_path, orc):
     pre = str(tmp_path) + "/npr_"
     rc = run_main(["-reso=32", "-obj=npr", "-optimizer=mma", "-sym=rotate3", "-vol=0.2", "-N=4", "-solver=pcg",
                    "-relthres=1e-5", "-prefix=" + pre])
@@ -65,10 +65,21 @@ def test_example_opti_npr_mma_and_shear(tmp_path):
     rho = np.fromfile(pre + "rho", dtype=np.float32)
     assert rho.min() >= 1e-3 - 1e-7 and rho.max() <= 1.0 + 1e-7
     pre2 = str(tmp_path) + "/shear_"
-    rc = run_main(["-reso=32", "-obj=shear", "-sym=reflect6", "-vol=0.3", "-N=3", "-solver=pcg", "-relthres=1e-4",
-                   "-prefix=" + pre2])
+    rc = run_main(["-reso=32", "-obj=shear", "-sym=reflect6", "-vol=0.3", "-N=3", "-solver=pcg", "-relthres=1e-6",
+                   "-logc=1", "-prefix=" + pre2])
     assert rc == 0
     assert np.isfinite(np.loadtxt(pre2 + "objlist")).all()
+    # the solved density is sgm(conv(rho, Spline4(2)))^3 of the design (Framework.cu:288): its C^H
+    # from the FP64 oracle on that pipeline equals the driver's first C^H
+    from oracle import filter_oracle as F
+
+    rho0 = np.fromfile(pre2 + "initRho", dtype=np.float32).astype(np.float64).reshape(32, 32, 32)
+    rhat = F.sgm(F.conv(rho0, 2.0, periodic=False)[0])
+    mg = orc.MG(32)
+    mg.set_density((rhat ** 3).ravel())
+    Cref, _, _, _ = mg.homogenize(tol=1e-10, maxit=400, mode="pcg")
+    C0 = read_clog(pre2 + "Clog")[0]
+    assert np.abs(C0 - Cref).max() <= 1e-4 * np.abs(Cref).max()
 
 
 def test_shear_isotropy_terms_follow_reference_scaling():
