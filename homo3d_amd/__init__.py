@@ -17,9 +17,9 @@ import numpy as np
 
 from .api import (HOMO3D_OK, HOMO3D_NOT_CONVERGED, HOMO3D_DIVERGED, HOMO3D_BAD_ARG, HOMO3D_HIP_ERROR,
                   HOMO3D_NO_SUPPORT, SOLVER_MG, SOLVER_PCG, FIELD_U, FIELD_F, FIELD_R, SYM, Homo3dError,
-                  lib, lib_path, DeviceBuffer, HomoConfig, Homogenization, OCOptimizer, MMAOptimizer, WarmStartNet, randc_density)
+                  lib, lib_path, DeviceBuffer, HomoConfig, Homogenization, OCOptimizer, MMAOptimizer, WarmStartNet, Batch, randc_density)
 
 __all__ = ["HOMO3D_OK", "HOMO3D_NOT_CONVERGED", "HOMO3D_DIVERGED", "HOMO3D_BAD_ARG", "HOMO3D_HIP_ERROR",
            "HOMO3D_NO_SUPPORT", "SOLVER_MG", "SOLVER_PCG", "FIELD_U", "FIELD_F", "FIELD_R", "SYM",
-           "Homo3dError", "lib", "lib_path", "DeviceBuffer", "HomoConfig", "Homogenization", "OCOptimizer", "MMAOptimizer", "WarmStartNet",
+           "Homo3dError", "lib", "lib_path", "DeviceBuffer", "HomoConfig", "Homogenization", "OCOptimizer", "MMAOptimizer", "WarmStartNet", "Batch",
            "randc_density"]

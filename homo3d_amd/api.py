@@ -39,6 +39,10 @@ class _NetConfig(ctypes.Structure):
                 ("device", ctypes.c_int)]
 
 
+class _TuneReport(ctypes.Structure):
+    _fields_ = [("row_ms", ctypes.c_double), ("mfma_ms", ctypes.c_double), ("chosen_mfma", ctypes.c_int)]
+
+
 class _Stats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int * 6), ("relres", ctypes.c_double * 6),
                 ("seconds", ctypes.c_double)]
@@ -55,6 +59,7 @@ EXPORTS = {
     "homo3d_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_Stats)]),
     "homo3d_elastic_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_residual_norms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "homo3d_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_TuneReport)]),
     "homo3d_sensitivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                           ctypes.c_int, ctypes.c_int]),
     "homo3d_sensitivity_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
@@ -121,6 +126,13 @@ EXPORTS = {
                                         ctypes.c_uint64, ctypes.c_void_p]),
     "homo3d_net_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_net_warm_start": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "homo3d_batch_create": (ctypes.c_int, [ctypes.POINTER(_Config), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "homo3d_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "homo3d_batch_handle": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_double,
+                                          ctypes.c_int, ctypes.POINTER(_Stats), ctypes.POINTER(ctypes.c_double)]),
     "homo3d_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_profile_only": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "homo3d_profile_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long),
@@ -346,6 +358,12 @@ class Homogenization:
         C = (ctypes.c_double * 36)()
         self._chk(lib().homo3d_elastic_matrix(self._h, C), "elasticMatrix")
         return np.array(C[:]).reshape(6, 6)
+
+    def autotune(self):
+        """Time the finest-level operator's tile forms on this grid and density; keep the faster."""
+        r = _TuneReport()
+        self._chk(lib().homo3d_autotune(self._h, ctypes.byref(r)), "autotune")
+        return {"row_ms": r.row_ms, "mfma_ms": r.mfma_ms, "chosen": "mfma" if r.chosen_mfma else "row"}
 
     def residual_norms(self):
         """FP64 relres per load case of the current FP64 iterate (no solve)."""
@@ -665,3 +683,43 @@ class WarmStartNet:
                   target_scale=float(d["target_scale"]), device=device)
         net.set_params(d["params"])
         return net
+
+
+class Batch:
+    """Batched uneven designs (homo3d_batch_*): one handle per design, solved concurrently."""
+
+    def __init__(self, sizes, config=None, max_inflight=8):
+        cfg = (config or HomoConfig(solver="pcg"))._c()
+        self.sizes = [int(n) for n in sizes]
+        arr = (ctypes.c_int * len(self.sizes))(*self.sizes)
+        self._h = ctypes.c_void_p()
+        rc = lib().homo3d_batch_create(ctypes.byref(cfg), len(self.sizes), arr, int(max_inflight), ctypes.byref(self._h))
+        if rc:
+            raise Homo3dError(rc, "homo3d_batch_create")
+
+    def close(self):
+        if self._h:
+            lib().homo3d_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, xs, tol=1e-5, autotune=False):
+        """xs: per-design arrays x = rho^3 (n_i^3 floats).  Returns (C [count, 6, 6], list of stats dicts)."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float32).ravel() for x in xs]
+        if len(arrs) != len(self.sizes) or any(a.size != n ** 3 for a, n in zip(arrs, self.sizes)):
+            raise ValueError("one n^3 density per design expected")
+        ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        st = (_Stats * len(arrs))()
+        C = np.zeros((len(arrs), 36))
+        rc = lib().homo3d_batch_solve(self._h, ptrs, float(tol), int(bool(autotune)), st,
+                                      C.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc and rc != HOMO3D_NOT_CONVERGED:
+            raise Homo3dError(rc, "batch_solve: %s" % lib().homo3d_batch_last_error(self._h).decode())
+        stats = [{"status": s.status, "iters": list(s.iters), "relres": list(s.relres), "seconds": s.seconds}
+                 for s in st]
+        return C.reshape(-1, 6, 6), stats

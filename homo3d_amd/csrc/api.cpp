@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -12,6 +13,7 @@
 #include <limits>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/homo3d.h"
@@ -67,7 +69,8 @@ struct homo3d_handle {
     float* d_ke = nullptr;
     void* d_ke16 = nullptr;     // element matrix as matrix-core operands (ke_mfma_table), or null
     FineMfma mfma{};            // its launch parameters
-    const FineMfma* mf() const { return d_ke16 ? &mfma : nullptr; }
+    bool use_mfma = false;      // run the finest-level operator in element form (config or homo3d_autotune)
+    const FineMfma* mf() const { return d_ke16 && use_mfma ? &mfma : nullptr; }
     float* d_kchi = nullptr;
     float* d_G = nullptr;
     float* d_w = nullptr;
@@ -831,7 +834,10 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
         hipMalloc(&h->d_gflip, sizeof(int) * 48))
         return fail(HOMO3D_HIP_ERROR);
     double kl[576], km[576];
-    if (cfg->fine_mfma && lame_tables(kl, km)) {
+    // the matrix-core operand table is built whenever the element form can run, so that homo3d_autotune
+    // can choose it; cfg->fine_mfma selects it from the start
+    h->use_mfma = cfg->fine_mfma != 0;
+    if (fine_apply_mfma_ok(n) && lame_tables(kl, km)) {
         std::vector<unsigned char> t16(kKeMfmaBytes);
         double lam, mu;
         lame_params(cfg->youngs_modulus, cfg->poisson_ratio, lam, mu);
@@ -1079,6 +1085,54 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
     }
     if (status == HOMO3D_DIVERGED) set_err(h, status, "MG numerical explode");
     return status;
+}
+
+// Autotuner of the finest-level operator's tile form: the row form (512-thread workgroups over
+// 128 x 2 x 2 parity tiles, neighbour gathers) against the element form on the matrix cores
+// (32 x 4 x 4 vertex tiles, one 16x16x32 MMA per cell group), timed with HIP events on this handle's
+// grid and density (r = K z on the packed-FP16 iterate, the form every CG pass and V-cycle residual
+// runs); the faster one is used from then on.
+int homo3d_autotune(homo3d_handle* h, homo3d_tune_report* rep) {
+    if (!h) return HOMO3D_BAD_ARG;
+    int rc = require_density(h);
+    if (rc) return rc;
+    const Level& L0 = h->lv[0];
+    homo3d_tune_report r{};
+    r.row_ms = r.mfma_ms = -1;
+    const bool half = L0.n % 8 == 0;
+    const bool can_mfma = h->d_ke16 != nullptr && half;
+    if (!half) {
+        h->use_mfma = false;
+        if (rep) *rep = r;
+        return HOMO3D_OK;
+    }
+    launch_pack_half(h->stream, L0.N, h->X, h->Z);  // a defined operand (the current iterate)
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    auto time_form = [&](bool mfma_form, double* ms) -> int {
+        h->use_mfma = mfma_form;
+        for (int i = 0; i < 3; i++) launch_fine_apply(h->stream, L0.n, h->d_ke, h->mf(), h->d_w, h->Z, 1, nullptr, L0.r);
+        HIPCHK(h, hipEventRecord(e0, h->stream));
+        const int reps = 10;
+        for (int i = 0; i < reps; i++)
+            launch_fine_apply(h->stream, L0.n, h->d_ke, h->mf(), h->d_w, h->Z, 1, nullptr, L0.r);
+        HIPCHK(h, hipEventRecord(e1, h->stream));
+        HIPCHK(h, hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(h, hipEventElapsedTime(&t, e0, e1));
+        *ms = t / reps;
+        return HOMO3D_OK;
+    };
+    rc = time_form(false, &r.row_ms);
+    if (!rc && can_mfma) rc = time_form(true, &r.mfma_ms);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    h->use_mfma = can_mfma && r.mfma_ms < r.row_ms;
+    r.chosen_mfma = h->use_mfma ? 1 : 0;
+    if (rep) *rep = r;
+    return HOMO3D_OK;
 }
 
 // FP64 relres ||f - K u|| / ||f|| per load case of the current FP64 iterate (the caller's level-0 u when
@@ -1660,6 +1714,93 @@ int homo3d_profile_get(homo3d_handle* h, const char* kernel, long* launches, dou
             return HOMO3D_OK;
         }
     return set_err(h, HOMO3D_BAD_ARG, "unknown profiled kernel %s", kernel);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched uneven designs: one handle (and stream) per design; up to max_inflight designs are driven
+// at once by host threads, so the small grids' launches overlap on the GPU.
+// ---------------------------------------------------------------------------------------------
+struct homo3d_batch {
+    std::vector<homo3d_handle*> h;
+    std::vector<int> tuned;
+    int device = 0;
+    int inflight = 8;
+    std::string err;
+};
+
+int homo3d_batch_create(const homo3d_config* base, int count, const int* sizes, int max_inflight,
+                        homo3d_batch** out) {
+    if (!base || !sizes || !out || count < 1 || max_inflight < 1) return HOMO3D_BAD_ARG;
+    *out = nullptr;
+    homo3d_batch* b = new homo3d_batch();
+    b->device = base->device;
+    b->inflight = std::min(max_inflight, 64);
+    for (int i = 0; i < count; i++) {
+        homo3d_config c = *base;
+        c.reso[0] = c.reso[1] = c.reso[2] = sizes[i];
+        homo3d_handle* h = nullptr;
+        const int rc = homo3d_create(&c, &h);
+        if (rc) {
+            homo3d_batch_destroy(b);
+            return rc;
+        }
+        b->h.push_back(h);
+    }
+    b->tuned.assign(count, 0);
+    *out = b;
+    return HOMO3D_OK;
+}
+
+int homo3d_batch_destroy(homo3d_batch* b) {
+    if (!b) return HOMO3D_OK;
+    for (homo3d_handle* h : b->h) homo3d_destroy(h);
+    delete b;
+    return HOMO3D_OK;
+}
+
+const char* homo3d_batch_last_error(const homo3d_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+homo3d_handle* homo3d_batch_handle(homo3d_batch* b, int i) {
+    return b && i >= 0 && i < (int)b->h.size() ? b->h[i] : nullptr;
+}
+
+int homo3d_batch_solve(homo3d_batch* b, const float* const* x, double tol, int autotune, homo3d_solve_stats* stats,
+                       double* C) {
+    if (!b || !x) return HOMO3D_BAD_ARG;
+    const int count = (int)b->h.size();
+    std::atomic<int> next{0};
+    std::vector<int> status(count, HOMO3D_OK);
+    auto worker = [&]() {
+        if (hipSetDevice(b->device) != hipSuccess) return;
+        for (int i = next++; i < count; i = next++) {
+            homo3d_handle* h = b->h[i];
+            const int n = h->lv[0].n;
+            int rc = homo3d_update_density(h, x[i], n, -1);
+            if (!rc && autotune && !b->tuned[i]) {
+                rc = homo3d_autotune(h, nullptr);
+                b->tuned[i] = 1;
+            }
+            homo3d_solve_stats st{};
+            if (!rc) rc = homo3d_solve(h, tol, 0, &st);
+            if (rc == HOMO3D_OK || rc == HOMO3D_NOT_CONVERGED) {
+                const int rc2 = C ? homo3d_elastic_matrix(h, C + 36 * (size_t)i) : HOMO3D_OK;
+                if (rc2) rc = rc2;
+            }
+            if (stats) stats[i] = st;
+            status[i] = rc;
+        }
+    };
+    const int nt = std::min(b->inflight, count);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    int worst = HOMO3D_OK;
+    for (int i = 0; i < count; i++)
+        if (status[i] > worst) {
+            worst = status[i];
+            b->err = "design " + std::to_string(i) + ": " + b->h[i]->err;
+        }
+    return worst;
 }
 
 }  // extern "C"

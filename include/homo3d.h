@@ -118,6 +118,16 @@ void* homo3d_stream(homo3d_handle* h);
 int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_device);
 int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* stats);
 int homo3d_elastic_matrix(homo3d_handle* h, double C[36]);
+/* Autotuner (no reference counterpart): times the finest-level operator's compiled tile forms on this
+ * handle's grid and density (row form: 512-thread workgroups over 128 x 2 x 2 parity tiles; element form
+ * on the matrix cores: 32 x 4 x 4 vertex tiles, needs n % 32 == 0) and keeps the faster for every later
+ * CG pass and V-cycle residual.  Times of -1 mean the form cannot run at this n. */
+typedef struct {
+    double row_ms;   /* average r = K z (packed FP16, six load cases) in the row form */
+    double mfma_ms;  /* the same in element form on the matrix cores */
+    int chosen_mfma; /* 1 when the element form was chosen */
+} homo3d_tune_report;
+int homo3d_autotune(homo3d_handle* h, homo3d_tune_report* report);
 /* FP64 relres ||f - K u|| / ||f|| per load case of the refined solver's current iterate, without solving
  * (Grid::update_residual with the norms of MG::solveEquation, MG.cpp:113-203, in FP64). */
 int homo3d_residual_norms(homo3d_handle* h, double relres[6]);
@@ -253,6 +263,20 @@ int homo3d_net_eval(homo3d_net* net, int first, int count, double* loss);
 /* Predict the solution of the handle's current density and make it the FP64 iterate that the next warm
  * solve (homo3d_solve with warm = 1) starts from.  n % grid == 0. */
 int homo3d_net_warm_start(homo3d_handle* h, homo3d_net* net);
+
+/* Batched uneven designs (no reference counterpart: the reference solves one design per Homogenization,
+ * homogenization.cu:464-575).  One handle and stream per design (sizes[i]^3, each as homo3d_create with the
+ * base configuration); homo3d_batch_solve drives up to max_inflight designs at once from host threads, so the
+ * small grids' launches overlap on the GPU.  x[i]: design i's density (x = rho^p, unpitched, host or device
+ * memory); stats[i] and C[36 i .. 36 i + 35] receive its solve statistics and C^H (either may be NULL).
+ * autotune = 1 runs homo3d_autotune on each design's first solve.  Returns the worst status. */
+typedef struct homo3d_batch homo3d_batch;
+int homo3d_batch_create(const homo3d_config* base, int count, const int* sizes, int max_inflight, homo3d_batch** out);
+int homo3d_batch_destroy(homo3d_batch* batch);
+const char* homo3d_batch_last_error(const homo3d_batch* batch);
+homo3d_handle* homo3d_batch_handle(homo3d_batch* batch, int i);
+int homo3d_batch_solve(homo3d_batch* batch, const float* const* x, double tol, int autotune, homo3d_solve_stats* stats,
+                       double* C);
 
 /* Kernel timing: enable event pairs around the finest-level kernel launches, then read totals per
  * kernel: "fine_gs" (full GS colour passes), "fine_gs_zero" (colour passes of sweeps that start from
