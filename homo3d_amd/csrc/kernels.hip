@@ -308,11 +308,37 @@ __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke
     if constexpr (DN < 27) gather_step<R, MASK, DN, T, A, XL>(c, n, ke, ru, plane, r0, nxt, acc, xl);
 }
 
+// The same walk with the loads issued two neighbours ahead (cur: this neighbour, nx1: the next one,
+// already in flight): more loads in flight per wave for the latency-bound gather.
+template <int R, uint32_t MASK, int D, typename T, typename A>
+__device__ __forceinline__ void gather_step2(const FineCtx& c, int n, const T* ke, Rsrc ru, uint32_t plane, int r0,
+                                             const A (&cur)[R / CasesPerWord<A>::value][3],
+                                             const A (&nx1)[R / CasesPerWord<A>::value][3], T (&acc)[R][3]) {
+    constexpr int W = R / CasesPerWord<A>::value;
+    constexpr int DN = next_nb(MASK, D);
+    constexpr int DN2 = DN < 27 ? next_nb(MASK, DN) : 27;
+    __builtin_amdgcn_sched_barrier(0);
+    A nx2[W][3];
+    if constexpr (DN2 < 27) load_nb<W, A>(nx2, ru, c.nb(DN2) * (uint32_t)sizeof(A), plane, r0);
+    T S[9];
+    if constexpr (sizeof(T) == 8)
+        fine_block<D>(c.wd, ke, S);
+    else
+        fine_block<D>(c.wc, ke, S);
+    block_fma<W, T, A>(S, cur, acc);
+    if constexpr (DN < 27) {
+        if constexpr (DN2 < 27)
+            gather_step2<R, MASK, DN, T, A>(c, n, ke, ru, plane, r0, nx1, nx2, acc);
+        else
+            gather_step2<R, MASK, DN, T, A>(c, n, ke, ru, plane, r0, nx1, nx1, acc);
+    }
+}
+
 // acc[r][i] = sum_d S_d u_{v+d} over the neighbours d selected by MASK, load cases r0..r0+R-1.
 // ke points to the element matrix staged in LDS.  The pinned origin's entries of u are zero by
 // construction (every entry point zeroes them before the fine operator reads u).  XL: the two
 // x-neighbours come from LDS (xl).
-template <int R, uint32_t MASK, typename T, typename A, bool XL = false>
+template <int R, uint32_t MASK, typename T, typename A, bool XL = false, int PF = 1>
 __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, const T* ke, const A* __restrict__ u,
                                             T (&acc)[R][3], int r0 = 0, const XNbLds<A>& xl = XNbLds<A>()) {
     constexpr int W = R / CasesPerWord<A>::value;
@@ -326,7 +352,13 @@ __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, c
     if constexpr (D0 < 27) {
         A first[W][3];
         load_nb_at<W, A, XL, D0>(first, c, ru, plane, r0, xl);
-        gather_step<R, MASK, D0, T, A, XL>(c, n, ke, ru, plane, r0, first, acc, xl);
+        if constexpr (PF == 2 && !XL && next_nb(MASK, D0) < 27) {
+            A second[W][3];
+            load_nb<W, A>(second, ru, c.nb(next_nb(MASK, D0)) * (uint32_t)sizeof(A), plane, r0);
+            gather_step2<R, MASK, D0, T, A>(c, n, ke, ru, plane, r0, first, second, acc);
+        } else {
+            gather_step<R, MASK, D0, T, A, XL>(c, n, ke, ru, plane, r0, first, acc, xl);
+        }
     }
 }
 
@@ -375,7 +407,7 @@ __device__ __forceinline__ uint32_t colour_block(int colour) { return (uint32_t)
 // PX (with DOT, u = z): the CG vector updates of the step, fused: p = z (PX 1, first step) or
 // x += alpha p_old, p = z + beta p_old (PX 2), per load case with alpha, beta from the device; the dot
 // is then p . out with the new p (dotv unused).  Load cases that have converged carry alpha = beta = 0.
-template <int R, int MODE, bool DOT, typename A, bool BETA = false, int PX = 0>
+template <int R, int MODE, bool DOT, typename A, bool BETA = false, int PX = 0, int PF = 1>
 __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N, const float* __restrict__ ke,
                                                                 const float* __restrict__ w,
                                                                 const A* __restrict__ u,
@@ -395,7 +427,7 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
     FineCtx c;
     fine_ctx(c, x, y, z, L, w);
     float acc[R][3];
-    fine_gather<R, kRowAll, float, A>(c, L.n, N, ke_s, u, acc);
+    fine_gather<R, kRowAll, float, A, false, PF>(c, L.n, N, ke_s, u, acc);
     double dot[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -2461,12 +2493,14 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf
                            nullptr, nullptr, nullptr, nullptr);
 }
 
+constexpr int kCgPrefetch = 2;  // neighbours whose loads are in flight ahead of the CG pass's block products
 template <typename A, bool FIRST>
 static void fine_apply_cg_row(hipStream_t s, const Lay& L, size_t N, const float* ke, const float* w, const A* z,
                               float* p, float* x, const double* sc, float* q, double* partial) {
     constexpr int R = 6;
-    hipLaunchKernelGGL((k_fine_apply<R, 0, true, A, !FIRST, FIRST ? 1 : 2>), dim3(full_grid(L)), dim3(kFullThreads), 0,
-                       s, L, N, ke, w, z, nullptr, q, partial, nullptr, sc + 4 * R, p, x, sc + 2 * R);
+    hipLaunchKernelGGL((k_fine_apply<R, 0, true, A, !FIRST, FIRST ? 1 : 2, kCgPrefetch>), dim3(full_grid(L)),
+                       dim3(kFullThreads), 0, s, L, N, ke, w, z, nullptr, q, partial, nullptr, sc + 4 * R, p, x,
+                       sc + 2 * R);
 }
 
 void launch_fine_apply_cg(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* z,
