@@ -19,10 +19,18 @@
 //                             symmetry under reflect6; SPD.
 //   orc_sensitivity           Euler homogeneity sum_e w_e dC/dx_e = C; central differences.
 //   orc_symmetrize_*          projection: idempotent, invariant under every group element.
-//   orc_oc_filter/update      uniform-field invariants; volume within 1e-4; box/move limits.
-//   orc_mma_update            KKT of the convex toy min sum x s.t. sum 1/x <= c.
+//   orc_oc_filter/update      uniform-field invariants; volume within 1e-4; box/move limits;
+//                             the power law rho'/rho = (-s/g)^damp; the B = 0.01 branch for
+//                             positive sensitivities; the hand-computed filter impulse table.
+//   orc_mma_update            KKT of the convex toy min sum x s.t. sum 1/x <= c; closed-form
+//                             asymptotes (0.5 span, then 1.2 / 0.7 / 1.0 with their clamps);
+//                             the subproblem optimum against an independent SLSQP solve.
 //   orc_philox4x32_10         Random123 known-answer vectors.
-//   orc_randc_density         parity unpinned beyond its definition (project generator).
+//   orc_randc_coeffs/field    coefficients: standard-normal moments (mean, variance, fourth
+//                             moment 3), seed determinism; field: a single-coefficient design
+//                             against the closed form sigmoid(3 cos sin cos / 8) scaled to the
+//                             volume by an independent root finder.
+//   orc_randc_density         the composition of the two above plus symmetry projection.
 #pragma once
 #include <stdint.h>
 
