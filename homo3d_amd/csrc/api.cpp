@@ -588,6 +588,19 @@ int stage_in(homo3d_handle* h, const float* a, int pitch, int on_device, const f
     return HOMO3D_OK;
 }
 
+// Return a pitched lexicographic result: only the n live columns of each row are written, so the
+// caller's padding (pitch > n) keeps its values, as the reference's pitched tensor views do.
+int stage_out(homo3d_handle* h, float* dst, const float* src, int pitch, int on_device) {
+    const int n = h->lv[0].n;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (pitch == n)
+        HIPCHK(h, hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n * n * n, kind, h->stream));
+    else
+        HIPCHK(h, hipMemcpy2DAsync(dst, sizeof(float) * pitch, src, sizeof(float) * pitch, sizeof(float) * n,
+                                   (size_t)n * n, kind, h->stream));
+    return HOMO3D_OK;
+}
+
 constexpr int kMaxOuter = 12;
 // Inner solve target per refinement round (measured at 256^3: 1e-4 and 1e-6 cost more CG steps than
 // they save in FP64 residual rounds; 3e-3 and 1e-2 need a third round).
@@ -1201,7 +1214,13 @@ static int sensitivity(homo3d_handle* h, const float dC[36], float* sens, int pi
         dst = h->d_stage;
     }
     launch_sensitivity(h->stream, n, pitch, h->d_ke, h->X, h->d_ftmp, dst);
-    if (!on_device) HIPCHK(h, hipMemcpyAsync(sens, dst, sizeof(float) * elems, hipMemcpyDeviceToHost, h->stream));
+    if (!on_device) {
+        if (lexi_order)
+            rc = stage_out(h, sens, dst, pitch, 0);
+        else
+            HIPCHK(h, hipMemcpyAsync(sens, dst, sizeof(float) * elems, hipMemcpyDeviceToHost, h->stream));
+        if (rc) return rc;
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
@@ -1565,8 +1584,8 @@ int homo3d_density_filter(homo3d_handle* h, const float* in, float* out, int pit
         launch_conv_filter(h->stream, n, pitch, hw, periodic, h->d_ftmp, din, dres, dws);  // weight sums W
         launch_conv_filter_back(h->stream, n, pitch, hw, periodic, h->d_ftmp, din, dws, dres);
     }
-    HIPCHK(h, hipMemcpyAsync(out, dres, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                             h->stream));
+    rc = stage_out(h, out, dres, pitch, on_device);
+    if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
@@ -1591,8 +1610,8 @@ int homo3d_projection(homo3d_handle* h, const float* in, const float* y, float* 
     } else {
         launch_sgm(h->stream, n, pitch, k, c, din, dres);
     }
-    HIPCHK(h, hipMemcpyAsync(out, dres, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                             h->stream));
+    rc = stage_out(h, out, dres, pitch, on_device);
+    if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }
@@ -1634,8 +1653,8 @@ int homo3d_oc_update(homo3d_handle* h, const float* sens, float* rho, int pitch,
         else
             break;
     }
-    HIPCHK(h, hipMemcpyAsync(rho, dn, sizeof(float) * elems, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                             h->stream));
+    rc = stage_out(h, rho, dn, pitch, on_device);
+    if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return HOMO3D_OK;
 }

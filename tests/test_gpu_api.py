@@ -142,3 +142,60 @@ def test_warm_start_after_level0_prolongation(h3d):
     assert sum(st["iters"]) > 0  # it started from the perturbed field
     # u itself is barely determined in the near-void cells (stiffness ~1e-9), C^H is not
     assert rel(hom.elasticMatrix(solve=False), C0) < 1e-4
+
+
+def test_pitched_padding_is_left_alone(h3d):
+    """Every entry point that takes a row-pitched array writes only the n live columns: the caller's
+    padding (pitch > n) keeps its values, on host and on device arrays, after earlier calls have left
+    other data in the library's staging memory.  The live columns equal the pitch = n results."""
+    import ctypes
+
+    n, pitch, sentinel = 16, 24, np.float32(7.25)
+    L = h3d.lib()
+    hom, x = make(h3d, n, solver="pcg")
+    hom.solve(1e-5, warm=False)
+    rng = np.random.default_rng(3)
+    rho = (0.2 + 0.6 * rng.random((n, n, n))).astype(np.float32)
+    sens = (-rng.random((n, n, n))).astype(np.float32)
+    dC = np.eye(6, dtype=np.float32).ravel()
+    dCp = dC.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def pad(a):
+        p = np.full((n, n, pitch), sentinel, np.float32)
+        p[:, :, :n] = a
+        return p
+
+    def calls(pt):
+        """name -> function(args dict of pitched arrays as pointers) for pitch pt."""
+        return {
+            "oc_update": lambda a, dev: L.homo3d_oc_update(hom._h, a["sens"], a["rho"], pt, 0.5, 1e-3, 0.05, 0.5, dev),
+            "oc_filter": lambda a, dev: L.homo3d_oc_filter(hom._h, a["sens"], a["rho"], pt, 2.0, 0, dev),
+            "filter": lambda a, dev: L.homo3d_density_filter(hom._h, a["rho"], a["out"], pt, 2.0, 0, 0, 0, dev),
+            "filter_back": lambda a, dev: L.homo3d_density_filter(hom._h, a["sens"], a["out"], pt, 2.0, 0, 0, 1, dev),
+            "sgm": lambda a, dev: L.homo3d_projection(hom._h, a["rho"], None, a["out"], pt, 30.0, 0.5, 0, dev),
+            "sgm_back": lambda a, dev: L.homo3d_projection(hom._h, a["sens"], a["rho"], a["out"], pt, 30.0, 0.5, 1,
+                                                           dev),
+            "symmetrize": lambda a, dev: L.homo3d_symmetrize(hom._h, a["rho"], pt, 2, dev),
+            "sensitivity": lambda a, dev: L.homo3d_sensitivity(hom._h, dCp, a["out"], pt, dev),
+        }
+
+    outputs = {"oc_update": "rho", "oc_filter": "sens", "filter": "out", "filter_back": "out", "sgm": "out",
+               "sgm_back": "out", "symmetrize": "rho", "sensitivity": "out"}
+    for name in outputs:
+        # the unpitched result
+        flat = {"rho": rho.copy(), "sens": sens.copy(), "out": np.zeros_like(rho)}
+        assert calls(n)[name]({k: v.ctypes.data_as(ctypes.c_void_p) for k, v in flat.items()}, 0) == 0, name
+        want = flat[outputs[name]]
+        for dev in (0, 1):
+            arrs = {"rho": pad(rho), "sens": pad(sens), "out": np.full((n, n, pitch), sentinel, np.float32)}
+            if dev:
+                bufs = {k: h3d.DeviceBuffer.from_array(v) for k, v in arrs.items()}
+                assert calls(pitch)[name]({k: b.value for k, b in bufs.items()}, 1) == 0, name
+                got = bufs[outputs[name]].download(np.float32).reshape(n, n, pitch)
+            else:
+                assert calls(pitch)[name]({k: v.ctypes.data_as(ctypes.c_void_p) for k, v in arrs.items()}, 0) == 0
+                got = arrs[outputs[name]]
+            say("  %s on_device=%d: padding %s" % (name, dev, "kept" if np.all(got[:, :, n:] == sentinel) else "WRITTEN"))
+            assert np.all(got[:, :, n:] == sentinel), (name, dev)
+            assert np.array_equal(got[:, :, :n], want), (name, dev)
+    hom.close()
