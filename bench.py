@@ -30,7 +30,8 @@ PROF_KINDS = ("fine_gs", "fine_gs_zero", "fine_apply", "fine_apply_first", "fine
 # instantiations), for the traffic lookup
 PROF_KERNEL = {"fine_gs": ("k_fine_gs2<6, true, -1", "k_fine_gs2ILi6ELb1ELin1E", "k_fine_gs<6, true, -1",
                            "k_fine_gsILi6ELb1ELin1E"),
-               "fine_apply": ("k_fine_applyILi6ELi0ELb1EDv2_DF16_Lb1ELi2E", "k_fine_apply_efILi0ELb1ELb1ELi1ELi2E",
+               "fine_apply": ("k_fine_cg_stagedILb1ELi2E", "k_fine_cg_staged<true, 2>",
+                              "k_fine_applyILi6ELi0ELb1EDv2_DF16_Lb1ELi2E", "k_fine_apply_efILi0ELb1ELb1ELi1ELi2E",
                               "k_fine_apply_efILi0ELb1ELb1ELi2ELi2E", "k_fine_apply<6, 0, true, half2, true, 2"),
                "fine_residual": ("k_fine_apply<6, 1, false", "k_fine_applyILi6ELi1ELb0E")}
 DOMINANT = "fine_apply"  # the kernel with the largest total time per step

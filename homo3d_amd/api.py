@@ -59,6 +59,7 @@ EXPORTS = {
     "homo3d_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_Stats)]),
     "homo3d_elastic_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_residual_norms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "homo3d_stage_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "homo3d_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_TuneReport)]),
     "homo3d_sensitivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                           ctypes.c_int, ctypes.c_int]),
@@ -364,6 +365,10 @@ class Homogenization:
         r = _TuneReport()
         self._chk(lib().homo3d_autotune(self._h, ctypes.byref(r)), "autotune")
         return {"row_ms": r.row_ms, "mfma_ms": r.mfma_ms, "chosen": "mfma" if r.chosen_mfma else "row"}
+
+    def stage_tiles(self, on=True):
+        """CG pass: stage each workgroup's z node box in LDS (True, the default) or gather from memory."""
+        self._chk(lib().homo3d_stage_tiles(self._h, int(bool(on))), "stage_tiles")
 
     def residual_norms(self):
         """FP64 relres per load case of the current FP64 iterate (no solve)."""

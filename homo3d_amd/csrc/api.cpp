@@ -71,6 +71,7 @@ struct homo3d_handle {
     FineMfma mfma{};            // its launch parameters
     bool use_mfma = false;      // run the finest-level operator in element form (config or homo3d_autotune)
     const FineMfma* mf() const { return d_ke16 && use_mfma ? &mfma : nullptr; }
+    bool stage_tiles = true;    // CG pass: z node boxes staged in LDS where fine_cg_staged_ok (homo3d_stage_tiles)
     float* d_kchi = nullptr;
     float* d_G = nullptr;
     float* d_w = nullptr;
@@ -652,8 +653,8 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
         {
             ProfEvent ev;
             prof_begin(h, it ? kProfApply : kProfApplyFirst, ev);
-            launch_fine_apply_cg(h->stream, L0.n, h->d_ke, h->mf(), h->d_w, h->Z, zh, h->P, h->X, h->d_sc, it == 0,
-                                 h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
+            launch_fine_apply_cg(h->stream, L0.n, h->d_ke, h->mf(), h->d_w, h->Z, zh, h->stage_tiles ? 1 : 0, h->P,
+                                 h->X, h->d_sc, it == 0, h->Q, h->d_partial, kDotBlocks, h->d_sc + kRhs);  // pq
             prof_end(h, ev);
         }
         launch_pcg_alpha(h->stream, h->d_sc);
@@ -1150,6 +1151,12 @@ int homo3d_autotune(homo3d_handle* h, homo3d_tune_report* rep) {
 
 // FP64 relres ||f - K u|| / ||f|| per load case of the current FP64 iterate (the caller's level-0 u when
 // that was written last), without solving: the measure a warm start is judged by.
+int homo3d_stage_tiles(homo3d_handle* h, int on) {
+    if (!h) return HOMO3D_BAD_ARG;
+    h->stage_tiles = on != 0;
+    return HOMO3D_OK;
+}
+
 int homo3d_residual_norms(homo3d_handle* h, double relres[6]) {
     if (!h || !relres) return HOMO3D_BAD_ARG;
     int rc = require_density(h);

@@ -30,9 +30,12 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf
 //   first: p = z, q = K p;  else: x += alpha p_old, p = z + beta p_old, q = K z + beta q_old (= K p),
 // and the per-load-case FP64 dot p . q -> out6.  partial holds 6 * n^3 / 512 doubles (and at least
 // 6 * dot_blocks).
+// staged (packed FP16 z, fine_cg_staged_ok(n)): each workgroup stages its z node box in LDS instead of
+// gathering the neighbours from memory; the results are the same bit for bit.
+int fine_cg_staged_ok(int n);
 void launch_fine_apply_cg(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* z,
-                          int z_half, float* p, float* x, const double* sc, int first, float* q, double* partial,
-                          int dot_blocks, double* out6);
+                          int z_half, int staged, float* p, float* x, const double* sc, int first, float* q,
+                          double* partial, int dot_blocks, double* out6);
 // zero_start: colour pass of a forward sweep that starts from u = 0 (u need not be cleared)
 void launch_fine_gs(hipStream_t s, int n, int colour, float omega, int reverse, int zero_start, const float* ke,
                     const float* w, void* u, int u_half, const float* f);

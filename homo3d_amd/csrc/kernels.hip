@@ -473,6 +473,158 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
 }
 
 // ---------------------------------------------------------------------------------------------
+// CG pass with each workgroup's node box of z staged in LDS
+// ---------------------------------------------------------------------------------------------
+// The step of k_fine_apply<6, 0, true, half2_t, BETA, PX> on the same 128 x 2 x 2 vertex tiles, with the
+// same arithmetic in the same order (so the same results bit for bit), for a parity-blocked level with
+// 64 | n/2.  The row form gathers 243 single words per vertex from memory, and on gfx950 it is the texture
+// addresser that those gathers keep busy, not HBM; here the workgroup loads the 130 x 4 x 4 node box of
+// the packed-FP16 z (9 word planes) once, row by row with coalesced loads (about 37 per thread), and
+// reads the 27 neighbours of each vertex from LDS.
+// Box layout: word plane q, then z row lz and y row ly (coordinate - (2 by - 1), bz alike), then x
+// parity s, then the half-x position k = (x >> 1) - (bx0 - s).  A row of one parity is contiguous in
+// memory and in LDS, so the 64 lanes of a wave (consecutive x >> 1) read 64 consecutive words.
+constexpr int kStK = 65;                    // half-x positions per staged row: 64 owned + 1 halo
+constexpr int kStRow = 2 * kStK;            // both x parities of one (y, z) row
+constexpr int kStRows = 16;                 // 4 z rows x 4 y rows
+constexpr int kStPlane = 2112;              // 16 x 130 words padded to a multiple of 64 (256-byte steps)
+constexpr int kStPos = kStRows * kStRow;   // staged positions per word plane
+constexpr int kStIter = (kStPos + kFullThreads - 1) / kFullThreads;
+static_assert(kStPlane >= kStRows * kStRow && kStPlane % 64 == 0, "staged plane stride");
+
+// neighbour d of the vertex: its 9 words from the staged box; xb[dx + 1] holds the thread's word index of
+// the x-neighbour dx in row (py, pz) of every plane
+template <int D>
+__device__ __forceinline__ void staged_nb(half2_t (&dst)[3][3], const uint32_t* zs, const int (&xb)[3]) {
+    constexpr int dx = D % 3, dy = D / 3 % 3, dz = D / 9;
+    const uint32_t* a = zs + xb[dx] + (dz * 4 + dy) * kStRow;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) dst[p][i] = __builtin_bit_cast(half2_t, a[(p * 3 + i) * kStPlane]);
+}
+
+template <int D>
+__device__ __forceinline__ void staged_step(const FineCtx& c, const float* ke, const uint32_t* zs, const int (&xb)[3],
+                                            const half2_t (&cur)[3][3], float (&acc)[6][3]) {
+    __builtin_amdgcn_sched_barrier(0);
+    half2_t nxt[3][3];
+    if constexpr (D < 26) staged_nb<D + 1>(nxt, zs, xb);
+    float S[9];
+    fine_block<D>(c.wc, ke, S);
+    block_fma<3, float, half2_t>(S, cur, acc);
+    if constexpr (D < 26) staged_step<D + 1>(c, ke, zs, xb, nxt, acc);
+}
+
+template <bool BETA, int PX>
+__global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_t N, const float* __restrict__ ke,
+                                                                    const float* __restrict__ w,
+                                                                    const uint32_t* __restrict__ z,
+                                                                    float* __restrict__ out,
+                                                                    double* __restrict__ partial,
+                                                                    const double* __restrict__ beta,
+                                                                    float* __restrict__ pp, float* __restrict__ xx,
+                                                                    const double* __restrict__ alpha) {
+    // one LDS array with KE first: its wave-uniform reads then take immediate offsets below 64 KB
+    __shared__ uint32_t lds[576 + 9 * kStPlane];
+    float* ke_s = reinterpret_cast<float*>(lds);
+    uint32_t* zs = lds + 576;
+    const int h = L.h;
+    const uint32_t t0 = xcd_chunk(blockIdx.x, gridDim.x) * 64u;
+    const int bx0 = (int)(t0 % (uint32_t)h), by = (int)((t0 / (uint32_t)h) % (uint32_t)h);
+    const int bz = (int)(t0 / ((uint32_t)h * h));
+    {
+        // position pos = ((lz * 4 + ly) * 2 + s) * 65 + k of the box: one address, nine word planes
+        const Rsrc rz = make_rsrc(z, 9 * N * sizeof(uint32_t));
+        const uint32_t plane = (uint32_t)(N * sizeof(uint32_t));
+        uint32_t tmp[kStIter][9];
+#pragma unroll
+        for (int it = 0; it < kStIter; it++) {
+            const int pos = it * kFullThreads + (int)threadIdx.x;
+            if (kStPos % kFullThreads == 0 || pos < kStPos) {
+                const int row = pos / kStK, k = pos - row * kStK;
+                const int s = row & 1, ly = (row >> 1) & 3, lz = row >> 3;
+                int hx = bx0 - s + k;
+                hx = hx < 0 ? hx + h : (hx >= h ? hx - h : hx);
+                const int yy = L.wrap(2 * by - 1 + ly), zz = L.wrap(2 * bz - 1 + lz);
+                const uint32_t blk = (uint32_t)(s | (yy & 1) << 1 | (zz & 1) << 2);
+                const uint32_t id = blk * L.h3 + (uint32_t)hx +
+                                    (uint32_t)h * ((uint32_t)(yy >> 1) + (uint32_t)h * (uint32_t)(zz >> 1));
+#pragma unroll
+                for (int q = 0; q < 9; q++)
+                    tmp[it][q] = __builtin_amdgcn_raw_buffer_load_b32(rz, (int)(id * 4u), (int)(q * plane), 0);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kStIter; it++) {
+            const int pos = it * kFullThreads + (int)threadIdx.x;
+            if (kStPos % kFullThreads == 0 || pos < kStPos) {
+#pragma unroll
+                for (int q = 0; q < 9; q++) zs[q * kStPlane + pos] = tmp[it][q];
+            }
+        }
+    }
+    stage_ke(ke_s, ke);  // ends with the workgroup barrier that also publishes the staged box
+    const int b = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63u);
+    const int px = b & 1, py = (b >> 1) & 1, pz = b >> 2;
+    const uint32_t v = (uint32_t)b * L.h3 + t0 + (uint32_t)lane;
+    const int x = 2 * (bx0 + lane) + px, y = 2 * by + py, zc = 2 * bz + pz;
+    FineCtx c;
+    fine_ctx(c, x, y, zc, L, w);
+    // x-neighbour dx: parity s' = (px + dx) & 1, half-x position k = lane + (px + dx - s') / 2 + s'
+    int xb[3];
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+        const int sp = (px + dx) & 1;
+        xb[dx + 1] = lane + (pz * 4 + py) * kStRow + sp * kStK + (px + dx - sp) / 2 + sp;
+    }
+    float acc[6][3];
+#pragma unroll
+    for (int r = 0; r < 6; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
+    {
+        half2_t first[3][3];
+        staged_nb<0>(first, zs, xb);
+        staged_step<0>(c, ke_s, zs, xb, first, acc);
+    }
+    half2_t zv[3][3];  // z of the vertex itself (d = 13)
+    staged_nb<13>(zv, zs, xb);
+    double dot[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        float bt = 0.f, a = 0.f;
+        if constexpr (BETA) bt = (float)beta[r];
+        if constexpr (PX == 2) a = (float)alpha[r];
+        dot[r] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const size_t o = (size_t)(r * 3 + i) * N + v;
+            float val = acc[r][i];
+            if constexpr (BETA) val = fmaf(bt, out[o], val);
+            val = (v == 0) ? 0.f : val;
+            out[o] = val;
+            const float zr = (float)zv[r >> 1][i][r & 1];
+            float dv = zr;
+            if constexpr (PX == 2) {
+                const float po = pp[o];
+                xx[o] = fmaf(a, po, xx[o]);
+                dv = fmaf(bt, po, zr);
+            }
+            pp[o] = dv;
+            dot[r] = fma((double)dv, (double)val, dot[r]);
+        }
+    }
+    __shared__ double sh[kFullThreads / 64];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        const double d = block_sum<kFullThreads>(dot[r], sh);
+        if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = d;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Finest-level operator in element form on the matrix cores (packed-FP16 u, n % 32 == 0)
 // ---------------------------------------------------------------------------------------------
 // K u = sum over cells e of w_e KE u_e (the per-cell form of grid.cu:1277-1461).  A workgroup owns a
@@ -2503,9 +2655,11 @@ static void fine_apply_cg_row(hipStream_t s, const Lay& L, size_t N, const float
                        sc + 2 * R);
 }
 
+int fine_cg_staged_ok(int n) { return n % 128 == 0; }
+
 void launch_fine_apply_cg(hipStream_t s, int n, const float* ke, const FineMfma* mf, const float* w, const void* z,
-                          int z_half, float* p, float* x, const double* sc, int first, float* q, double* partial,
-                          int dot_blocks, double* out6) {
+                          int z_half, int staged, float* p, float* x, const double* sc, int first, float* q,
+                          double* partial, int dot_blocks, double* out6) {
     constexpr int R = 6;
     const size_t N = (size_t)n * n * n, M = 3 * N;
     const Lay L = make_lay(n, 0);
@@ -2526,6 +2680,14 @@ void launch_fine_apply_cg(hipStream_t s, int n, const float* ke, const FineMfma*
         launch_fine_apply(s, n, ke, nullptr, w, p, 0, nullptr, q);
         launch_dot6(s, M, p, q, partial, dot_blocks, out6);
         return;
+    } else if (z_half && staged && fine_cg_staged_ok(n)) {
+        const uint32_t* zw = static_cast<const uint32_t*>(z);
+        if (first)
+            hipLaunchKernelGGL((k_fine_cg_staged<false, 1>), dim3(nb), dim3(kFullThreads), 0, s, L, N, ke, w, zw, q,
+                               partial, nullptr, p, x, nullptr);
+        else
+            hipLaunchKernelGGL((k_fine_cg_staged<true, 2>), dim3(nb), dim3(kFullThreads), 0, s, L, N, ke, w, zw, q,
+                               partial, sc + 4 * R, p, x, sc + 2 * R);
     } else if (z_half) {
         const half2_t* zh = static_cast<const half2_t*>(z);
         if (first)

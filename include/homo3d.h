@@ -128,6 +128,11 @@ typedef struct {
     int chosen_mfma; /* 1 when the element form was chosen */
 } homo3d_tune_report;
 int homo3d_autotune(homo3d_handle* h, homo3d_tune_report* report);
+/* CG pass of the refined solver (no reference counterpart): on = 1 (the default) stages each workgroup's
+ * node box of the packed-FP16 z in LDS where the grid allows it (parity-blocked, n % 128 == 0) and reads
+ * the 27 neighbours from there; on = 0 gathers them from memory.  Both give the same results bit for bit;
+ * the switch exists for timing and testing. */
+int homo3d_stage_tiles(homo3d_handle* h, int on);
 /* FP64 relres ||f - K u|| / ||f|| per load case of the refined solver's current iterate, without solving
  * (Grid::update_residual with the norms of MG::solveEquation, MG.cpp:113-203, in FP64). */
 int homo3d_residual_norms(homo3d_handle* h, double relres[6]);
