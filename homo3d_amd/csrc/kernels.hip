@@ -108,6 +108,10 @@ __device__ __forceinline__ half2_t bload<half2_t>(Rsrc r, uint32_t voff, uint32_
     return __builtin_bit_cast(half2_t, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
 
+__device__ __forceinline__ void bstore(float v, Rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
+
 // load cases per stored word of a field type
 template <typename A>
 struct CasesPerWord {
@@ -525,6 +529,7 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
                                                                     const double* __restrict__ beta,
                                                                     float* __restrict__ pp, float* __restrict__ xx,
                                                                     const double* __restrict__ alpha) {
+    static_assert(BETA == (PX == 2), "the first pass (PX 1) has no beta; later passes (PX 2) have both");
     // one LDS array with KE first: its wave-uniform reads then take immediate offsets below 64 KB
     __shared__ uint32_t lds[576 + 9 * kStPlane];
     float* ke_s = reinterpret_cast<float*>(lds);
@@ -533,7 +538,28 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
     const uint32_t t0 = xcd_chunk(blockIdx.x, gridDim.x) * 64u;
     const int bx0 = (int)(t0 % (uint32_t)h), by = (int)((t0 / (uint32_t)h) % (uint32_t)h);
     const int bz = (int)(t0 / ((uint32_t)h * h));
+    const int b = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t v = (uint32_t)b * L.h3 + t0 + (uint32_t)lane;
+    // PX 2: the vertex's p_old and x are loaded with the box and q_old right after it, so their latency
+    // overlaps the staging and the neighbour loop; x += alpha p_old is stored before the loop, q_old and
+    // p_old stay in registers
+    // (buffer accesses: one 32-bit offset v * 4 per thread, the (load case, component) plane in an SGPR)
+    const uint32_t vo = v * 4u, fplane = (uint32_t)(N * sizeof(float));
+    const size_t fbytes = 18 * N * sizeof(float);
+    const Rsrc rq = make_rsrc(out, fbytes), rp = make_rsrc(pp, fbytes), rx = make_rsrc(xx, fbytes);
+    float qo[6][3], po[6][3];
     {
+        float xo[6][3];
+        if constexpr (PX == 2) {
+#pragma unroll
+            for (int r = 0; r < 6; r++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    po[r][i] = bload<float>(rp, vo, (uint32_t)(r * 3 + i) * fplane);
+                    xo[r][i] = bload<float>(rx, vo, (uint32_t)(r * 3 + i) * fplane);
+                }
+        }
         // position pos = ((lz * 4 + ly) * 2 + s) * 65 + k of the box: one address, nine word planes
         const Rsrc rz = make_rsrc(z, 9 * N * sizeof(uint32_t));
         const uint32_t plane = (uint32_t)(N * sizeof(uint32_t));
@@ -563,12 +589,21 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
                 for (int q = 0; q < 9; q++) zs[q * kStPlane + pos] = tmp[it][q];
             }
         }
+        if constexpr (PX == 2) {
+#pragma unroll
+            for (int r = 0; r < 6; r++) {
+                const float a = (float)alpha[r];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const uint32_t so = (uint32_t)(r * 3 + i) * fplane;
+                    bstore(fmaf(a, po[r][i], xo[r][i]), rx, vo, so);
+                    qo[r][i] = bload<float>(rq, vo, so);
+                }
+            }
+        }
     }
     stage_ke(ke_s, ke);  // ends with the workgroup barrier that also publishes the staged box
-    const int b = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int lane = (int)(threadIdx.x & 63u);
     const int px = b & 1, py = (b >> 1) & 1, pz = b >> 2;
-    const uint32_t v = (uint32_t)b * L.h3 + t0 + (uint32_t)lane;
     const int x = 2 * (bx0 + lane) + px, y = 2 * by + py, zc = 2 * bz + pz;
     FineCtx c;
     fine_ctx(c, x, y, zc, L, w);
@@ -594,25 +629,20 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
     double dot[6];
 #pragma unroll
     for (int r = 0; r < 6; r++) {
-        float bt = 0.f, a = 0.f;
+        float bt = 0.f;
         if constexpr (BETA) bt = (float)beta[r];
-        if constexpr (PX == 2) a = (float)alpha[r];
         dot[r] = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const size_t o = (size_t)(r * 3 + i) * N + v;
+            const uint32_t so = (uint32_t)(r * 3 + i) * fplane;
             float val = acc[r][i];
-            if constexpr (BETA) val = fmaf(bt, out[o], val);
+            if constexpr (BETA) val = fmaf(bt, qo[r][i], val);
             val = (v == 0) ? 0.f : val;
-            out[o] = val;
+            bstore(val, rq, vo, so);
             const float zr = (float)zv[r >> 1][i][r & 1];
             float dv = zr;
-            if constexpr (PX == 2) {
-                const float po = pp[o];
-                xx[o] = fmaf(a, po, xx[o]);
-                dv = fmaf(bt, po, zr);
-            }
-            pp[o] = dv;
+            if constexpr (PX == 2) dv = fmaf(bt, po[r][i], zr);
+            bstore(dv, rp, vo, so);
             dot[r] = fma((double)dv, (double)val, dot[r]);
         }
     }
