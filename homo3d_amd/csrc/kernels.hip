@@ -122,6 +122,35 @@ struct CasesPerWord<half2_t> {
     static constexpr int value = 2;
 };
 
+// Packed FP16 six-load-case fields (half2_t words): word q = p * 3 + i of vertex v (component i of load
+// cases 2p (.x) and 2p + 1 (.y)) is word hw_index(q, v, N) of the field.  Words 0-3 and 4-7 form two
+// planes of 16-byte quads and word 8 a plane of single words, so the nine words of a vertex are two
+// 16-byte loads and one 4-byte load: a 27-neighbour gather costs the texture addresser per load
+// instruction, not per byte (nine 4-byte loads per neighbour take about four times as long).
+__host__ __device__ __forceinline__ size_t hw_index(int q, size_t v, size_t N) {
+    return q < 8 ? (size_t)(q >> 2) * 4 * N + 4 * v + (size_t)(q & 3) : 8 * N + v;
+}
+// word q of vertex v in a field of A words: packed FP16 as above, FP32 planes q * N + v
+template <typename A>
+__device__ __forceinline__ size_t fw_index(int q, size_t v, size_t N) {
+    if constexpr (CasesPerWord<A>::value == 2)
+        return hw_index(q, v, N);
+    else
+        return (size_t)q * N + v;
+}
+// the nine words of vertex idx of a packed FP16 field through a buffer resource over it (plane = 4 N
+// bytes; the byte offsets stay below 36 N)
+__device__ __forceinline__ void hload9(uint32_t (&wv)[9], Rsrc r, uint32_t idx, uint32_t plane) {
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(idx * 16u), 0, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(idx * 16u), (int)(4u * plane), 0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        wv[k] = (uint32_t)a[k];
+        wv[4 + k] = (uint32_t)b[k];
+    }
+    wv[8] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), (int)(8u * plane), 0);
+}
+
 // 3x3 block of the assembled fine operator between vertex v and its neighbour v+d:
 // S = sum over the cells containing both of w_e * KE[node(v)][node(v+d)].
 // Cell e (0..7) sits at offset (e&1, e>>1&1, e>>2) - 1 from v; v is its node 7-e.
@@ -233,13 +262,32 @@ __host__ __device__ constexpr uint32_t later_mask(int c) {
 
 // the stored words of one neighbour: W = R / (load cases per word) words per component; r0 counts
 // load cases (a multiple of the cases per word)
+// (qo = 4 * vertex index; plane = 4 N bytes)
 template <int W, typename A>
 __device__ __forceinline__ void load_nb(A (&dst)[W][3], Rsrc ru, uint32_t qo, uint32_t plane, int r0) {
     constexpr int cpw = CasesPerWord<A>::value;
+    if constexpr (cpw == 2 && W == 3) {  // all six load cases (r0 = 0): the vertex's nine words
+        uint32_t wv[9];
+        hload9(wv, ru, qo >> 2, plane);
 #pragma unroll
-    for (int p = 0; p < W; p++)
+        for (int p = 0; p < 3; p++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) dst[p][i] = bload<A>(ru, qo, (uint32_t)((r0 / cpw + p) * 3 + i) * plane);
+            for (int i = 0; i < 3; i++) dst[p][i] = __builtin_bit_cast(A, wv[p * 3 + i]);
+    } else if constexpr (cpw == 2) {
+#pragma unroll
+        for (int p = 0; p < W; p++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int q = (r0 / 2 + p) * 3 + i;
+                dst[p][i] = q < 8 ? bload<A>(ru, 4u * qo + 4u * (uint32_t)(q & 3), (uint32_t)(q >> 2) * 4u * plane)
+                                  : bload<A>(ru, qo, 8u * plane);
+            }
+    } else {
+#pragma unroll
+        for (int p = 0; p < W; p++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) dst[p][i] = bload<A>(ru, qo, (uint32_t)((r0 / cpw + p) * 3 + i) * plane);
+    }
 }
 
 // The two x-neighbours (d = 12, 14) of a vertex taken from LDS instead of memory: `base` holds 3W word
@@ -450,7 +498,7 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
             if constexpr (PX != 0) {
                 float zc;  // z of the vertex itself
                 if constexpr (CasesPerWord<A>::value == 2)
-                    zc = (float)u[(size_t)((r >> 1) * 3 + i) * N + v][r & 1];
+                    zc = (float)u[hw_index((r >> 1) * 3 + i, v, N)][r & 1];
                 else
                     zc = (float)u[o];
                 dv = zc;
@@ -576,9 +624,7 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
                 const uint32_t blk = (uint32_t)(s | (yy & 1) << 1 | (zz & 1) << 2);
                 const uint32_t id = blk * L.h3 + (uint32_t)hx +
                                     (uint32_t)h * ((uint32_t)(yy >> 1) + (uint32_t)h * (uint32_t)(zz >> 1));
-#pragma unroll
-                for (int q = 0; q < 9; q++)
-                    tmp[it][q] = __builtin_amdgcn_raw_buffer_load_b32(rz, (int)(id * 4u), (int)(q * plane), 0);
+                hload9(tmp[it], rz, id, plane);
             }
         }
 #pragma unroll
@@ -811,28 +857,28 @@ __global__ __launch_bounds__(kEfThreads, 2) void k_fine_apply_ef(Lay L, size_t N
         const int ex = c % kEfCX - 1, ey = (c / kEfCX) % kEfCY - 1, ez = c / (kEfCX * kEfCY) - 1;
         wcell[c] = w[L.idx(L.wrap(x0 + ex), L.wrap(y0 + ey), L.wrap(z0 + ez))];
     }
-    // node-box words of this thread: byte offsets within load-case pair 0 (~0u: none).  The next
-    // pair's words are fetched into registers before the current pair is computed.
-    uint32_t soff[kEfStage], pre[kEfStage];
+    // node-box words of this thread: component i and vertex index of each (~0u: none).  The next pair's
+    // words (q = 3 p + i, hw_index) are fetched into registers before the current pair is computed.
+    uint32_t nidx[kEfStage], pre[kEfStage];
+    int ncomp[kEfStage];
 #pragma unroll
     for (int k = 0; k < kEfStage; k++) {
         const int s = tid + k * kEfThreads, i = s / kEfNodes, b = s - i * kEfNodes;
         const int bx = b % kEfBX, by = (b / kEfBX) % kEfBY, bz = b / (kEfBX * kEfBY);
-        soff[k] = s < 3 * kEfNodes
-                      ? 4u * ((uint32_t)i * (uint32_t)N + L.idx(L.wrap(x0 - 1 + bx), L.wrap(y0 - 1 + by), L.wrap(z0 - 1 + bz)))
-                      : ~0u;
+        ncomp[k] = i;
+        nidx[k] = s < 3 * kEfNodes ? L.idx(L.wrap(x0 - 1 + bx), L.wrap(y0 - 1 + by), L.wrap(z0 - 1 + bz)) : ~0u;
     }
-    const uint32_t pair_bytes = (uint32_t)(3 * N * sizeof(uint32_t));
     auto fetch = [&](int p) {
 #pragma unroll
         for (int k = 0; k < kEfStage; k++)
-            if (soff[k] != ~0u)
-                pre[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(ru, (int)soff[k], (int)(p * pair_bytes), 0);
+            if (nidx[k] != ~0u)
+                pre[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
+                    ru, (int)(4u * (uint32_t)hw_index(3 * p + ncomp[k], nidx[k], N)), 0, 0);
     };
     auto put = [&]() {
 #pragma unroll
         for (int k = 0; k < kEfStage; k++)
-            if (soff[k] != ~0u) nodes[tid + k * kEfThreads] = pre[k];
+            if (nidx[k] != ~0u) nodes[tid + k * kEfThreads] = pre[k];
     };
     fetch(0);
     put();
@@ -988,7 +1034,7 @@ __device__ __forceinline__ void gs_vertex(const Lay& L, size_t N, uint32_t v, fl
         float uu[cpw][3], ff[cpw][3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const size_t o = (size_t)((r0 / cpw + p) * 3 + i) * N + v;
+            const size_t o = fw_index<A>((r0 / cpw + p) * 3 + i, v, N);
             if constexpr (cpw == 2) {
                 const half2_t wv = ZC >= 0 ? half2_t{0, 0} : u[o];
                 uu[0][i] = (float)wv.x;
@@ -1004,7 +1050,7 @@ __device__ __forceinline__ void gs_vertex(const Lay& L, size_t N, uint32_t v, fl
         float us[cpw][3];  // the stored values
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const size_t o = (size_t)((r0 / cpw + p) * 3 + i) * N + v;
+            const size_t o = fw_index<A>((r0 / cpw + p) * 3 + i, v, N);
             A word;
             if constexpr (cpw == 2)
                 word = half2_t{(_Float16)uu[0][i], (_Float16)uu[1][i]};
@@ -1344,7 +1390,7 @@ __global__ __launch_bounds__(256) void k_widen_f64(size_t M, const float* __rest
 }
 
 // zero the pinned origin's entries of a 6-load-case field of `planes` word planes
-// FP32 planes [6][3][N] <-> packed FP16 words [3][3][N] (word plane p * 3 + i: cases 2p, 2p + 1)
+// FP32 planes [6][3][N] <-> packed FP16 words (word q = p * 3 + i: cases 2p, 2p + 1; hw_index)
 __global__ __launch_bounds__(256) void k_pack_half(size_t N, const float* __restrict__ u, half2_t* __restrict__ z) {
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= 9 * N) return;
@@ -1352,20 +1398,21 @@ __global__ __launch_bounds__(256) void k_pack_half(size_t N, const float* __rest
     half2_t h;
     h.x = (_Float16)u[((2 * p) * 3 + i) * N + v];
     h.y = (_Float16)u[((2 * p + 1) * 3 + i) * N + v];
-    z[t] = h;
+    z[hw_index((int)q, v, N)] = h;
 }
 __global__ __launch_bounds__(256) void k_unpack_half(size_t N, const half2_t* __restrict__ z, float* __restrict__ u) {
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= 9 * N) return;
     const size_t q = t / N, v = t - q * N, p = q / 3, i = q - 3 * p;
-    const half2_t h = z[t];
+    const half2_t h = z[hw_index((int)q, v, N)];
     u[((2 * p) * 3 + i) * N + v] = (float)h.x;
     u[((2 * p + 1) * 3 + i) * N + v] = (float)h.y;
 }
 
-template <typename T>
+// (half: the 9 words of a packed FP16 field)
+template <typename T, bool HALF = false>
 __global__ void k_zero_pin(size_t N, int planes, T* __restrict__ u) {
-    if ((int)threadIdx.x < planes) u[(size_t)threadIdx.x * N] = T(0);
+    if ((int)threadIdx.x < planes) u[HALF ? hw_index((int)threadIdx.x, 0, N) : (size_t)threadIdx.x * N] = T(0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1476,7 +1523,7 @@ __device__ __forceinline__ void stencil_gs_vertex(const Lay& L, size_t N, uint32
     for (int p = 0; p < R / cpw; p++) {
         A wd[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) wd[i] = ZC >= 0 ? A(0) : u[(size_t)((r0 / cpw + p) * 3 + i) * N + v];
+        for (int i = 0; i < 3; i++) wd[i] = ZC >= 0 ? A(0) : u[fw_index<A>((r0 / cpw + p) * 3 + i, v, N)];
 #pragma unroll
         for (int hh = 0; hh < cpw; hh++) {
             const int r = p * cpw + hh;
@@ -1515,7 +1562,7 @@ __device__ __forceinline__ void stencil_gs_vertex(const Lay& L, size_t N, uint32
         }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            u[(size_t)((r0 / cpw + p) * 3 + i) * N + v] = wd[i];
+            u[fw_index<A>((r0 / cpw + p) * 3 + i, v, N)] = wd[i];
             if constexpr (TO_LDS) out_lds[(p * 3 + i) * 256 + threadIdx.x] = wd[i];
         }
     }
@@ -1791,89 +1838,77 @@ __global__ __launch_bounds__(kFullThreads) void k_prolong4(Lay Lf, Lay Lc, const
     }
 }
 
-// k_prolong4 for a fine field in packed FP16 (half2_t word planes): a thread takes the load-case pair
-// blockIdx.y (0..2), both halves of each word.
-// CT: coarse storage, FP32 planes or packed FP16 pairs (the latter on parity-blocked levels only).
+// Prolongation into a packed FP16 fine field (hw_index words): uf += P uc, one thread per fine vertex
+// and all nine of its words (two 16-byte and one 4-byte read-modify-write), on the 512-thread parity
+// tiles of full_vertex.  Per axis an even fine coordinate has one coarse parent (weight 1) and an odd one
+// two (weight 1/2 each): the x parents are averaged first, then the (y, z) parents accumulate with their
+// weights.  CT: coarse storage, FP32 planes (lexicographic or parity-blocked) or packed FP16 words
+// (parity-blocked).  The pinned origin (pin) keeps its value.
 template <typename CT>
-__global__ __launch_bounds__(kFullThreads) void k_prolong4h(Lay Lf, Lay Lc, const CT* __restrict__ uc,
-                                                            half2_t* __restrict__ uf, int pin) {
-    const uint32_t hf = (uint32_t)Lf.h, q = hf / 4;
-    const uint32_t t4 = xcd_chunk(blockIdx.x, gridDim.x) * 64u + (threadIdx.x & 63u);
-    if (t4 >= Lf.h3 / 4) return;
-    const uint32_t b = threadIdx.x >> 6;
-    const int px = b & 1, py = (b >> 1) & 1, pz = (int)(b >> 2);
-    const uint32_t j = t4 % q, row = t4 / q;
-    const int ly = (int)(row % hf), lz = (int)(row / hf);
-    const int nc = Lc.n;
-    const int Ys[2] = {ly, ly + 1 == nc ? 0 : ly + 1}, Zs[2] = {lz, lz + 1 == nc ? 0 : lz + 1};
-    const float wy[2] = {py ? 0.5f : 1.f, 0.5f}, wz[2] = {pz ? 0.5f : 1.f, 0.5f};
-    const int X4 = (int)(4 * j + 4) == nc ? 0 : (int)(4 * j + 4);
-    const size_t Nf = (size_t)Lf.n * Lf.n * Lf.n, Nc = (size_t)nc * nc * nc;
-    const int pr = blockIdx.y;
-    const bool skip0 = pin && b == 0 && row == 0 && j == 0;  // pinned origin
+__device__ __forceinline__ void coarse_words(float (&c)[9][2], const CT* __restrict__ uc, uint32_t idx, size_t Nc) {
+    if constexpr (CasesPerWord<CT>::value == 2) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(uc);
+        const uint4 a = *reinterpret_cast<const uint4*>(w + 4 * (size_t)idx);
+        const uint4 b = *reinterpret_cast<const uint4*>(w + 4 * Nc + 4 * (size_t)idx);
+        const uint32_t wv[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, w[8 * Nc + idx]};
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float a[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-        if constexpr (CasesPerWord<CT>::value == 2) {
-            // one word holds both load cases of the pair (parity-blocked coarse level)
-            const half2_t* cp = uc + (size_t)(pr * 3 + c) * Nc;
-            for (int iz = 0; iz <= pz; iz++)
-                for (int iy = 0; iy <= py; iy++) {
-                    const uint32_t yz = Lc.part(Ys[iy], 1) + Lc.part(Zs[iz], 2);
-                    const uint2 ev = *reinterpret_cast<const uint2*>(cp + yz + Lc.part(4 * j, 0));
-                    const uint2 od = *reinterpret_cast<const uint2*>(cp + yz + Lc.part(4 * j + 1, 0));
-                    const half2_t cw[5] = {__builtin_bit_cast(half2_t, ev.x), __builtin_bit_cast(half2_t, od.x),
-                                           __builtin_bit_cast(half2_t, ev.y), __builtin_bit_cast(half2_t, od.y),
-                                           px ? cp[yz + Lc.part(X4, 0)] : half2_t(0)};
-                    const float w = wy[iy] * wz[iz];
-#pragma unroll
-                    for (int h = 0; h < 2; h++)
-#pragma unroll
-                        for (int k = 0; k < 4; k++) {
-                            const float c0 = (float)cw[k][h], c1 = (float)cw[k + 1][h];
-                            a[h][k] = fmaf(w, px ? 0.5f * (c0 + c1) : c0, a[h][k]);
-                        }
-                }
-        } else {
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const float* cp = uc + (size_t)((2 * pr + h) * 3 + c) * Nc;
-                for (int iz = 0; iz <= pz; iz++)
-                    for (int iy = 0; iy <= py; iy++) {
-                        float cv[5];
-                        if (Lc.lex) {
-                            const uint32_t base = (uint32_t)nc * (Ys[iy] + (uint32_t)nc * Zs[iz]);
-                            const float4 v4 = *reinterpret_cast<const float4*>(cp + base + 4 * j);
-                            cv[0] = v4.x; cv[1] = v4.y; cv[2] = v4.z; cv[3] = v4.w;
-                            cv[4] = px ? cp[base + X4] : 0.f;
-                        } else {
-                            const uint32_t yz = Lc.part(Ys[iy], 1) + Lc.part(Zs[iz], 2);
-                            const float2 ev = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j, 0));
-                            const float2 od = *reinterpret_cast<const float2*>(cp + yz + Lc.part(4 * j + 1, 0));
-                            cv[0] = ev.x; cv[1] = od.x; cv[2] = ev.y; cv[3] = od.y;
-                            cv[4] = px ? cp[yz + Lc.part(X4, 0)] : 0.f;
-                        }
-                        const float w = wy[iy] * wz[iz];
-#pragma unroll
-                        for (int k = 0; k < 4; k++)
-                            a[h][k] = fmaf(w, px ? 0.5f * (cv[k] + cv[k + 1]) : cv[k], a[h][k]);
-                    }
-            }
+        for (int q = 0; q < 9; q++) {
+            const half2_t hv = __builtin_bit_cast(half2_t, wv[q]);
+            c[q][0] = (float)hv.x;
+            c[q][1] = (float)hv.y;
         }
-        half2_t* dst = uf + (size_t)(pr * 3 + c) * Nf + b * Lf.h3 + row * hf + 4 * j;
-        uint4 raw = *reinterpret_cast<const uint4*>(dst);
-        uint32_t wv[4] = {raw.x, raw.y, raw.z, raw.w};
+    } else {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            half2_t hv = __builtin_bit_cast(half2_t, wv[k]);
-            if (!(skip0 && k == 0)) {
-                hv.x = (_Float16)((float)hv.x + a[0][k]);
-                hv.y = (_Float16)((float)hv.y + a[1][k]);
-            }
-            wv[k] = __builtin_bit_cast(uint32_t, hv);
-        }
-        *reinterpret_cast<uint4*>(dst) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+        for (int q = 0; q < 9; q++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) c[q][h] = uc[(size_t)((2 * (q / 3) + h) * 3 + q % 3) * Nc + idx];
     }
+}
+
+template <typename CT>
+__global__ __launch_bounds__(kFullThreads) void k_prolong_h(Lay Lf, Lay Lc, const CT* __restrict__ uc,
+                                                           half2_t* __restrict__ uf, int pin) {
+    uint32_t v;
+    if (!full_vertex(Lf, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
+    if (pin && v == 0) return;
+    int x, y, z;
+    Lf.pos(v, x, y, z);
+    const int nc = Lc.n;
+    const int px = x & 1, py = y & 1, pz = z & 1;
+    const int X0 = x >> 1, X1 = X0 + 1 == nc ? 0 : X0 + 1;
+    const int Ys[2] = {y >> 1, (y >> 1) + 1 == nc ? 0 : (y >> 1) + 1};
+    const int Zs[2] = {z >> 1, (z >> 1) + 1 == nc ? 0 : (z >> 1) + 1};
+    const float wy[2] = {py ? 0.5f : 1.f, 0.5f}, wz[2] = {pz ? 0.5f : 1.f, 0.5f};
+    const size_t Nf = (size_t)Lf.n * Lf.n * Lf.n, Nc = (size_t)nc * nc * nc;
+    float a[9][2];
+#pragma unroll
+    for (int q = 0; q < 9; q++) a[q][0] = a[q][1] = 0.f;
+    for (int iz = 0; iz <= pz; iz++)
+        for (int iy = 0; iy <= py; iy++) {
+            const float w = wy[iy] * wz[iz];
+            float c0[9][2], c1[9][2];
+            coarse_words<CT>(c0, uc, Lc.idx(X0, Ys[iy], Zs[iz]), Nc);
+            if (px) coarse_words<CT>(c1, uc, Lc.idx(X1, Ys[iy], Zs[iz]), Nc);
+#pragma unroll
+            for (int q = 0; q < 9; q++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) a[q][h] = fmaf(w, px ? 0.5f * (c0[q][h] + c1[q][h]) : c0[q][h], a[q][h]);
+        }
+    uint32_t* fw = reinterpret_cast<uint32_t*>(uf);
+    uint4* f0 = reinterpret_cast<uint4*>(fw + 4 * (size_t)v);
+    uint4* f1 = reinterpret_cast<uint4*>(fw + 4 * Nf + 4 * (size_t)v);
+    const uint4 o0 = *f0, o1 = *f1;
+    uint32_t wv[9] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, o1.z, o1.w, fw[8 * Nf + v]};
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        half2_t hv = __builtin_bit_cast(half2_t, wv[q]);
+        hv.x = (_Float16)((float)hv.x + a[q][0]);
+        hv.y = (_Float16)((float)hv.y + a[q][1]);
+        wv[q] = __builtin_bit_cast(uint32_t, hv);
+    }
+    *f0 = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+    *f1 = make_uint4(wv[4], wv[5], wv[6], wv[7]);
+    fw[8 * Nf + v] = wv[8];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2173,23 +2208,40 @@ __global__ __launch_bounds__(256) void k_pcg_update_px(int R, size_t M, const do
     }
 }
 
-// The same CG steps with the preconditioned vector z in packed FP16 (half2_t word planes; load-case
-// pair pr = blockIdx.y holds cases 2pr (.x) and 2pr + 1 (.y); M values per load case, M % 4 == 0).
+// The same CG steps with the preconditioned vector z in packed FP16 (hw_index words; load-case pair
+// pr = blockIdx.y holds cases 2pr (.x) and 2pr + 1 (.y); M = 3 N values per load case, N % 4 == 0).
 //
+// words q = pr * 3 + c (c = e / N, the entry e = 4 i of the pair's M = 3 N values) of the four vertices
+// e % N .. e % N + 3 of a packed FP16 field (hw_index; N % 4 == 0)
+__device__ __forceinline__ void hwords4(half2_t (&o)[4], const half2_t* __restrict__ z, int pr, size_t i, size_t N) {
+    const size_t e = 4 * i, c = e / N, v0 = e - c * N;
+    const int q = pr * 3 + (int)c;
+    if (q == 8) {
+        const uint4 w = *reinterpret_cast<const uint4*>(z + 8 * N + v0);
+        o[0] = __builtin_bit_cast(half2_t, w.x);
+        o[1] = __builtin_bit_cast(half2_t, w.y);
+        o[2] = __builtin_bit_cast(half2_t, w.z);
+        o[3] = __builtin_bit_cast(half2_t, w.w);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = z[hw_index(q, v0 + k, N)];
+    }
+}
+
 // partial[(2pr + h) * gridDim.x + block] = block's share of r_{2pr+h} . z_{2pr+h}
 __global__ __launch_bounds__(256) void k_dot_fh(size_t M, const float* __restrict__ r, const half2_t* __restrict__ z,
                                                 double* __restrict__ partial) {
     __shared__ double sh[4];
     const int pr = blockIdx.y;
+    const size_t N = M / 3;
     const float4* r0 = reinterpret_cast<const float4*>(r + (size_t)(2 * pr) * M);
     const float4* r1 = reinterpret_cast<const float4*>(r + (size_t)(2 * pr + 1) * M);
-    const uint4* z4 = reinterpret_cast<const uint4*>(z + (size_t)pr * M);
     double s0 = 0.0, s1 = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
         const float4 a = r0[i], b = r1[i];
-        const uint4 zw = z4[i];
-        const half2_t h0 = __builtin_bit_cast(half2_t, zw.x), h1 = __builtin_bit_cast(half2_t, zw.y),
-                      h2 = __builtin_bit_cast(half2_t, zw.z), h3 = __builtin_bit_cast(half2_t, zw.w);
+        half2_t zq[4];
+        hwords4(zq, z, pr, i, N);
+        const half2_t h0 = zq[0], h1 = zq[1], h2 = zq[2], h3 = zq[3];
         s0 += ((double)a.x * (double)h0.x + (double)a.y * (double)h1.x) +
               ((double)a.z * (double)h2.x + (double)a.w * (double)h3.x);
         s1 += ((double)b.x * (double)h0.y + (double)b.y * (double)h1.y) +
@@ -2215,11 +2267,10 @@ __global__ __launch_bounds__(256) void k_pcg_update_px_h(int R, size_t M, const 
         b[h] = set_p ? 0.f : (float)sc[4 * R + k];
         upd[h] = set_p || (do_p && sc[5 * R + k] != 0.0);
     }
-    const uint4* z4 = reinterpret_cast<const uint4*>(z + (size_t)pr * M);
+    const size_t N = M / 3;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
-        const uint4 zw = z4[i];
-        const half2_t zh[4] = {__builtin_bit_cast(half2_t, zw.x), __builtin_bit_cast(half2_t, zw.y),
-                               __builtin_bit_cast(half2_t, zw.z), __builtin_bit_cast(half2_t, zw.w)};
+        half2_t zh[4];
+        hwords4(zh, z, pr, i, N);
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             float4* p4 = reinterpret_cast<float4*>(p + (size_t)(2 * pr + h) * M);
@@ -2892,7 +2943,7 @@ void launch_unpack_half(hipStream_t s, size_t N, const void* z, float* u) {
 
 void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half) {
     if (u_half)
-        hipLaunchKernelGGL((k_zero_pin<uint32_t>), dim3(1), dim3(64), 0, s, N, 9, static_cast<uint32_t*>(u));
+        hipLaunchKernelGGL((k_zero_pin<uint32_t, true>), dim3(1), dim3(64), 0, s, N, 9, static_cast<uint32_t*>(u));
     else
         hipLaunchKernelGGL((k_zero_pin<float>), dim3(1), dim3(64), 0, s, N, 18, static_cast<float*>(u));
 }
@@ -3096,10 +3147,10 @@ void launch_prolong(hipStream_t s, int nf, int lexc, const void* uc, int uc_half
     const Lay Lf = make_lay(nf, 0), Lc = make_lay(nf / 2, lexc);
     if (uc_half && (!u_half || lexc)) return;  // packed FP16 coarse u: parity-blocked, into a packed FP16 fine u
     if (u_half && uc_half)
-        hipLaunchKernelGGL((k_prolong4h<half2_t>), dim3(nblk(Lf.h3 / 4, 64), 3), dim3(kFullThreads), 0, s, Lf, Lc,
+        hipLaunchKernelGGL((k_prolong_h<half2_t>), dim3(full_grid(Lf)), dim3(kFullThreads), 0, s, Lf, Lc,
                            static_cast<const half2_t*>(uc), static_cast<half2_t*>(uf), pin);
     else if (u_half)  // packed FP16 fine fields are only used with nf % 8 == 0
-        hipLaunchKernelGGL((k_prolong4h<float>), dim3(nblk(Lf.h3 / 4, 64), 3), dim3(kFullThreads), 0, s, Lf, Lc,
+        hipLaunchKernelGGL((k_prolong_h<float>), dim3(full_grid(Lf)), dim3(kFullThreads), 0, s, Lf, Lc,
                            static_cast<const float*>(uc), static_cast<half2_t*>(uf), pin);
     else if (nf % 8 == 0)
         hipLaunchKernelGGL(k_prolong4, dim3(nblk(Lf.h3 / 4, 64), 6), dim3(kFullThreads), 0, s, Lf, Lc,
