@@ -1443,6 +1443,46 @@ struct StencilIO<half2_t> {
     }
 };
 
+// One neighbour of a coarse row: issue the loads of the next selected neighbour (its stencil block and
+// words), accumulate this one, recurse.  The barrier keeps the loads one neighbour ahead: with the whole
+// row unrolled freely the compiler issues every neighbour's loads at once and needs 256 VGPRs.
+struct StencilRow {
+    Rsrc rs, ru;
+    uint32_t X[3], Y[3], Z[3], vo, plane;
+};
+template <typename SA, typename A, int W, bool XL, int D>
+__device__ __forceinline__ void stencil_nb(float (&S)[9], A (&cur)[W][3], const StencilRow& c, int r0,
+                                           const XNbLds<A>& xl) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) S[k] = StencilIO<SA>::get(c.rs, c.vo, c.plane, D * 9 + k);
+    if constexpr (XL && (D == 12 || D == 14)) {
+        load_nb_lds<W, A>(cur, xl, D == 12 ? 0 : 1);
+    } else {
+        const uint32_t qo = (c.X[D % 3] + c.Y[D / 3 % 3] + c.Z[D / 9]) * 4u;
+        load_nb<W, A>(cur, c.ru, qo, c.plane, r0);
+    }
+}
+template <int R, uint32_t MASK, int D, typename SA, typename A, bool XL>
+__device__ __forceinline__ void stencil_step(const StencilRow& c, int r0, const XNbLds<A>& xl, const float (&S)[9],
+                                             const A (&cur)[R / CasesPerWord<A>::value][3], float (&acc)[R][3]) {
+    constexpr int W = R / CasesPerWord<A>::value;
+    constexpr int DN = next_nb(MASK, D);
+    __builtin_amdgcn_sched_barrier(0);
+    // the block products only read values loaded one step earlier, so without this the instruction
+    // selector may sink all of them below the last barrier, keeping every neighbour's operands live:
+    // pass the accumulators through an empty volatile statement here (no instruction is emitted)
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) __asm__ volatile("" : "+v"(acc[r][i]));
+    float Sn[9];
+    A nxt[W][3];
+    if constexpr (DN < 27) stencil_nb<SA, A, W, XL, DN>(Sn, nxt, c, r0, xl);
+    block_fma<W, float, A>(S, cur, acc);
+    if constexpr (DN < 27) stencil_step<R, MASK, DN, SA, A, XL>(c, r0, xl, Sn, nxt, acc);
+}
+
+// acc = sum over the neighbours selected by MASK of the stored 3x3 blocks times u, load cases r0 .. r0 + R - 1.
 // A: storage of u (float planes, or half2_t words of load-case pairs; r0 then even).  XL: the two
 // x-neighbours (d = 12, 14) come from LDS (xl).
 template <int R, uint32_t MASK, typename SA, typename A = float, bool XL = false>
@@ -1450,31 +1490,24 @@ __device__ __forceinline__ void stencil_gather(int x, int y, int z, const Lay& L
                                                const SA* __restrict__ st, const A* __restrict__ u,
                                                float (&acc)[R][3], int r0, const XNbLds<A>& xl = XNbLds<A>()) {
     constexpr int W = R / CasesPerWord<A>::value;
-    const uint32_t X[3] = {L.part(wrap_dec(x, L.n), 0), L.part(x, 0), L.part(wrap_inc(x, L.n), 0)};
-    const uint32_t Y[3] = {L.part(wrap_dec(y, L.n), 1), L.part(y, 1), L.part(wrap_inc(y, L.n), 1)};
-    const uint32_t Z[3] = {L.part(wrap_dec(z, L.n), 2), L.part(z, 2), L.part(wrap_inc(z, L.n), 2)};
-    const Rsrc rs = make_rsrc(st, (size_t)StencilIO<SA>::words * N * 4);
-    const Rsrc ru = make_rsrc(u, (size_t)(18 / CasesPerWord<A>::value) * N * sizeof(A));
-    const uint32_t plane = (uint32_t)(N * sizeof(float));  // stencil and u words are 4 bytes
-    const uint32_t vo = v * 4u;
+    StencilRow c;
+    c.X[0] = L.part(wrap_dec(x, L.n), 0); c.X[1] = L.part(x, 0); c.X[2] = L.part(wrap_inc(x, L.n), 0);
+    c.Y[0] = L.part(wrap_dec(y, L.n), 1); c.Y[1] = L.part(y, 1); c.Y[2] = L.part(wrap_inc(y, L.n), 1);
+    c.Z[0] = L.part(wrap_dec(z, L.n), 2); c.Z[1] = L.part(z, 2); c.Z[2] = L.part(wrap_inc(z, L.n), 2);
+    c.rs = make_rsrc(st, (size_t)StencilIO<SA>::words * N * 4);
+    c.ru = make_rsrc(u, (size_t)(18 / CasesPerWord<A>::value) * N * sizeof(A));
+    c.plane = (uint32_t)(N * sizeof(float));  // stencil and u words are 4 bytes
+    c.vo = v * 4u;
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
         for (int i = 0; i < 3; i++) acc[r][i] = 0.f;
-#pragma unroll
-    for (int d = 0; d < 27; d++) {
-        if (!((MASK >> d) & 1u)) continue;
+    constexpr int D0 = next_nb(MASK, -1);
+    if constexpr (D0 < 27) {
         float S[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) S[k] = StencilIO<SA>::get(rs, vo, plane, d * 9 + k);
         A cur[W][3];
-        if (XL && (d == 12 || d == 14)) {
-            load_nb_lds<W, A>(cur, xl, d == 12 ? 0 : 1);
-        } else {
-            const uint32_t qo = (X[d % 3] + Y[d / 3 % 3] + Z[d / 9]) * 4u;
-            load_nb<W, A>(cur, ru, qo, plane, r0);
-        }
-        block_fma<W, float, A>(S, cur, acc);
+        stencil_nb<SA, A, W, XL, D0>(S, cur, c, r0, xl);
+        stencil_step<R, MASK, D0, SA, A, XL>(c, r0, xl, S, cur, acc);
     }
 }
 
