@@ -10,7 +10,7 @@ import time
 import numpy as np
 
 HOMO3D_OK, HOMO3D_NOT_CONVERGED, HOMO3D_DIVERGED, HOMO3D_BAD_ARG, HOMO3D_HIP_ERROR, HOMO3D_NO_SUPPORT = range(6)
-SOLVER_MG, SOLVER_PCG = 0, 1
+SOLVER_MG, SOLVER_PCG, SOLVER_NONE = 0, 1, 2
 FIELD_U, FIELD_F, FIELD_R = 0, 1, 2
 SYM = {"reflect3": 0, "reflect6": 1, "rotate3": 2, "none": 3}
 
@@ -60,6 +60,16 @@ EXPORTS = {
     "homo3d_elastic_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_residual_norms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "homo3d_stage_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "homo3d_density_pow": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_int]),
+    "homo3d_density_pow_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "homo3d_field_sum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "homo3d_field_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int]),
+    "homo3d_device_alloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
+    "homo3d_device_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "homo3d_device_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]),
     "homo3d_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_TuneReport)]),
     "homo3d_sensitivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                           ctypes.c_int, ctypes.c_int]),
@@ -278,7 +288,7 @@ class HomoConfig:
         c.emin = self.emin
         c.fem_rel_thres = self.femRelThres
         c.max_iter = self.max_iter
-        c.solver = SOLVER_PCG if self.solver == "pcg" else SOLVER_MG
+        c.solver = {"pcg": SOLVER_PCG, "none": SOLVER_NONE}.get(self.solver, SOLVER_MG)
         c.device = self.device
         c.sor_omega = self.sor_omega
         c.diag_clamp = self.diag_clamp

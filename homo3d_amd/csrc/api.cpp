@@ -1552,6 +1552,113 @@ int homo3d_oc_filter(homo3d_handle* h, float* sens, const float* rho, int pitch,
     return HOMO3D_OK;
 }
 
+// Elementwise density maps of the optimisation drivers: the reference forms rho^3, its chain rule and the
+// volume sum with its expression templates on the device (Framework.cu:137-200 rho.pow(3), Tensor::sum).
+int homo3d_density_pow(homo3d_handle* h, const float* rho, float* x, int pitch, double p, int on_device) {
+    if (!h || !rho || !x) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(x, on_device);
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const size_t elems = (size_t)pitch * n * n;
+    const float* dr = rho;
+    float* dx = x;
+    if (!on_device) {
+        int rc = ensure_stage(h, 2 * elems);
+        if (rc) return rc;
+        HIPCHK(h, hipMemcpyAsync(h->d_stage, rho, sizeof(float) * elems, hipMemcpyHostToDevice, h->stream));
+        dr = h->d_stage;
+        dx = h->d_stage + elems;
+    }
+    launch_density_pow(h->stream, n, pitch, (float)p, dr, dx);
+    if (!on_device) {
+        int rc = stage_out(h, x, dx, pitch, 0);
+        if (rc) return rc;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_density_pow_grad(homo3d_handle* h, const float* rho, const float* dx, float* drho, int pitch, double p,
+                            int on_device) {
+    if (!h || !rho || !dx || !drho) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(drho, on_device);
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const size_t elems = (size_t)pitch * n * n;
+    const float *dr = rho, *dd = dx;
+    float* dg = drho;
+    if (!on_device) {
+        int rc = ensure_stage(h, 3 * elems);
+        if (rc) return rc;
+        HIPCHK(h, hipMemcpyAsync(h->d_stage, rho, sizeof(float) * elems, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->d_stage + elems, dx, sizeof(float) * elems, hipMemcpyHostToDevice, h->stream));
+        dr = h->d_stage;
+        dd = h->d_stage + elems;
+        dg = h->d_stage + 2 * elems;
+    }
+    launch_density_pow_grad(h->stream, n, pitch, (float)p, dr, dd, dg);
+    if (!on_device) {
+        int rc = stage_out(h, drho, dg, pitch, 0);
+        if (rc) return rc;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+int homo3d_field_sum(homo3d_handle* h, const float* a, int pitch, double* sum, int on_device) {
+    if (!h || !a || !sum) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(a, on_device);
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    const float* da = a;
+    if (!on_device) {
+        int rc = stage_in(h, a, pitch, 0, &da);
+        if (rc) return rc;
+    }
+    constexpr int nb = 1024;
+    launch_field_sum(h->stream, n, pitch, da, h->d_partial, nb);
+    std::vector<double> part(nb);
+    HIPCHK(h, hipMemcpyAsync(part.data(), h->d_partial, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double t = 0;
+    for (double v : part) t += v;
+    *sum = t;
+    return HOMO3D_OK;
+}
+
+int homo3d_field_fill(homo3d_handle* h, float* a, int pitch, float value, int on_device) {
+    if (!h || !a) return HOMO3D_BAD_ARG;
+    on_device = resolve_on_device(a, on_device);
+    const int n = h->lv[0].n;
+    if (pitch < n) return set_err(h, HOMO3D_BAD_ARG, "pitch %d < n %d", pitch, n);
+    if (!on_device) {
+        for (int z = 0; z < n; z++)
+            for (int y = 0; y < n; y++)
+                for (int x = 0; x < n; x++) a[x + ((size_t)y + (size_t)z * n) * pitch] = value;
+        return HOMO3D_OK;
+    }
+    launch_field_fill(h->stream, n, pitch, value, a);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return HOMO3D_OK;
+}
+
+// Device memory for host code that does not include the HIP headers (the optimisation drivers keep their
+// density fields resident), on the calling thread's current device.
+int homo3d_device_alloc(void** ptr, size_t bytes) {
+    if (!ptr) return HOMO3D_BAD_ARG;
+    *ptr = nullptr;
+    if (bytes == 0) return HOMO3D_OK;
+    return hipMalloc(ptr, bytes) == hipSuccess ? HOMO3D_OK : HOMO3D_HIP_ERROR;
+}
+
+int homo3d_device_free(void* ptr) { return !ptr || hipFree(ptr) == hipSuccess ? HOMO3D_OK : HOMO3D_HIP_ERROR; }
+
+int homo3d_device_copy(void* dst, const void* src, size_t bytes, int kind) {
+    if (kind < 0 || kind > 2 || (bytes && (!dst || !src))) return HOMO3D_BAD_ARG;
+    const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    return bytes == 0 || hipMemcpy(dst, src, bytes, k[kind]) == hipSuccess ? HOMO3D_OK : HOMO3D_HIP_ERROR;
+}
+
 // OC update of rho in place (OCOptimizer::update, OCOptimizer.cu:218-247).
 // Density filter conv(rho, radial_convker_t<Spline4 | Linear>(radius)) (TensorExpression.h:1007-1046,
 // TensorExpression.cuh:288-368), forward (backward = 0) or its adjoint (backward = 1: in holds dJ/dout,

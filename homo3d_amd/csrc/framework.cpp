@@ -4,7 +4,9 @@
 //
 // The expression layer of the reference (TensorVar, Ch(i,j) autodiff) is replaced by explicit
 // objectives: each driver forms dJ/dC^H by hand, calls Homogenization::Sensitivity and applies the
-// chain rule of x = rho^3.
+// chain rule of x = rho^3.  The density fields of the loops (rho, x, sensitivities, gradients) stay in
+// device memory, as the reference's tensors do; only C^H, the objective and constraint values and the
+// logged densities cross to the host.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -130,22 +132,71 @@ struct DensityOps {
     ~DensityOps() {
         if (h) homo3d_destroy(h);
     }
-    void symmetrize(std::vector<float>& f, cfg::Symmetry s, bool usesym) {
-        if (!usesym || s == cfg::Symmetry::NONE) return;
-        int rc = homo3d_symmetrize(h, f.data(), n, sym_code(s), 0);
-        if (rc) throw Homo3dError(rc, std::string("symmetrize: ") + homo3d_last_error(h));
+    void chk(int rc, const char* what) const {
+        if (rc) throw Homo3dError(rc, std::string(what) + ": " + homo3d_last_error(h));
     }
+    // n^3 arrays with pitch n, in host or device memory (the C ABI tells them apart)
+    void symmetrize(float* f, cfg::Symmetry s, bool usesym) {
+        if (!usesym || s == cfg::Symmetry::NONE) return;
+        chk(homo3d_symmetrize(h, f, n, sym_code(s), -1), "symmetrize");
+    }
+    void symmetrize(std::vector<float>& f, cfg::Symmetry s, bool usesym) { symmetrize(f.data(), s, usesym); }
     // conv(radial_convker_t<Spline4>(radius)) (non-periodic, normalised) or its adjoint
-    void spline4_filter(const std::vector<float>& in, std::vector<float>& out, double radius, bool backward) {
-        out.resize(in.size());
-        int rc = homo3d_density_filter(h, in.data(), out.data(), n, radius, 0, 0, backward ? 1 : 0, 0);
-        if (rc) throw Homo3dError(rc, std::string("density filter: ") + homo3d_last_error(h));
+    void spline4_filter(const float* in, float* out, double radius, bool backward) {
+        chk(homo3d_density_filter(h, in, out, n, radius, 0, 0, backward ? 1 : 0, -1), "density filter");
     }
     // sgm(k = 30, c = 0.5) or its backward (y the forward output, in = dJ/dy)
-    void sgm(const std::vector<float>& in, const std::vector<float>* y, std::vector<float>& out) {
-        out.resize(in.size());
-        int rc = homo3d_projection(h, in.data(), y ? y->data() : nullptr, out.data(), n, 30.0, 0.5, y ? 1 : 0, 0);
-        if (rc) throw Homo3dError(rc, std::string("projection: ") + homo3d_last_error(h));
+    void sgm(const float* in, const float* y, float* out) {
+        chk(homo3d_projection(h, in, y, out, n, 30.0, 0.5, y ? 1 : 0, -1), "projection");
+    }
+    // x = rho^p and drho = p rho^(p-1) dx
+    void pow(const float* rho, float* x, double p) { chk(homo3d_density_pow(h, rho, x, n, p, -1), "density pow"); }
+    void pow_grad(const float* rho, const float* dx, float* drho, double p) {
+        chk(homo3d_density_pow_grad(h, rho, dx, drho, n, p, -1), "density pow grad");
+    }
+    double sum(const float* a) {
+        double t = 0;
+        chk(homo3d_field_sum(h, a, n, &t, -1), "field sum");
+        return t;
+    }
+    void fill(float* a, float v) { chk(homo3d_field_fill(h, a, n, v, -1), "field fill"); }
+};
+
+// An n^3 float array in device memory (the C ABI's allocation calls: this file needs no HIP headers).
+struct DevField {
+    float* p = nullptr;
+    size_t count = 0;
+    explicit DevField(size_t c = 0) { resize(c); }
+    DevField(const DevField&) = delete;
+    DevField& operator=(const DevField&) = delete;
+    DevField(DevField&& o) noexcept : p(o.p), count(o.count) {
+        o.p = nullptr;
+        o.count = 0;
+    }
+    ~DevField() {
+        if (p) homo3d_device_free(p);
+    }
+    void resize(size_t c) {
+        if (c == count) return;
+        if (p) homo3d_device_free(p);
+        p = nullptr;
+        count = c;
+        if (!c) return;
+        void* q = nullptr;
+        int rc = homo3d_device_alloc(&q, sizeof(float) * c);
+        if (rc) throw Homo3dError(rc, "device allocation of " + std::to_string(c) + " floats");
+        p = static_cast<float*>(q);
+    }
+    void upload(const std::vector<float>& h) {
+        resize(h.size());
+        int rc = homo3d_device_copy(p, h.data(), sizeof(float) * count, 0);
+        if (rc) throw Homo3dError(rc, "upload");
+    }
+    std::vector<float> download() const {
+        std::vector<float> h(count);
+        int rc = homo3d_device_copy(h.data(), p, sizeof(float) * count, 1);
+        if (rc) throw Homo3dError(rc, "download");
+        return h;
     }
 };
 
@@ -258,13 +309,15 @@ void initDensity(std::vector<float>& rho, const cfg::HomoConfig& c, DensityOps& 
 
 // logIter (Framework.cu:7-58): C every 5 iterations (36 float32, row-major), Clog/objlog text logs,
 // rho as raw float32 (rho_XXXX when logrho divides the iteration).
-void logIter(int iter, const cfg::HomoConfig& c, const std::vector<float>& rho, const double C[6][6], double obj) {
+void logIter(int iter, const cfg::HomoConfig& c, const DevField& rho_dev, const double C[6][6], double obj) {
+    std::vector<float> rho;  // read back only when a density file is due
     auto write_C = [&](const std::string& f) {
         float Cf[36];
         for (int i = 0; i < 36; i++) Cf[i] = (float)C[i / 6][i % 6];
         std::ofstream(f, std::ios::binary).write(reinterpret_cast<const char*>(Cf), sizeof(Cf));
     };
     auto write_rho = [&](const std::string& f) {
+        if (rho.empty()) rho = rho_dev.download();
         std::ofstream(f, std::ios::binary).write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
     };
     if (iter % 5 == 0) {
@@ -291,19 +344,20 @@ void logIter(int iter, const cfg::HomoConfig& c, const std::vector<float>& rho, 
 // One forward/backward pass: x = rho^3 -> C^H, then dJ/drho = 3 rho^2 * dJ/dx for dJ/dC = dC.
 struct Evaluator {
     Homogenization& hom;
-    std::vector<float> x, sens;
-    explicit Evaluator(Homogenization& h) : hom(h) {}
-    void forward(const std::vector<float>& rho, double C[6][6]) {
-        x.resize(rho.size());
-        for (size_t i = 0; i < rho.size(); i++) x[i] = rho[i] * rho[i] * rho[i];
-        hom.update(x.data(), hom.grid->cellReso[0]);
+    DensityOps& ops;
+    DevField x, sens;
+    Evaluator(Homogenization& h, DensityOps& o) : hom(h), ops(o) {}
+    void forward(const DevField& rho, double C[6][6]) {
+        x.resize(rho.count);
+        ops.pow(rho.p, x.p, 3.0);
+        hom.update(x.p, hom.grid->cellReso[0]);
         hom.elasticMatrix(C);
     }
-    void backward(const std::vector<float>& rho, float dC[6][6], std::vector<float>& grad) {
-        sens.resize(rho.size());
-        hom.Sensitivity(dC, sens.data(), hom.grid->cellReso[0], true);
-        grad.resize(rho.size());
-        for (size_t i = 0; i < rho.size(); i++) grad[i] = 3.f * rho[i] * rho[i] * sens[i];
+    void backward(const DevField& rho, float dC[6][6], DevField& grad) {
+        sens.resize(rho.count);
+        hom.Sensitivity(dC, sens.p, hom.grid->cellReso[0], true);
+        grad.resize(rho.count);
+        ops.pow_grad(rho.p, sens.p, grad.p, 3.0);
     }
 };
 
@@ -313,11 +367,15 @@ using ObjectiveFn = std::function<double(int iter, const double C[6][6], float d
 std::vector<double> run_oc(cfg::HomoConfig config, const ObjectiveFn& objective) {
     Homogenization hom(config);
     DensityOps ops(config.reso[0]);
-    std::vector<float> rho, grad;
-    initDensity(rho, config, ops);
-    std::ofstream(out_path(config, "initRho"), std::ios::binary)
-        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
-    Evaluator ev(hom);
+    DevField rho, grad;
+    {
+        std::vector<float> r0;
+        initDensity(r0, config, ops);
+        std::ofstream(out_path(config, "initRho"), std::ios::binary)
+            .write(reinterpret_cast<const char*>(r0.data()), sizeof(float) * r0.size());
+        rho.upload(r0);
+    }
+    Evaluator ev(hom, ops);
     OCOptimizer oc(0.001f, (float)config.designStep, (float)config.dampRatio);
     oc.periodic = config.periodfilt;
     ConvergeChecker criteria(config.finthres);
@@ -337,14 +395,17 @@ std::vector<double> run_oc(cfg::HomoConfig config, const ObjectiveFn& objective)
             std::printf("= converged\n");
             break;
         }
-        ops.symmetrize(grad, config.sym, config.usesym);
-        oc.filterSens(grad.data(), rho.data(), reso[0], reso, (float)config.filterRadius);
-        oc.update(grad.data(), rho.data(), (float)config.volRatio);
-        ops.symmetrize(rho, config.sym, config.usesym);
+        ops.symmetrize(grad.p, config.sym, config.usesym);
+        oc.filterSens(grad.p, rho.p, reso[0], reso, (float)config.filterRadius);
+        oc.update(grad.p, rho.p, (float)config.volRatio);
+        ops.symmetrize(rho.p, config.sym, config.usesym);
         logIter(iter, config, rho, C, val);
     }
-    std::ofstream(out_path(config, "rho"), std::ios::binary)
-        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
+    {
+        const std::vector<float> rh = rho.download();
+        std::ofstream(out_path(config, "rho"), std::ios::binary)
+            .write(reinterpret_cast<const char*>(rh.data()), sizeof(float) * rh.size());
+    }
     float Cf[36];
     for (int i = 0; i < 36; i++) Cf[i] = (float)C[i / 6][i % 6];
     std::ofstream(out_path(config, "C"), std::ios::binary).write(reinterpret_cast<const char*>(Cf), sizeof(Cf));
@@ -386,12 +447,16 @@ using ConstraintFn = std::function<std::vector<MmaConstraint>(const double C[6][
 std::vector<double> run_mma(cfg::HomoConfig config, const ObjectiveFn& objective, const ConstraintFn& extra) {
     Homogenization hom(config);
     DensityOps ops(config.reso[0]);
-    std::vector<float> rho, grad;
-    initDensity(rho, config, ops);
-    std::ofstream(out_path(config, "initRho"), std::ios::binary)
-        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
-    Evaluator ev(hom);
-    const int ne = (int)rho.size();
+    DevField rho, grad;
+    {
+        std::vector<float> r0;
+        initDensity(r0, config, ops);
+        std::ofstream(out_path(config, "initRho"), std::ios::binary)
+            .write(reinterpret_cast<const char*>(r0.data()), sizeof(float) * r0.size());
+        rho.upload(r0);
+    }
+    Evaluator ev(hom, ops);
+    const int ne = (int)rho.count;
     double C[6][6];
     ev.forward(rho, C);
     const int m = 1 + (int)extra(C).size();
@@ -399,7 +464,9 @@ std::vector<double> run_mma(cfg::HomoConfig config, const ObjectiveFn& objective
     mma.setBound(0.001f, 1.f);
     ConvergeChecker criteria(config.finthres);
     std::vector<double> objlist;
-    std::vector<std::vector<float>> dg(m, std::vector<float>(ne));
+    std::vector<DevField> dg;
+    for (int i = 0; i < m; i++) dg.emplace_back(ne);
+    ops.fill(dg[0].p, (float)(1000.0 / ne));  // the volume constraint's gradient
     for (int iter = 0; iter < config.max_iter; iter++) {
         if (iter > 0) ev.forward(rho, C);
         float dC[6][6] = {};
@@ -411,29 +478,28 @@ std::vector<double> run_mma(cfg::HomoConfig config, const ObjectiveFn& objective
             std::printf("= converged\n");
             break;
         }
-        ops.symmetrize(grad, config.sym, config.usesym);
+        ops.symmetrize(grad.p, config.sym, config.usesym);
         // volume constraint g0 = (mean(rho) - vol) * 1000
-        double vsum = 0;
-        for (float r : rho) vsum += r;
+        const double vsum = ops.sum(rho.p);
         std::vector<float> gval(m);
         gval[0] = (float)((vsum / ne - config.volRatio) * 1000.0);
-        std::fill(dg[0].begin(), dg[0].end(), (float)(1000.0 / ne));
         auto ex = extra(C);
         for (size_t k = 0; k < ex.size(); k++) {
             gval[1 + k] = (float)ex[k].value;
-            std::vector<float> gk;
-            ev.backward(rho, ex[k].dC, gk);
-            ops.symmetrize(gk, config.sym, config.usesym);
-            dg[1 + k] = gk;
+            ev.backward(rho, ex[k].dC, dg[1 + k]);
+            ops.symmetrize(dg[1 + k].p, config.sym, config.usesym);
         }
         std::vector<float*> dgp(m);
-        for (int i = 0; i < m; i++) dgp[i] = dg[i].data();
-        mma.update(iter, rho.data(), grad.data(), gval.data(), dgp.data());
-        ops.symmetrize(rho, config.sym, config.usesym);
+        for (int i = 0; i < m; i++) dgp[i] = dg[i].p;
+        mma.update(iter, rho.p, grad.p, gval.data(), dgp.data());
+        ops.symmetrize(rho.p, config.sym, config.usesym);
         logIter(iter, config, rho, C, val);
     }
-    std::ofstream(out_path(config, "rho"), std::ios::binary)
-        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
+    {
+        const std::vector<float> rh = rho.download();
+        std::ofstream(out_path(config, "rho"), std::ios::binary)
+            .write(reinterpret_cast<const char*>(rh.data()), sizeof(float) * rh.size());
+    }
     std::ofstream ol(out_path(config, "objlist"));
     for (double v : objlist) ol << v << "\n";
     return objlist;
@@ -472,29 +538,33 @@ MmaConstraint shear_isotropy_constraint(const double C[6][6]) {
 std::vector<double> example_opti_shear_isotropy(cfg::HomoConfig config) {
     Homogenization hom(config);
     DensityOps ops(config.reso[0]);
-    std::vector<float> rho;
-    initDensity(rho, config, ops);
-    std::ofstream(out_path(config, "initRho"), std::ios::binary)
-        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
-    const int ne = (int)rho.size();
+    DevField rho;
+    {
+        std::vector<float> r0;
+        initDensity(r0, config, ops);
+        std::ofstream(out_path(config, "initRho"), std::ios::binary)
+            .write(reinterpret_cast<const char*>(r0.data()), sizeof(float) * r0.size());
+        rho.upload(r0);
+    }
+    const int ne = (int)rho.count;
     MMAOptimizer mma(2, ne, 1, 0, 1000, 1);
     mma.setBound(0.001f, 1.f);
     ConvergeChecker criteria(config.finthres);
     std::vector<double> objlist;
-    std::vector<float> c, rhat, x(ne), sens(ne), t(ne), t2, objGrad, gGrad, volGrad;
+    DevField c(ne), rhat(ne), x(ne), sens(ne), t(ne), t2(ne), objGrad(ne), gGrad(ne), volGrad(ne);
     // dJ/drho for dJ/dC = dC through x = rhat^3, rhat = sgm(c), c = conv(rho)
-    auto backward = [&](float dC[6][6], std::vector<float>& grad) {
-        hom.Sensitivity(dC, sens.data(), hom.grid->cellReso[0], true);
-        for (int i = 0; i < ne; i++) t[i] = 3.f * rhat[i] * rhat[i] * sens[i];
-        ops.sgm(t, &rhat, t2);
-        ops.spline4_filter(t2, grad, config.filterRadius, true);
+    auto backward = [&](float dC[6][6], DevField& grad) {
+        hom.Sensitivity(dC, sens.p, hom.grid->cellReso[0], true);
+        ops.pow_grad(rhat.p, sens.p, t.p, 3.0);
+        ops.sgm(t.p, rhat.p, t2.p);
+        ops.spline4_filter(t2.p, grad.p, config.filterRadius, true);
     };
     double C[6][6];
     for (int iter = 0; iter < config.max_iter; iter++) {
-        ops.spline4_filter(rho, c, config.filterRadius, false);
-        ops.sgm(c, nullptr, rhat);
-        for (int i = 0; i < ne; i++) x[i] = rhat[i] * rhat[i] * rhat[i];
-        hom.update(x.data(), hom.grid->cellReso[0]);
+        ops.spline4_filter(rho.p, c.p, config.filterRadius, false);
+        ops.sgm(c.p, nullptr, rhat.p);
+        ops.pow(rhat.p, x.p, 3.0);
+        hom.update(x.p, hom.grid->cellReso[0]);
         hom.elasticMatrix(C);
         float dC[6][6] = {};
         const double val = shear_objective(C, dC);
@@ -505,25 +575,26 @@ std::vector<double> example_opti_shear_isotropy(cfg::HomoConfig config) {
             std::printf("= converged\n");
             break;
         }
-        ops.symmetrize(objGrad, config.sym, config.usesym);
+        ops.symmetrize(objGrad.p, config.sym, config.usesym);
         MmaConstraint k = shear_isotropy_constraint(C);
         backward(k.dC, gGrad);
-        ops.symmetrize(gGrad, config.sym, config.usesym);
-        double vsum = 0;
-        for (float r : rhat) vsum += r;
-        const double vol_ratio = vsum / ne;
+        ops.symmetrize(gGrad.p, config.sym, config.usesym);
+        const double vol_ratio = ops.sum(rhat.p) / ne;
         float gval[2] = {(float)((vol_ratio - config.volRatio) * 1000.0), (float)k.value};
-        std::fill(t.begin(), t.end(), 1.f / ne);
-        ops.sgm(t, &rhat, t2);
-        ops.spline4_filter(t2, volGrad, config.filterRadius, true);
+        ops.fill(t.p, 1.f / ne);
+        ops.sgm(t.p, rhat.p, t2.p);
+        ops.spline4_filter(t2.p, volGrad.p, config.filterRadius, true);
         std::printf("zener ratio = %4.2e ; obj = %4.2e ; vol = %4.2e\n", std::sqrt(std::max(k.value + 0.1, 0.0) / 1e6) + 1,
                     val, vol_ratio);
-        float* dgdx[2] = {volGrad.data(), gGrad.data()};
-        mma.update(iter, rho.data(), objGrad.data(), gval, dgdx);
+        float* dgdx[2] = {volGrad.p, gGrad.p};
+        mma.update(iter, rho.p, objGrad.p, gval, dgdx);
         logIter(iter, config, rho, C, val);
     }
-    std::ofstream(out_path(config, "rho"), std::ios::binary)
-        .write(reinterpret_cast<const char*>(rho.data()), sizeof(float) * rho.size());
+    {
+        const std::vector<float> rh = rho.download();
+        std::ofstream(out_path(config, "rho"), std::ios::binary)
+            .write(reinterpret_cast<const char*>(rh.data()), sizeof(float) * rh.size());
+    }
     float Cf[36];
     for (int i = 0; i < 36; i++) Cf[i] = (float)C[i / 6][i % 6];
     std::ofstream(out_path(config, "C"), std::ios::binary).write(reinterpret_cast<const char*>(Cf), sizeof(Cf));

@@ -122,6 +122,11 @@ void launch_oc_filter(hipStream_t s, int n, int pitch, int h, int periodic, cons
 void launch_oc_trial(hipStream_t s, int n, int pitch, double g, double min_rho, double step, double damp,
                      const float* sens, const float* rho, float* out, double* partial, int nblocks, double* sum);
 void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* partial, int nblocks);
+// x = rho^p; drho = p rho^(p - 1) dx (drho may alias dx); FP64 block sums of a (nblocks partials); a = value
+void launch_density_pow(hipStream_t s, int n, int pitch, float p, const float* rho, float* x);
+void launch_density_pow_grad(hipStream_t s, int n, int pitch, float p, const float* rho, const float* dx, float* drho);
+void launch_field_sum(hipStream_t s, int n, int pitch, const float* a, double* partial, int nblocks);
+void launch_field_fill(hipStream_t s, int n, int pitch, float value, float* a);
 void launch_set_weights(hipStream_t s, int n, int pitch, double emin, const float* x, float* w, float* xc);
 void launch_copy_pitched(hipStream_t s, int n, int pitch_in, int pitch_out, const float* in, float* out);
 

@@ -339,6 +339,17 @@ __device__ __forceinline__ void block_fma(const T (&S)[9], const A (&cur)[W][3],
     }
 }
 
+// Keep a neighbour's products at its step: the products read values loaded one step earlier, and without
+// this the instruction selector may sink all of them below the last scheduling barrier, keeping every
+// neighbour's operands live (hundreds of spilled registers).  The empty volatile statement emits nothing.
+template <int R, typename T>
+__device__ __forceinline__ void pin_acc(T (&acc)[R][3]) {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) __asm__ volatile("" : "+v"(acc[r][i]));
+}
+
 // One neighbour of the row: issue the loads of the next neighbour, build the 3x3 block of this one
 // from the cell weights and KE (in LDS), accumulate, recurse.
 template <int R, uint32_t MASK, int D, typename T, typename A, bool XL>
@@ -349,6 +360,7 @@ __device__ __forceinline__ void gather_step(const FineCtx& c, int n, const T* ke
     constexpr int DN = next_nb(MASK, D);
     // keep each neighbour's block build next to its use (bounds register pressure)
     __builtin_amdgcn_sched_barrier(0);
+    pin_acc<R, T>(acc);
     A nxt[W][3];
     if constexpr (DN < 27) load_nb_at<W, A, XL, DN>(nxt, c, ru, plane, r0, xl);
     T S[9];
@@ -370,6 +382,7 @@ __device__ __forceinline__ void gather_step2(const FineCtx& c, int n, const T* k
     constexpr int DN = next_nb(MASK, D);
     constexpr int DN2 = DN < 27 ? next_nb(MASK, DN) : 27;
     __builtin_amdgcn_sched_barrier(0);
+    pin_acc<R, T>(acc);
     A nx2[W][3];
     if constexpr (DN2 < 27) load_nb<W, A>(nx2, ru, c.nb(DN2) * (uint32_t)sizeof(A), plane, r0);
     T S[9];
@@ -568,6 +581,50 @@ __device__ __forceinline__ void staged_step(const FineCtx& c, const float* ke, c
     if constexpr (D < 26) staged_step<D + 1>(c, ke, zs, xb, nxt, acc);
 }
 
+// Load the workgroup's node box of the packed-FP16 z (tile of chunk t0: half-x bx0.. bx0 + 63, half-y by,
+// half-z bz) into zs: position pos = ((lz * 4 + ly) * 2 + s) * 65 + k of the box, one address and its
+// nine words per position.  The caller publishes it with a workgroup barrier.
+__device__ __forceinline__ void stage_z_box(uint32_t* zs, const uint32_t* __restrict__ z, const Lay& L, size_t N,
+                                            int bx0, int by, int bz) {
+    const int h = L.h;
+    const Rsrc rz = make_rsrc(z, 9 * N * sizeof(uint32_t));
+    const uint32_t plane = (uint32_t)(N * sizeof(uint32_t));
+    uint32_t tmp[kStIter][9];
+#pragma unroll
+    for (int it = 0; it < kStIter; it++) {
+        const int pos = it * kFullThreads + (int)threadIdx.x;
+        if (kStPos % kFullThreads == 0 || pos < kStPos) {
+            const int row = pos / kStK, k = pos - row * kStK;
+            const int s = row & 1, ly = (row >> 1) & 3, lz = row >> 3;
+            int hx = bx0 - s + k;
+            hx = hx < 0 ? hx + h : (hx >= h ? hx - h : hx);
+            const int yy = L.wrap(2 * by - 1 + ly), zz = L.wrap(2 * bz - 1 + lz);
+            const uint32_t blk = (uint32_t)(s | (yy & 1) << 1 | (zz & 1) << 2);
+            const uint32_t id = blk * L.h3 + (uint32_t)hx +
+                                (uint32_t)h * ((uint32_t)(yy >> 1) + (uint32_t)h * (uint32_t)(zz >> 1));
+            hload9(tmp[it], rz, id, plane);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kStIter; it++) {
+        const int pos = it * kFullThreads + (int)threadIdx.x;
+        if (kStPos % kFullThreads == 0 || pos < kStPos) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) zs[q * kStPlane + pos] = tmp[it][q];
+        }
+    }
+}
+
+// word index of the x-neighbours dx = -1, 0, 1 of a thread's vertex in the staged box (row (py, pz) of
+// every plane): parity s' = (px + dx) & 1, half-x position k = lane + (px + dx - s') / 2 + s'
+__device__ __forceinline__ void staged_xb(int (&xb)[3], int lane, int px, int py, int pz) {
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+        const int sp = (px + dx) & 1;
+        xb[dx + 1] = lane + (pz * 4 + py) * kStRow + sp * kStK + (px + dx - sp) / 2 + sp;
+    }
+}
+
 template <bool BETA, int PX>
 __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_t N, const float* __restrict__ ke,
                                                                     const float* __restrict__ w,
@@ -608,33 +665,7 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
                     xo[r][i] = bload<float>(rx, vo, (uint32_t)(r * 3 + i) * fplane);
                 }
         }
-        // position pos = ((lz * 4 + ly) * 2 + s) * 65 + k of the box: one address, nine word planes
-        const Rsrc rz = make_rsrc(z, 9 * N * sizeof(uint32_t));
-        const uint32_t plane = (uint32_t)(N * sizeof(uint32_t));
-        uint32_t tmp[kStIter][9];
-#pragma unroll
-        for (int it = 0; it < kStIter; it++) {
-            const int pos = it * kFullThreads + (int)threadIdx.x;
-            if (kStPos % kFullThreads == 0 || pos < kStPos) {
-                const int row = pos / kStK, k = pos - row * kStK;
-                const int s = row & 1, ly = (row >> 1) & 3, lz = row >> 3;
-                int hx = bx0 - s + k;
-                hx = hx < 0 ? hx + h : (hx >= h ? hx - h : hx);
-                const int yy = L.wrap(2 * by - 1 + ly), zz = L.wrap(2 * bz - 1 + lz);
-                const uint32_t blk = (uint32_t)(s | (yy & 1) << 1 | (zz & 1) << 2);
-                const uint32_t id = blk * L.h3 + (uint32_t)hx +
-                                    (uint32_t)h * ((uint32_t)(yy >> 1) + (uint32_t)h * (uint32_t)(zz >> 1));
-                hload9(tmp[it], rz, id, plane);
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < kStIter; it++) {
-            const int pos = it * kFullThreads + (int)threadIdx.x;
-            if (kStPos % kFullThreads == 0 || pos < kStPos) {
-#pragma unroll
-                for (int q = 0; q < 9; q++) zs[q * kStPlane + pos] = tmp[it][q];
-            }
-        }
+        stage_z_box(zs, z, L, N, bx0, by, bz);
         if constexpr (PX == 2) {
 #pragma unroll
             for (int r = 0; r < 6; r++) {
@@ -653,13 +684,8 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
     const int x = 2 * (bx0 + lane) + px, y = 2 * by + py, zc = 2 * bz + pz;
     FineCtx c;
     fine_ctx(c, x, y, zc, L, w);
-    // x-neighbour dx: parity s' = (px + dx) & 1, half-x position k = lane + (px + dx - s') / 2 + s'
     int xb[3];
-#pragma unroll
-    for (int dx = -1; dx <= 1; dx++) {
-        const int sp = (px + dx) & 1;
-        xb[dx + 1] = lane + (pz * 4 + py) * kStRow + sp * kStK + (px + dx - sp) / 2 + sp;
-    }
+    staged_xb(xb, lane, px, py, pz);
     float acc[6][3];
 #pragma unroll
     for (int r = 0; r < 6; r++)
@@ -1097,7 +1123,7 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour,
     const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.h3) return;
     gs_vertex<R, REVERSE, ZC, A, false, false>(L, N, colour_block(colour) * L.h3 + t, omega, ke_s, w, u, f,
-                                              blockIdx.y * R, XNbLds<A>(), nullptr);
+                                              R == 6 ? 0 : (int)blockIdx.y * R, XNbLds<A>(), nullptr);
 }
 
 // Two consecutive colours of a sweep in one launch: cfirst, then cfirst + 1 (forward) or cfirst - 1
@@ -1468,13 +1494,7 @@ __device__ __forceinline__ void stencil_step(const StencilRow& c, int r0, const 
     constexpr int W = R / CasesPerWord<A>::value;
     constexpr int DN = next_nb(MASK, D);
     __builtin_amdgcn_sched_barrier(0);
-    // the block products only read values loaded one step earlier, so without this the instruction
-    // selector may sink all of them below the last barrier, keeping every neighbour's operands live:
-    // pass the accumulators through an empty volatile statement here (no instruction is emitted)
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) __asm__ volatile("" : "+v"(acc[r][i]));
+    pin_acc<R, float>(acc);
     float Sn[9];
     A nxt[W][3];
     if constexpr (DN < 27) stencil_nb<SA, A, W, XL, DN>(Sn, nxt, c, r0, xl);
@@ -2612,6 +2632,47 @@ __global__ __launch_bounds__(256) void k_oc_apply(int n, int pitch, double g, do
     }
 }
 
+// Elementwise density maps of the optimisation loop (rho.pow(p) of the reference's expression templates
+// and its chain rule), over the n^3 live cells of pitched lexicographic arrays.  p = 3 is evaluated as
+// r * r * r and 3 r r d, the products the host loops of the drivers form.
+__device__ __forceinline__ size_t pitched(long e, int n, int pitch) {
+    const int x = (int)(e % n), y = (int)((e / n) % n), z = (int)(e / ((long)n * n));
+    return x + ((size_t)y + (size_t)z * n) * pitch;
+}
+__global__ __launch_bounds__(256) void k_density_pow(int n, int pitch, float p, const float* __restrict__ rho,
+                                                     float* __restrict__ x) {
+    const long N = (long)n * n * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const size_t id = pitched(e, n, pitch);
+        const float r = rho[id];
+        x[id] = p == 3.f ? r * r * r : powf(r, p);
+    }
+}
+__global__ __launch_bounds__(256) void k_density_pow_grad(int n, int pitch, float p, const float* __restrict__ rho,
+                                                          const float* dx, float* drho) {
+    const long N = (long)n * n * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const size_t id = pitched(e, n, pitch);
+        const float r = rho[id];
+        drho[id] = p == 3.f ? 3.f * r * r * dx[id] : p * powf(r, p - 1.f) * dx[id];
+    }
+}
+__global__ __launch_bounds__(256) void k_field_sum(int n, int pitch, const float* __restrict__ a,
+                                                   double* __restrict__ partial) {
+    __shared__ double sh[4];
+    const long N = (long)n * n * n;
+    double s = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x)
+        s += (double)a[pitched(e, n, pitch)];
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(256) void k_field_fill(int n, int pitch, float value, float* __restrict__ a) {
+    const long N = (long)n * n * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x)
+        a[pitched(e, n, pitch)] = value;
+}
+
 __global__ __launch_bounds__(256) void k_absmax(long N, int n, int pitch, const float* __restrict__ a,
                                                 float* __restrict__ partial) {
     __shared__ float sh[256];
@@ -3336,6 +3397,19 @@ void launch_oc_trial(hipStream_t s, int n, int pitch, double g, double min_rho, 
     hipLaunchKernelGGL(k_oc_trial, dim3(nblocks), dim3(256), 0, s, n, pitch, g, min_rho, step, damp, sens, rho, out,
                        partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kSumThreads), 0, s, nblocks, partial, sum);
+}
+
+void launch_density_pow(hipStream_t s, int n, int pitch, float p, const float* rho, float* x) {
+    hipLaunchKernelGGL(k_density_pow, dim3(1024), dim3(256), 0, s, n, pitch, p, rho, x);
+}
+void launch_density_pow_grad(hipStream_t s, int n, int pitch, float p, const float* rho, const float* dx, float* drho) {
+    hipLaunchKernelGGL(k_density_pow_grad, dim3(1024), dim3(256), 0, s, n, pitch, p, rho, dx, drho);
+}
+void launch_field_sum(hipStream_t s, int n, int pitch, const float* a, double* partial, int nblocks) {
+    hipLaunchKernelGGL(k_field_sum, dim3(nblocks), dim3(256), 0, s, n, pitch, a, partial);
+}
+void launch_field_fill(hipStream_t s, int n, int pitch, float value, float* a) {
+    hipLaunchKernelGGL(k_field_fill, dim3(1024), dim3(256), 0, s, n, pitch, value, a);
 }
 
 void launch_absmax(hipStream_t s, int n, int pitch, const float* a, float* partial, int nblocks) {

@@ -52,6 +52,7 @@
 #ifndef HOMO3D_H
 #define HOMO3D_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -193,6 +194,22 @@ int homo3d_density_filter(homo3d_handle* h, const float* in, float* out, int pit
  * in = dJ/dy, out = dJ/dx = in k y (1 - y). */
 int homo3d_projection(homo3d_handle* h, const float* in, const float* y, float* out, int pitch, double k, double c,
                       int backward, int on_device);
+
+/* Elementwise density maps of the optimisation loop, which the reference forms with its device expression
+ * templates (rho.pow(3) and its chain rule, Tensor::sum; Framework.cu:137-356): x = rho^p;
+ * drho = p rho^(p-1) dx (drho may alias dx); *sum = FP64 sum over the n^3 cells; a = value.  Pitched
+ * lexicographic arrays on the handle's grid; on_device as elsewhere (-1: from the pointer). */
+int homo3d_density_pow(homo3d_handle* h, const float* rho, float* x, int pitch, double p, int on_device);
+int homo3d_density_pow_grad(homo3d_handle* h, const float* rho, const float* dx, float* drho, int pitch, double p,
+                            int on_device);
+int homo3d_field_sum(homo3d_handle* h, const float* a, int pitch, double* sum, int on_device);
+int homo3d_field_fill(homo3d_handle* h, float* a, int pitch, float value, int on_device);
+/* Device memory on the calling thread's current device, for host code without the HIP headers (no
+ * reference counterpart: the reference allocates through CUDA directly).  kind: 0 host to device, 1 device
+ * to host, 2 device to device; the copy is synchronous. */
+int homo3d_device_alloc(void** ptr, size_t bytes);
+int homo3d_device_free(void* ptr);
+int homo3d_device_copy(void* dst, const void* src, size_t bytes, int kind);
 
 /* homo3d_oc_update without any allocation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that

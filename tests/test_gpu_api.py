@@ -219,3 +219,67 @@ def test_staged_cg_pass_matches_gather_form(h3d, n):
     assert res[True][0] == res[False][0]
     assert np.array_equal(res[True][1], res[False][1])
     assert np.array_equal(res[True][2], res[False][2])
+
+
+def test_density_maps_and_device_memory(h3d):
+    """x = rho^p, drho = p rho^(p-1) dx, the FP64 field sum and fill (the element maps the optimisation
+    drivers keep on the device) against NumPy, on host and device arrays with row padding; the C ABI's
+    device allocation and copies round-trip."""
+    import ctypes
+
+    n, pitch = 20, 24
+    L = h3d.lib()
+    rng = np.random.default_rng(11)
+    rho = np.zeros((n, n, pitch), np.float32)
+    rho[:, :, :n] = (0.001 + rng.random((n, n, n))).astype(np.float32)
+    dx = np.zeros_like(rho)
+    dx[:, :, :n] = rng.standard_normal((n, n, n)).astype(np.float32)
+    ops = h3d.Homogenization(h3d.HomoConfig(reso=n, solver="none"))
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    r, d = rho[:, :, :n], dx[:, :, :n]
+    want = {(3.0, "pow"): r * r * r, (3.0, "grad"): 3.0 * r * r * d,
+            (2.5, "pow"): np.power(r, np.float32(2.5)), (2.5, "grad"): np.float32(2.5) * np.power(r, np.float32(1.5)) * d}
+    for dev in (0, 1):
+        for p in (3.0, 2.5):
+            x = np.full_like(rho, 7.0)
+            g = np.full_like(rho, 7.0)
+            if dev:
+                bufs = [h3d.DeviceBuffer.from_array(a) for a in (rho, dx, x, g)]
+                assert L.homo3d_density_pow(ops._h, bufs[0].value, bufs[2].value, pitch, p, 1) == 0
+                assert L.homo3d_density_pow_grad(ops._h, bufs[0].value, bufs[1].value, bufs[3].value, pitch, p, 1) == 0
+                x = bufs[2].download(np.float32).reshape(rho.shape)
+                g = bufs[3].download(np.float32).reshape(rho.shape)
+            else:
+                assert L.homo3d_density_pow(ops._h, vp(rho), vp(x), pitch, p, 0) == 0
+                assert L.homo3d_density_pow_grad(ops._h, vp(rho), vp(dx), vp(g), pitch, p, 0) == 0
+            tol = 0 if p == 3.0 else 4e-7
+            assert np.all(x[:, :, n:] == 7.0) and np.all(g[:, :, n:] == 7.0)
+            assert np.abs(x[:, :, :n] - want[(p, "pow")]).max() <= tol * np.abs(want[(p, "pow")]).max()
+            assert np.abs(g[:, :, :n] - want[(p, "grad")]).max() <= tol * np.abs(want[(p, "grad")]).max()
+        s = ctypes.c_double()
+        src = h3d.DeviceBuffer.from_array(rho) if dev else None
+        assert L.homo3d_field_sum(ops._h, src.value if dev else vp(rho), pitch, ctypes.byref(s), dev) == 0
+        assert abs(s.value - r.astype(np.float64).sum()) <= 1e-9 * r.sum()
+        f = np.full_like(rho, 7.0)
+        if dev:
+            fb = h3d.DeviceBuffer.from_array(f)
+            assert L.homo3d_field_fill(ops._h, fb.value, pitch, ctypes.c_float(0.25), 1) == 0
+            f = fb.download(np.float32).reshape(rho.shape)
+        else:
+            assert L.homo3d_field_fill(ops._h, vp(f), pitch, ctypes.c_float(0.25), 0) == 0
+        assert np.all(f[:, :, :n] == 0.25) and np.all(f[:, :, n:] == 7.0)
+        say("  density maps on_device=%d ok" % dev)
+    # device memory through the C ABI
+    ptr = ctypes.c_void_p()
+    a = rng.standard_normal(1000).astype(np.float32)
+    b = np.zeros_like(a)
+    assert L.homo3d_device_alloc(ctypes.byref(ptr), a.nbytes) == 0 and ptr.value
+    ptr2 = ctypes.c_void_p()
+    assert L.homo3d_device_alloc(ctypes.byref(ptr2), a.nbytes) == 0
+    assert L.homo3d_device_copy(ptr, vp(a), a.nbytes, 0) == 0
+    assert L.homo3d_device_copy(ptr2, ptr, a.nbytes, 2) == 0
+    assert L.homo3d_device_copy(vp(b), ptr2, a.nbytes, 1) == 0
+    assert np.array_equal(a, b)
+    assert L.homo3d_device_copy(ptr2, ptr, a.nbytes, 3) == 3  # bad kind
+    assert L.homo3d_device_free(ptr) == 0 and L.homo3d_device_free(ptr2) == 0
+    ops.close()
