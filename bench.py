@@ -75,13 +75,37 @@ def measured_traffic(name, n):
     return sum(vals) / len(vals) if vals else None
 
 
-def cpu_baseline(n_sample, n_target, tol):
-    """The FP64 oracle's solve of the same design (Galerkin set-up + six load cases by MG-PCG), timed on
-    the host threads.  Returns the baseline record and the oracle's C^H."""
+def cpu_baseline(n_sample, n_target, tol, iters):
+    """The FP64 oracle (test infrastructure) on the host threads, as a bounded sample of the same solve.
+
+    At the target size (the default): the full Galerkin set-up of the design's hierarchy is timed, then one
+    operator application K u and one symmetric V-cycle, the work of one CG step of the oracle's MG-PCG; the
+    six load cases are extrapolated as (steps + 1) of those each (the start forms one residual and one
+    V-cycle), with `iters` the CG step counts of the measured GPU solve.  With a smaller --cpu-n: one full
+    solve there, scaled by the cell count."""
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     rho = oracle.randc_density(n_sample, 0, 0.2, "rotate3").astype(np.float64)
+    if n_sample == n_target:
+        t0 = time.perf_counter()
+        mg = oracle.MG(n_sample)
+        mg.set_density(rho ** 3)
+        t_setup = time.perf_counter() - t0
+        f = oracle.macro_rhs(n_sample, mg.KE, mg.w, 0)
+        u = np.zeros_like(f)
+        t0 = time.perf_counter()
+        oracle.apply_K(n_sample, mg.KE, mg.w, f)
+        t_k = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        mg.vcycle(u, f, symmetric=True)
+        t_v = time.perf_counter() - t0
+        del mg, f, u
+        total = t_setup + sum((k + 1) * (t_k + t_v) for k in iters)
+        sample = ("%d^3 measured: FP64 oracle on %d threads, Galerkin set-up %.1f s, one K u %.2f s, one symmetric "
+                  "V-cycle %.2f s; six load cases at (CG steps + 1) x (K u + V-cycle) with the GPU run's CG steps "
+                  "%s: %.0f s per solve" % (n_sample, threads, t_setup, t_k, t_v, list(iters), total))
+        return {"value": 1.0 / total, "unit": "solves/s", "cores": threads, "kind": "port", "sample": sample}, None
     t0 = time.perf_counter()
     mg = oracle.MG(n_sample)
     mg.set_density(rho ** 3)
@@ -89,13 +113,9 @@ def cpu_baseline(n_sample, n_target, tol):
     dt = time.perf_counter() - t0
     del chi, mg
     scale = (n_sample / n_target) ** 3
-    if n_sample == n_target:
-        sample = ("%d^3 measured: FP64 oracle, one full solve (Galerkin set-up + 6 load cases, MG-PCG to %.0e, "
-                  "CG steps %s) in %.1f s on %d threads" % (n_sample, tol, its, dt, threads))
-    else:
-        sample = ("FP64 oracle, full solve at %d^3 (Galerkin set-up + 6 load cases, MG-PCG to %.0e, iters %s) "
-                  "in %.1f s on %d threads; scaled by (%d/%d)^3 to %d^3" %
-                  (n_sample, tol, its, dt, threads, n_sample, n_target, n_target))
+    sample = ("FP64 oracle, full solve at %d^3 (Galerkin set-up + 6 load cases, MG-PCG to %.0e, iters %s) "
+              "in %.1f s on %d threads; scaled by (%d/%d)^3 to %d^3" %
+              (n_sample, tol, its, dt, threads, n_sample, n_target, n_target))
     return {"value": (1.0 / dt) * scale, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": sample}, C
 
@@ -118,8 +138,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-5)
-    ap.add_argument("--cpu-n", type=int, default=96,
-                    help="oracle grid of the CPU baseline's bounded sample (a full solve there, scaled to --n)")
+    ap.add_argument("--cpu-n", type=int, default=0,
+                    help="oracle grid of the CPU baseline (default --n: set-up and one CG step's work timed there; "
+                         "a smaller grid: one full solve, scaled to --n)")
     ap.add_argument("--net", default=NET_WEIGHTS, help="trained warm-start network ('' to skip the learned leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--omega", type=float, default=None, help="smoother relaxation factor (default: the library's)")
@@ -231,7 +252,7 @@ def main():
     if not args.no_cpu:
         cpu_n = args.cpu_n or n
         log("cpu baseline: oracle at %d^3" % cpu_n)
-        out["cpu_baseline"], Cref = cpu_baseline(cpu_n, n, args.tol)
+        out["cpu_baseline"], Cref = cpu_baseline(cpu_n, n, args.tol, stats[-1]["iters"])
     print(json.dumps(out), flush=True)
 
 
