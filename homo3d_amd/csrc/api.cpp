@@ -1005,7 +1005,7 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
         if (warm) {
             launch_rhs_norm64(h->stream, L0.n, h->d_kchi64, h->d_x, h->cfg.emin, h->d_partial, h->d_tmp);
             HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
-            if (h->x_dirty) launch_widen_f64(h->stream, M6, h->X, h->U64);
+            if (h->x_dirty) launch_widen_f64(h->stream, L0.N, h->X, h->U64);
             launch_zero_pin64(h->stream, L0.N, h->U64);
         }
         h->x_dirty = false;
@@ -1025,7 +1025,7 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
                 bool reset = false;
                 for (int r = 0; r < kRhs; r++)
                     if (fn2[r] > 0 && !(rn2[r] < fn2[r])) {
-                        HIPCHK(h, hipMemsetAsync(h->U64 + r * M, 0, sizeof(double) * M, h->stream));
+                        launch_zero_case64(h->stream, L0.N, r, h->U64);
                         reset = true;
                     }
                 if (reset) {
@@ -1062,13 +1062,13 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             }
             if (rc) return rc;
             // U64 = or += the correction x + alpha p (the last CG step's x update, fused)
-            launch_refine_accumulate(h->stream, M, h->d_sc, x_pending ? 1 : 0, h->X, h->P, h->U64, u_zero ? 0 : 1);
+            launch_refine_accumulate(h->stream, L0.N, h->d_sc, x_pending ? 1 : 0, h->X, h->P, h->U64, u_zero ? 0 : 1);
             u_zero = false;
         }
         if (u_zero) HIPCHK(h, hipMemsetAsync(h->U64, 0, sizeof(double) * M6, h->stream));
         for (int r = 0; r < kRhs; r++)
             if (!done[r]) status = std::max(status, (int)HOMO3D_NOT_CONVERGED);
-        launch_round_f32(h->stream, M6, h->U64, h->X);
+        launch_round_f32(h->stream, L0.N, h->U64, h->X);
         HIPCHK(h, hipStreamSynchronize(h->stream));
         prof_flush(h);
         if (status == HOMO3D_DIVERGED) set_err(h, status, "non-finite residual in the refined MG-PCG solve");
@@ -1163,11 +1163,10 @@ int homo3d_residual_norms(homo3d_handle* h, double relres[6]) {
     if (rc) return rc;
     if (h->cfg.solver != HOMO3D_SOLVER_PCG) return set_err(h, HOMO3D_BAD_ARG, "needs the refined solver's FP64 iterate");
     const Level& L0 = h->lv[0];
-    const size_t M6 = (size_t)kRhs * 3 * L0.N;
     double fn2[kRhs], rn2[kRhs];
     launch_rhs_norm64(h->stream, L0.n, h->d_kchi64, h->d_x, h->cfg.emin, h->d_partial, h->d_tmp);
     HIPCHK(h, hipMemcpyAsync(fn2, h->d_tmp, sizeof(fn2), hipMemcpyDeviceToHost, h->stream));
-    if (h->x_dirty) launch_widen_f64(h->stream, M6, h->X, h->U64);
+    if (h->x_dirty) launch_widen_f64(h->stream, L0.N, h->X, h->U64);
     launch_zero_pin64(h->stream, L0.N, h->U64);
     h->x_dirty = false;
     launch_fine_residual64(h->stream, L0.n, h->d_ke64, h->d_kchi64, h->d_x, h->cfg.emin, h->U64, h->Rr, h->d_partial,
@@ -1276,17 +1275,22 @@ int homo3d_solution_copy(homo3d_handle* h, double* host, int to_device) {
     if (h->cfg.solver == HOMO3D_SOLVER_NONE) return set_err(h, HOMO3D_BAD_ARG, "density-only handle");
     const Level& L = h->lv[0];
     const size_t bytes = sizeof(double) * 3 * kRhs * L.N;
-    std::vector<double> tmp(3 * kRhs * L.N);
+    // host [18][n^3] planes <-> the iterate's pair planes in level storage order
+    std::vector<double> tmp(3 * kRhs * L.N), pairs(3 * kRhs * L.N);
     if (to_device) {
         permute_planes(L, host, tmp.data(), 3 * kRhs, true);
-        HIPCHK(h, hipMemcpyAsync(h->U64, tmp.data(), bytes, hipMemcpyHostToDevice, h->stream));
+        for (int c = 0; c < 3 * kRhs; c++)
+            for (size_t v = 0; v < L.N; v++) pairs[u64_index(c, v, L.N)] = tmp[c * L.N + v];
+        HIPCHK(h, hipMemcpyAsync(h->U64, pairs.data(), bytes, hipMemcpyHostToDevice, h->stream));
         launch_zero_pin64(h->stream, L.N, h->U64);
-        launch_round_f32(h->stream, 3 * kRhs * L.N, h->U64, h->X);
+        launch_round_f32(h->stream, L.N, h->U64, h->X);
         h->x_dirty = false;
         HIPCHK(h, hipStreamSynchronize(h->stream));
     } else {
-        HIPCHK(h, hipMemcpyAsync(tmp.data(), h->U64, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(pairs.data(), h->U64, bytes, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int c = 0; c < 3 * kRhs; c++)
+            for (size_t v = 0; v < L.N; v++) tmp[c * L.N + v] = pairs[u64_index(c, v, L.N)];
         permute_planes(L, tmp.data(), host, 3 * kRhs, false);
     }
     return HOMO3D_OK;

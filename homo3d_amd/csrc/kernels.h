@@ -59,12 +59,19 @@ void launch_fine_residual64(hipStream_t s, int n, const double* ke, const double
                             const double* u, float* r32, double* partial, double* out6, int do_k);
 void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc, double emin, double* partial,
                        double* out6);
-void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x);
-// u (+)= x + alpha p over 6 load cases of M floats (M % 4 == 0; alpha = sc + 12, only when pending):
-// the last CG step's x update fused into the FP64 refinement iterate (add) or its first value.
-void launch_refine_accumulate(hipStream_t s, size_t M, const double* sc, int pending, const float* x, const float* p,
+// The FP64 refinement iterate u of N vertices is nine planes of component pairs: component c = 3 r + i
+// (component i of load case r) of vertex v is u[u64_index(c, v, N)].  x, p are FP32 planes [18][N].
+__host__ __device__ inline size_t u64_index(int c, size_t v, size_t N) {
+    return (size_t)(c >> 1) * 2 * N + 2 * v + (size_t)(c & 1);
+}
+void launch_round_f32(hipStream_t s, size_t N, const double* u, float* x);
+// u (+)= x + alpha p for the 6 load cases (N % 2 == 0; alpha = sc + 12, only when pending): the last
+// CG step's x update fused into the FP64 refinement iterate (add) or its first value.
+void launch_refine_accumulate(hipStream_t s, size_t N, const double* sc, int pending, const float* x, const float* p,
                               double* u, int add);
-void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u);
+void launch_widen_f64(hipStream_t s, size_t N, const float* x, double* u);
+// zero load case r (all its components, every vertex) of the FP64 iterate
+void launch_zero_case64(hipStream_t s, size_t N, int r, double* u);
 void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half);
 // FP32 field [6][3][N] <-> packed FP16 field [3][3][N] words (round to nearest)
 void launch_pack_half(hipStream_t s, size_t N, const float* u, void* z);

@@ -108,6 +108,15 @@ __device__ __forceinline__ half2_t bload<half2_t>(Rsrc r, uint32_t voff, uint32_
     return __builtin_bit_cast(half2_t, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
 
+// The FP64 iterate of the refinement loop stores its 18 components as nine planes of 16-byte pairs
+// (u64_index), so a neighbour of all six load cases is nine 16-byte loads rather than eighteen
+// 8-byte ones: the gather's cost is per load instruction.
+typedef double double2_t __attribute__((ext_vector_type(2)));
+template <>
+__device__ __forceinline__ double2_t bload<double2_t>(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double2_t, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+
 __device__ __forceinline__ void bstore(float v, Rsrc r, uint32_t voff, uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
 }
@@ -1281,7 +1290,7 @@ __global__ __launch_bounds__(kFullThreads) __attribute__((amdgpu_waves_per_eu(4)
             for (int i = 0; i < 3; i++) acc[r][i] = 0.0;
         if (DO_K) {
             const Rsrc ru = make_rsrc(u, 18 * N * sizeof(double));
-            const uint32_t plane = (uint32_t)(N * sizeof(double));
+            const uint32_t plane = (uint32_t)(2 * N * sizeof(double));  // one plane of component pairs
             // cell weights and index parts as plain scalars: the loop below runs over a run-time d
             double wd[8];
 #pragma unroll
@@ -1293,12 +1302,14 @@ __global__ __launch_bounds__(kFullThreads) __attribute__((amdgpu_waves_per_eu(4)
                 const int dx = d % 3 - 1, dy = d / 3 % 3 - 1, dz = d / 9 - 1;
                 const uint32_t qo = ((dx < 0 ? X0 : (dx == 0 ? X1 : X2)) + (dy < 0 ? Y0 : (dy == 0 ? Y1 : Y2)) +
                                      (dz < 0 ? Z0 : (dz == 0 ? Z1 : Z2))) *
-                                    (uint32_t)sizeof(double);
+                                    (uint32_t)sizeof(double2_t);
                 double un[R][3];
 #pragma unroll
-                for (int r = 0; r < R; r++)
-#pragma unroll
-                    for (int i = 0; i < 3; i++) un[r][i] = bload<double>(ru, qo, (uint32_t)(r * 3 + i) * plane);
+                for (int q = 0; q < 9; q++) {
+                    const double2_t w = bload<double2_t>(ru, qo, (uint32_t)q * plane);
+                    un[(2 * q) / 3][(2 * q) % 3] = w.x;
+                    un[(2 * q + 1) / 3][(2 * q + 1) % 3] = w.y;
+                }
                 double S[9];
 #pragma unroll
                 for (int k = 0; k < 9; k++) S[k] = 0.0;
@@ -1371,28 +1382,34 @@ __global__ __launch_bounds__(kFullThreads) void k_rhs_norm64(Lay L, size_t N, co
     }
 }
 
-// U64 += d (FP32 correction), all load cases
-// u (+)= x + alpha p per load case (M floats each, M % 4 == 0; alpha = sc[12 + r], taken as 0 unless
-// pending): the last CG step's x update, rounded to FP32 as the CG pass rounds it, added to the FP64
-// iterate (add) or copied into it, in one pass.
-__global__ __launch_bounds__(256) void k_refine_accumulate(size_t M, const double* __restrict__ sc, int pending,
+// u (+)= x + alpha p for all six load cases (x, p FP32 planes [18][N], N % 2 == 0; u in pairs,
+// u64_index; alpha of load case r = sc[12 + r], taken as 0 unless pending): the last CG step's x
+// update, rounded to FP32 as the CG pass rounds it, added to the FP64 iterate (add) or copied into
+// it, in one pass.  Grid row q covers the pair plane of components 2q and 2q + 1.
+__global__ __launch_bounds__(256) void k_refine_accumulate(size_t N, const double* __restrict__ sc, int pending,
                                                            const float* __restrict__ x, const float* __restrict__ p,
                                                            double* __restrict__ u, int add) {
-    const int r = blockIdx.y;
-    const float a = pending ? (float)sc[12 + r] : 0.f;
-    const float4* x4 = reinterpret_cast<const float4*>(x + (size_t)r * M);
-    const float4* p4 = reinterpret_cast<const float4*>(p + (size_t)r * M);
-    double2* u2 = reinterpret_cast<double2*>(u + (size_t)r * M);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
-        float4 xv = x4[i];
-        if (a != 0.f) {
-            const float4 pv = p4[i];
-            xv.x = fmaf(a, pv.x, xv.x);
-            xv.y = fmaf(a, pv.y, xv.y);
-            xv.z = fmaf(a, pv.z, xv.z);
-            xv.w = fmaf(a, pv.w, xv.w);
+    const int q = blockIdx.y, c0 = 2 * q, c1 = 2 * q + 1;
+    const float a0 = pending ? (float)sc[12 + c0 / 3] : 0.f, a1 = pending ? (float)sc[12 + c1 / 3] : 0.f;
+    const float2* x0 = reinterpret_cast<const float2*>(x + (size_t)c0 * N);
+    const float2* x1 = reinterpret_cast<const float2*>(x + (size_t)c1 * N);
+    const float2* p0 = reinterpret_cast<const float2*>(p + (size_t)c0 * N);
+    const float2* p1 = reinterpret_cast<const float2*>(p + (size_t)c1 * N);
+    double2* u2 = reinterpret_cast<double2*>(u + (size_t)q * 2 * N);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N / 2; i += (size_t)gridDim.x * blockDim.x) {
+        float2 xa = x0[i], xb = x1[i];
+        if (a0 != 0.f) {
+            const float2 pa = p0[i];
+            xa.x = fmaf(a0, pa.x, xa.x);
+            xa.y = fmaf(a0, pa.y, xa.y);
         }
-        double2 lo = make_double2((double)xv.x, (double)xv.y), hi = make_double2((double)xv.z, (double)xv.w);
+        if (a1 != 0.f) {
+            const float2 pb = p1[i];
+            xb.x = fmaf(a1, pb.x, xb.x);
+            xb.y = fmaf(a1, pb.y, xb.y);
+        }
+        // vertices 2i and 2i + 1: pairs (c0, c1) of each
+        double2 lo = make_double2((double)xa.x, (double)xb.x), hi = make_double2((double)xa.y, (double)xb.y);
         if (add) {
             const double2 ulo = u2[2 * i], uhi = u2[2 * i + 1];
             lo.x += ulo.x;
@@ -1405,14 +1422,34 @@ __global__ __launch_bounds__(256) void k_refine_accumulate(size_t M, const doubl
     }
 }
 
-__global__ __launch_bounds__(256) void k_round_f32(size_t M, const double* __restrict__ u, float* __restrict__ x) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < M) x[i] = (float)u[i];
+// FP64 iterate (pairs) -> FP32 planes [18][N], and back; one thread per (pair plane, vertex)
+__global__ __launch_bounds__(256) void k_round_f32(size_t N, const double* __restrict__ u, float* __restrict__ x) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * N) return;
+    const size_t q = t / N, v = t - q * N;
+    const double2 w = reinterpret_cast<const double2*>(u)[t];
+    x[2 * q * N + v] = (float)w.x;
+    x[(2 * q + 1) * N + v] = (float)w.y;
 }
 
-__global__ __launch_bounds__(256) void k_widen_f64(size_t M, const float* __restrict__ x, double* __restrict__ u) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < M) u[i] = (double)x[i];
+__global__ __launch_bounds__(256) void k_widen_f64(size_t N, const float* __restrict__ x, double* __restrict__ u) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * N) return;
+    const size_t q = t / N, v = t - q * N;
+    reinterpret_cast<double2*>(u)[t] = make_double2((double)x[2 * q * N + v], (double)x[(2 * q + 1) * N + v]);
+}
+
+// zero the components of load case r (3 r .. 3 r + 2) of the FP64 iterate, every vertex
+__global__ __launch_bounds__(256) void k_zero_case64(size_t N, int r, double* __restrict__ u) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * N) return;
+    const size_t k = t / N, v = t - k * N;
+    u[u64_index(3 * r + (int)k, v, N)] = 0.0;
+}
+
+// zero the pinned origin's components of the FP64 iterate
+__global__ void k_zero_pin64(size_t N, double* __restrict__ u) {
+    if (threadIdx.x < 18) u[u64_index((int)threadIdx.x, 0, N)] = 0.0;
 }
 
 // zero the pinned origin's entries of a 6-load-case field of `planes` word planes
@@ -3014,17 +3051,17 @@ void launch_rhs_norm64(hipStream_t s, int n, const double* kchi, const float* xc
     hipLaunchKernelGGL(k_sum_partials, dim3(6), dim3(kSumThreads), 0, s, (int)nb, partial, out6);
 }
 
-void launch_refine_accumulate(hipStream_t s, size_t M, const double* sc, int pending, const float* x, const float* p,
+void launch_refine_accumulate(hipStream_t s, size_t N, const double* sc, int pending, const float* x, const float* p,
                               double* u, int add) {
-    hipLaunchKernelGGL(k_refine_accumulate, dim3(2048, 6), dim3(256), 0, s, M, sc, pending, x, p, u, add);
+    hipLaunchKernelGGL(k_refine_accumulate, dim3(2048, 9), dim3(256), 0, s, N, sc, pending, x, p, u, add);
 }
 
-void launch_round_f32(hipStream_t s, size_t M, const double* u, float* x) {
-    hipLaunchKernelGGL(k_round_f32, dim3(nblk(M)), dim3(256), 0, s, M, u, x);
+void launch_round_f32(hipStream_t s, size_t N, const double* u, float* x) {
+    hipLaunchKernelGGL(k_round_f32, dim3(nblk(9 * N)), dim3(256), 0, s, N, u, x);
 }
 
-void launch_widen_f64(hipStream_t s, size_t M, const float* x, double* u) {
-    hipLaunchKernelGGL(k_widen_f64, dim3(nblk(M)), dim3(256), 0, s, M, x, u);
+void launch_widen_f64(hipStream_t s, size_t N, const float* x, double* u) {
+    hipLaunchKernelGGL(k_widen_f64, dim3(nblk(9 * N)), dim3(256), 0, s, N, x, u);
 }
 
 void launch_pack_half(hipStream_t s, size_t N, const float* u, void* z) {
@@ -3043,7 +3080,11 @@ void launch_zero_pin(hipStream_t s, size_t N, void* u, int u_half) {
 }
 
 void launch_zero_pin64(hipStream_t s, size_t N, double* u) {
-    hipLaunchKernelGGL((k_zero_pin<double>), dim3(1), dim3(64), 0, s, N, 18, u);
+    hipLaunchKernelGGL(k_zero_pin64, dim3(1), dim3(64), 0, s, N, u);
+}
+
+void launch_zero_case64(hipStream_t s, size_t N, int r, double* u) {
+    hipLaunchKernelGGL(k_zero_case64, dim3(nblk(3 * N)), dim3(256), 0, s, N, r, u);
 }
 
 // Levels up to this size run the coarse-stencil kernels with one load case per thread (six grid
@@ -3457,7 +3498,7 @@ __global__ __launch_bounds__(256) void k_net_target_mean(Lay L, int g, const dou
     for (long v = threadIdx.x; v < g3; v += blockDim.x) {
         const int i = (int)(v % g), j = (int)((v / g) % g), k = (int)(v / ((long)g * g));
         const double r = bf2f(in[v * 8]);
-        s += r * r * r * u[c * N + L.idx(i * f, j * f, k * f)];
+        s += r * r * r * u[u64_index(c, L.idx(i * f, j * f, k * f), N)];
         ws += r * r * r;
     }
     sh[0][threadIdx.x] = s;
@@ -3485,7 +3526,7 @@ __global__ __launch_bounds__(256) void k_net_target(Lay L, int g, const double* 
     const int i = (int)(v % g), j = (int)((v / g) % g), k = (int)(v / ((long)g * g));
     const uint32_t e = L.idx(i * f, j * f, k * f);
     const double sc = scale / (double)L.n;
-    for (int c = 0; c < 18; c++) tg[v * 18 + c] = f2bf((float)(sc * (u[c * N + e] - mean[c])));
+    for (int c = 0; c < 18; c++) tg[v * 18 + c] = f2bf((float)(sc * (u[u64_index(c, e, N)] - mean[c])));
 }
 
 // Warm start from the prediction P ([g^3][ldp], channel 3 s + k = scale u_sk / n): periodic trilinear
@@ -3512,7 +3553,7 @@ __global__ __launch_bounds__(256) void k_net_warm_start(Lay L, int g, const floa
         float s = 0.f;
         for (int c = 0; c < 8; c++) s += wt[c] * P[cv[c] * ldp + ch];
         const double u = ((double)s - (double)P[ch]) * inv;
-        U64[ch * N + t] = u;
+        U64[u64_index(ch, t, N)] = u;
         X[ch * N + t] = (float)u;
     }
 }
