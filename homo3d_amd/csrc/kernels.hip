@@ -160,26 +160,58 @@ __device__ __forceinline__ void hload9(uint32_t (&wv)[9], Rsrc r, uint32_t idx, 
     wv[8] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), (int)(8u * plane), 0);
 }
 
-// 3x3 block of the assembled fine operator between vertex v and its neighbour v+d:
-// S = sum over the cells containing both of w_e * KE[node(v)][node(v+d)].
-// Cell e (0..7) sits at offset (e&1, e>>1&1, e>>2) - 1 from v; v is its node 7-e.
-template <int D, typename T, typename W>
-__device__ __forceinline__ void fine_block(const W wc[8], const T* ke, T S[9]) {
-    constexpr int dx = D % 3 - 1, dy = D / 3 % 3 - 1, dz = D / 9 - 1;
-#pragma unroll
-    for (int k = 0; k < 9; k++) S[k] = T(0);
+// The element matrix staged in LDS: KE (576 entries) followed by 243 block constants (kBlockConsts),
+// padded so that what follows it stays 16-byte aligned.
+constexpr int kBlockConsts = 576;
+constexpr int kKeStaged = 832;
+
+// Sum over the cells that contain both v and v + d of the cell weights w_e, each with the sign
+// prod over the axes m in SM of (e_m ? -1 : +1).  Cell e (0..7) sits at offset (e&1, e>>1&1, e>>2) - 1
+// from v; it holds v + d iff e_m = (d_m + 1) / 2 on every axis with d_m != 0.
+template <int D, int SM, typename T, typename W>
+__device__ __forceinline__ T cell_form(const W wc[8]) {
+    constexpr int dm[3] = {D % 3 - 1, D / 3 % 3 - 1, D / 9 - 1};
+    T s = T(0);
+    bool first = true;
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-        const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
-        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
-        const int a = 7 - e, b = bx + 2 * by + 4 * bz;
-        const T we = (T)wc[e];  // FP32 weights; the FP64 path passes its own array
+        bool in = true;
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) S[i * 3 + j] = fma(we, ke[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
+        for (int m = 0; m < 3; m++)
+            if (dm[m] != 0 && ((e >> m) & 1) != (dm[m] + 1) / 2) in = false;
+        if (!in) continue;
+        const bool neg = (__builtin_popcount(e & SM) & 1) != 0;
+        const T we = (T)wc[e];
+        s = first ? (neg ? -we : we) : (neg ? s - we : s + we);
+        first = false;
     }
+    return s;
+}
+
+// 3x3 block of the assembled fine operator between vertex v and its neighbour v+d:
+// S = sum over the cells e containing both of w_e * KE[node(v)][node(v+d)], v being node 7-e of cell e.
+// For the isotropic cube element, entry (i, j) of KE[a][b] over those cells is one constant times
+// s_i s_j (i != j; s_m = +1 at the upper node of axis m, the sign of every axis along which d is
+// non-zero being the same for all the cells) and one constant (i = j), so
+// S_ij = C_d,ij * (sum of w_e, with the sign s_m(e) for each axis m in {i, j} along which d is zero).
+// C_d,ij (ke[kBlockConsts + 9 d + 3 i + j], set by stage_ke) is the entry of the cell with e_m = 1
+// exactly where d_m = +1.  Four signed sums of two to eight weights and nine products per block,
+// instead of nine FMAs per cell.
+template <int D, typename T, typename W>
+__device__ __forceinline__ void fine_block(const W wc[8], const T* ke, T S[9]) {
+    constexpr int fm = (D % 3 == 1 ? 1 : 0) | (D / 3 % 3 == 1 ? 2 : 0) | (D / 9 == 1 ? 4 : 0);  // axes with d_m = 0
+    const T* cb = ke + kBlockConsts + 9 * D;
+    const T w0 = cell_form<D, 0, T>(wc);
+    const T fxy = cell_form<D, fm & 3, T>(wc), fxz = cell_form<D, fm & 5, T>(wc), fyz = cell_form<D, fm & 6, T>(wc);
+    S[0] = cb[0] * w0;
+    S[4] = cb[4] * w0;
+    S[8] = cb[8] * w0;
+    S[1] = cb[1] * fxy;
+    S[3] = cb[3] * fxy;
+    S[2] = cb[2] * fxz;
+    S[6] = cb[6] * fxz;
+    S[5] = cb[5] * fyz;
+    S[7] = cb[7] * fyz;
 }
 
 template <int BS>
@@ -436,10 +468,25 @@ __device__ __forceinline__ void fine_gather(const FineCtx& c, int n, size_t N, c
     }
 }
 
-// stage the element matrix in LDS (all threads of the block take part)
-template <typename T>
+// Stage the element matrix in LDS (all threads of the block take part), followed, when CONSTS (ke_s
+// then holds kKeStaged entries), by the block constants of fine_block: C_d,ij = KE[7 - e0][b0](i, j)
+// for the cell e0 with e0_m = 1 exactly where d_m = +1 and b0_m = d_m + 1 - e0_m.
+template <typename T, bool CONSTS = true>
 __device__ __forceinline__ void stage_ke(T* ke_s, const T* __restrict__ ke) {
     for (int i = threadIdx.x; i < 576; i += blockDim.x) ke_s[i] = ke[i];
+    __syncthreads();
+    if constexpr (!CONSTS) return;
+    for (int t = threadIdx.x; t < 243; t += blockDim.x) {
+        const int d = t / 9, i = t / 3 % 3, j = t % 3;
+        int e0 = 0, b0 = 0;
+        for (int m = 0; m < 3; m++) {
+            const int dm = (m == 0 ? d % 3 : (m == 1 ? d / 3 % 3 : d / 9)) - 1;
+            const int em = dm > 0 ? 1 : 0;
+            e0 |= em << m;
+            b0 |= (dm + 1 - em) << m;
+        }
+        ke_s[kBlockConsts + t] = ke_s[(3 * (7 - e0) + i) * 24 + 3 * b0 + j];
+    }
     __syncthreads();
 }
 
@@ -492,7 +539,7 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_apply(Lay L, size_t N,
                                                                 float* __restrict__ pp = nullptr,
                                                                 float* __restrict__ xx = nullptr,
                                                                 const double* __restrict__ alpha = nullptr) {
-    __shared__ float ke_s[576];
+    __shared__ float ke_s[kKeStaged];
     stage_ke(ke_s, ke);
     uint32_t v;
     if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
@@ -645,9 +692,9 @@ __global__ __launch_bounds__(kFullThreads, 4) void k_fine_cg_staged(Lay L, size_
                                                                     const double* __restrict__ alpha) {
     static_assert(BETA == (PX == 2), "the first pass (PX 1) has no beta; later passes (PX 2) have both");
     // one LDS array with KE first: its wave-uniform reads then take immediate offsets below 64 KB
-    __shared__ uint32_t lds[576 + 9 * kStPlane];
+    __shared__ uint32_t lds[kKeStaged + 9 * kStPlane];
     float* ke_s = reinterpret_cast<float*>(lds);
-    uint32_t* zs = lds + 576;
+    uint32_t* zs = lds + kKeStaged;
     const int h = L.h;
     const uint32_t t0 = xcd_chunk(blockIdx.x, gridDim.x) * 64u;
     const int bx0 = (int)(t0 % (uint32_t)h), by = (int)((t0 / (uint32_t)h) % (uint32_t)h);
@@ -1127,7 +1174,7 @@ template <int R, bool REVERSE, int ZC, typename A>
 __global__ __launch_bounds__(256, 4) void k_fine_gs(Lay L, size_t N, int colour, float omega,
                                                     const float* __restrict__ ke, const float* __restrict__ w,
                                                     A* __restrict__ u, const float* __restrict__ f) {
-    __shared__ float ke_s[576];
+    __shared__ float ke_s[kKeStaged];
     stage_ke(ke_s, ke);
     const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.h3) return;
@@ -1152,7 +1199,7 @@ __global__ __launch_bounds__(256, 4) void k_fine_gs2(Lay L, size_t N, int cfirst
                                                      double* __restrict__ partial, int pstride,
                                                      float* __restrict__ rown) {
     constexpr int W = R / CasesPerWord<A>::value;
-    __shared__ float ke_s[576];
+    __shared__ float ke_s[kKeStaged];
     __shared__ A xs[3 * W * 256];
     stage_ke(ke_s, ke);
     const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
@@ -1220,7 +1267,7 @@ __global__ __launch_bounds__(kFullThreads) void k_fine_resid_later(Lay L, size_t
                                                                    const float* __restrict__ w,
                                                                    const A* __restrict__ u, float* __restrict__ r,
                                                                    float zero) {
-    __shared__ float ke_s[576];
+    __shared__ float ke_s[kKeStaged];
     stage_ke(ke_s, ke);
     uint32_t v;
     if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
@@ -1272,7 +1319,7 @@ __global__ __launch_bounds__(kFullThreads) __attribute__((amdgpu_waves_per_eu(4)
     constexpr int R = 6;
     __shared__ double sh[kFullThreads / 64];
     __shared__ double ke_s[576];
-    stage_ke(ke_s, ke);
+    stage_ke<double, false>(ke_s, ke);  // the neighbour loop below reads KE itself
     uint32_t v;
     const bool live = full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v);
     double nrm[R];
@@ -2857,12 +2904,15 @@ void launch_fine_apply(hipStream_t s, int n, const float* ke, const FineMfma* mf
                            nullptr, nullptr, nullptr, nullptr);
 }
 
-constexpr int kCgPrefetch = 2;  // neighbours whose loads are in flight ahead of the CG pass's block products
+// neighbours whose loads are in flight ahead of the CG pass's block products (FP32 z: one, as two
+// spill there)
+template <typename A>
+constexpr int kCgPrefetch = CasesPerWord<A>::value == 2 ? 2 : 1;
 template <typename A, bool FIRST>
 static void fine_apply_cg_row(hipStream_t s, const Lay& L, size_t N, const float* ke, const float* w, const A* z,
                               float* p, float* x, const double* sc, float* q, double* partial) {
     constexpr int R = 6;
-    hipLaunchKernelGGL((k_fine_apply<R, 0, true, A, !FIRST, FIRST ? 1 : 2, kCgPrefetch>), dim3(full_grid(L)),
+    hipLaunchKernelGGL((k_fine_apply<R, 0, true, A, !FIRST, FIRST ? 1 : 2, kCgPrefetch<A>>), dim3(full_grid(L)),
                        dim3(kFullThreads), 0, s, L, N, ke, w, z, nullptr, q, partial, nullptr, sc + 4 * R, p, x,
                        sc + 2 * R);
 }
