@@ -66,6 +66,15 @@ def kernel_bytes(name, n, R=6, s_z=2, s_f=4, s_w=4):
     return None
 
 
+def gs_pair_must_read(n, R=6, s_z=2, s_f=4, s_w=4):
+    """HBM bytes a colour-pair GS launch must move at least: every vertex neighbours the pair's colours, so
+    the whole z field and every cell weight is read; f of the pair's vertices is read and their z words
+    written (c/8 of the vertices)."""
+    N = n ** 3
+    c = gs_colours_per_launch(n) / 8.0
+    return N * (3 * R * s_z + s_w) + N * c * (3 * R * s_f + 3 * R * s_z)
+
+
 def measured_traffic(name, n):
     """HBM bytes per launch of `name` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
     if n != 256 or not os.path.exists(TRAFFIC_JSON) or name not in PROF_KERNEL:
@@ -220,6 +229,17 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dom, n), "kernel": dom,
                 "avg_launch_ms": round(ms / cnt, 5),
                 "algorithmic_bytes_per_launch": kernel_bytes(dom, n, s_z=s_z)}
+    # the reverse colour-pair GS launch (VALU-bound): its must-read bytes against the counter traffic
+    gs_c, gs_t = prof["fine_gs"]
+    gs_pair = None
+    if gs_c and gs_colours_per_launch(n) == 2:
+        gb = gs_pair_must_read(n, s_z=s_z)
+        gs_ms = gs_t / gs_c
+        gtr = measured_traffic("fine_gs", n)
+        gs_pair = {"kernel": "fine_gs", "avg_launch_ms": round(gs_ms, 5), "must_read_bytes_per_launch": gb,
+                   "achieved": round(gb / (gs_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(gb / (gs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5), "traffic": gtr,
+                   "traffic_over_must_read": round(gtr / gb, 3) if gtr else None}
     out = {
         "metric": METRIC,
         "value": round(args.steps / dt, 5),
@@ -240,6 +260,7 @@ def main():
                    "iters": stats[-1]["iters"], "relres": [float("%.3e" % r) for r in stats[-1]["relres"]],
                    "worst_relres_all_steps": float("%.3e" % worst)},
         "roofline": roofline,
+        "gs_pair": gs_pair,
     }
     if n == 256 and os.path.exists(CH_FIXTURE):
         # the FP64 oracle's C^H of this design (tests/golden/make_ch256.py)

@@ -1258,28 +1258,35 @@ __device__ __forceinline__ void later_vertex(const Lay& L, size_t N, uint32_t v,
     }
 }
 
-// Colours 0..6 in one launch, on the 512-thread tiles of full_vertex: wave b of a workgroup holds
-// parity block b, i.e. colour 7 - b, and takes that colour's later neighbours (a wave-uniform branch;
-// the colour-7 wave has none).  The tile's eight blocks share their neighbour rows in L1/L2, so the
-// gathered u is fetched from HBM about once.
+// Colours 0..6 in one launch, on the 128 x 2 x 2 vertex tiles of full_vertex (64 positions t of the
+// eight parity blocks; colour c is block 7 - c) with four waves per tile: wave w takes colours w and
+// 7 - w at the tile's positions in turn (a wave-uniform branch).  Colour c has 26, 24, 20, 18, 8, 6, 2,
+// 0 later neighbours for c = 0..7, so each wave gathers 26 (one wave per colour left the colour-0
+// wave to finish alone).  The tile's blocks share their neighbour rows in L1/L2, so the gathered u is
+// fetched from HBM about once.  Parity-blocked levels only (make_lay(n, 0)).
 template <int R, typename A>
-__global__ __launch_bounds__(kFullThreads) void k_fine_resid_later(Lay L, size_t N, const float* __restrict__ ke,
-                                                                   const float* __restrict__ w,
-                                                                   const A* __restrict__ u, float* __restrict__ r,
-                                                                   float zero) {
+__global__ __launch_bounds__(256) void k_fine_resid_later(Lay L, size_t N, const float* __restrict__ ke,
+                                                          const float* __restrict__ w, const A* __restrict__ u,
+                                                          float* __restrict__ r, float zero) {
     __shared__ float ke_s[kKeStaged];
     stage_ke(ke_s, ke);
-    uint32_t v;
-    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * 64u + (threadIdx.x & 63u);
+    if (t >= L.h3) return;
+    const uint32_t h3 = L.h3;
     switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
-        case 7: later_vertex<R, 0, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 6: later_vertex<R, 1, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 5: later_vertex<R, 2, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 4: later_vertex<R, 3, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 3: later_vertex<R, 4, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 2: later_vertex<R, 5, A>(L, N, v, ke_s, w, u, r, zero); break;
-        case 1: later_vertex<R, 6, A>(L, N, v, ke_s, w, u, r, zero); break;
-        default: break;
+        case 0: later_vertex<R, 0, A>(L, N, 7 * h3 + t, ke_s, w, u, r, zero); break;  // colour 7: none
+        case 1:
+            later_vertex<R, 1, A>(L, N, 6 * h3 + t, ke_s, w, u, r, zero);
+            later_vertex<R, 6, A>(L, N, 1 * h3 + t, ke_s, w, u, r, zero);
+            break;
+        case 2:
+            later_vertex<R, 2, A>(L, N, 5 * h3 + t, ke_s, w, u, r, zero);
+            later_vertex<R, 5, A>(L, N, 2 * h3 + t, ke_s, w, u, r, zero);
+            break;
+        default:
+            later_vertex<R, 3, A>(L, N, 4 * h3 + t, ke_s, w, u, r, zero);
+            later_vertex<R, 4, A>(L, N, 3 * h3 + t, ke_s, w, u, r, zero);
+            break;
     }
 }
 
@@ -3063,7 +3070,7 @@ void launch_fine_residual_later(hipStream_t s, int n, const float* ke, const flo
                                 float* r) {
     const size_t N = (size_t)n * n * n;
     const Lay L = make_lay(n, 0);
-    const dim3 g(full_grid(L)), b(kFullThreads);
+    const dim3 g(nblk(L.h3, 64)), b(256);
     if (u_half)
         hipLaunchKernelGGL((k_fine_resid_later<6, half2_t>), g, b, 0, s, L, N, ke, w, static_cast<const half2_t*>(u),
                            r, 0.f);
