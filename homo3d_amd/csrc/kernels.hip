@@ -1748,9 +1748,9 @@ __global__ __launch_bounds__(256) void k_stencil_gs2(Lay L, size_t N, int cfirst
         L, N, colour_block(c2) * L.h3 + t, omega, st, u, f, r0, xl, nullptr, rown);
 }
 
-// The coarse-level counterpart of k_fine_resid_later: r = r_own - the later-colour terms, on the
-// 512-thread parity tiles (wave b takes colour 7 - b; colour 7 is final).  Load cases r0 .. r0 + R - 1
-// with r0 = blockIdx.y R.  zero as in k_fine_resid_later.
+// The coarse-level counterpart of k_fine_resid_later: r = r_own - the later-colour terms, on the same
+// parity tiles with four waves (wave w takes colours w and 7 - w; colour 7 is final).  Load cases
+// r0 .. r0 + R - 1 with r0 = blockIdx.y R.  zero as in k_fine_resid_later.
 template <int R, int C, typename SA, typename A>
 __device__ __forceinline__ void stencil_later_vertex(const Lay& L, size_t N, uint32_t v, const SA* __restrict__ st,
                                                      const A* __restrict__ u, float* __restrict__ r, int r0,
@@ -1773,21 +1773,27 @@ __device__ __forceinline__ void stencil_later_vertex(const Lay& L, size_t N, uin
 }
 
 template <int R, typename SA, typename A>
-__global__ __launch_bounds__(kFullThreads) void k_stencil_resid_later(Lay L, size_t N, const SA* __restrict__ st,
-                                                                      const A* __restrict__ u, float* __restrict__ r,
-                                                                      float zero) {
-    uint32_t v;
-    if (!full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v)) return;
+__global__ __launch_bounds__(256) void k_stencil_resid_later(Lay L, size_t N, const SA* __restrict__ st,
+                                                             const A* __restrict__ u, float* __restrict__ r,
+                                                             float zero) {
+    const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * 64u + (threadIdx.x & 63u);
+    if (t >= L.h3) return;
+    const uint32_t h3 = L.h3;
     const int r0 = blockIdx.y * R;
     switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
-        case 7: stencil_later_vertex<R, 0, SA, A>(L, N, v, st, u, r, r0, zero); break;
-        case 6: stencil_later_vertex<R, 1, SA, A>(L, N, v, st, u, r, r0, zero); break;
-        case 5: stencil_later_vertex<R, 2, SA, A>(L, N, v, st, u, r, r0, zero); break;
-        case 4: stencil_later_vertex<R, 3, SA, A>(L, N, v, st, u, r, r0, zero); break;
-        case 3: stencil_later_vertex<R, 4, SA, A>(L, N, v, st, u, r, r0, zero); break;
-        case 2: stencil_later_vertex<R, 5, SA, A>(L, N, v, st, u, r, r0, zero); break;
-        case 1: stencil_later_vertex<R, 6, SA, A>(L, N, v, st, u, r, r0, zero); break;
-        default: break;
+        case 0: stencil_later_vertex<R, 0, SA, A>(L, N, 7 * h3 + t, st, u, r, r0, zero); break;
+        case 1:
+            stencil_later_vertex<R, 1, SA, A>(L, N, 6 * h3 + t, st, u, r, r0, zero);
+            stencil_later_vertex<R, 6, SA, A>(L, N, 1 * h3 + t, st, u, r, r0, zero);
+            break;
+        case 2:
+            stencil_later_vertex<R, 2, SA, A>(L, N, 5 * h3 + t, st, u, r, r0, zero);
+            stencil_later_vertex<R, 5, SA, A>(L, N, 2 * h3 + t, st, u, r, r0, zero);
+            break;
+        default:
+            stencil_later_vertex<R, 3, SA, A>(L, N, 4 * h3 + t, st, u, r, r0, zero);
+            stencil_later_vertex<R, 4, SA, A>(L, N, 3 * h3 + t, st, u, r, r0, zero);
+            break;
     }
 }
 
@@ -3287,8 +3293,8 @@ void launch_stencil_gs2(hipStream_t s, int n, int cfirst, float omega, int rever
 
 template <int R, typename SA, typename A>
 static void stencil_resid_later(hipStream_t s, const Lay& L, size_t N, const SA* st, const A* u, float* r) {
-    hipLaunchKernelGGL((k_stencil_resid_later<R, SA, A>), dim3(full_grid(L), 6 / R), dim3(kFullThreads), 0, s, L, N,
-                       st, u, r, 0.f);
+    hipLaunchKernelGGL((k_stencil_resid_later<R, SA, A>), dim3(nblk(L.h3, 64), 6 / R), dim3(256), 0, s, L, N, st, u,
+                       r, 0.f);
 }
 
 template <typename SA>
