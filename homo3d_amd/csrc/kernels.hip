@@ -2311,14 +2311,17 @@ __global__ __launch_bounds__(256) void k_pcg_update_r(int R, size_t M, const dou
     const float4* q4 = reinterpret_cast<const float4*>(q + (size_t)k * M);
     float4* r4 = reinterpret_cast<float4*>(r + (size_t)k * M);
     double s = 0.0;
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f* qn = reinterpret_cast<const v4f*>(q4);
+    v4f* rn = reinterpret_cast<v4f*>(r4);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < M / 4; i += (size_t)gridDim.x * blockDim.x) {
-        const float4 qv = q4[i];
-        float4 rv = r4[i];
+        const v4f qv = __builtin_nontemporal_load(qn + i);
+        v4f rv = __builtin_nontemporal_load(rn + i);
         rv.x = fmaf(-a, qv.x, rv.x);
         rv.y = fmaf(-a, qv.y, rv.y);
         rv.z = fmaf(-a, qv.z, rv.z);
         rv.w = fmaf(-a, qv.w, rv.w);
-        r4[i] = rv;
+        __builtin_nontemporal_store(rv, rn + i);
         s += ((double)rv.x * rv.x + (double)rv.y * rv.y) + ((double)rv.z * rv.z + (double)rv.w * rv.w);
     }
     s = block_sum<256>(s, sh);
