@@ -78,12 +78,16 @@ struct homo3d_handle {
     float* d_x = nullptr;       // raw cell values x (FP32 input), for FP64 weights
     float* d_stage = nullptr;   // pitched staging buffer for host densities / sensitivities
     size_t stage_elems = 0;
-    double* d_Ainv = nullptr;
+    double* d_Ainv = nullptr;       // the coarsest level's dense FP64 inverse (mc x mc)
+    int* d_cstat = nullptr;         // factorisation status on the device
     int mc = 0;
-    // coarsest factorisation, deferred from the density update to the first coarse solve after it
-    float* h_cst = nullptr;          // pinned copy of the coarsest stencil
-    double* h_ainv = nullptr;        // pinned staging of the inverse
-    hipEvent_t ev_coarse = nullptr;  // the stencil copy has landed
+    // the coarsest factorisation runs on the GPU at the density update, on a stream of its own behind
+    // the Galerkin kernels (ev_gal), so it overlaps the solve's finest-level work; its status is read,
+    // and the main stream made to wait for it, at the first coarse solve after it
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_gal = nullptr;
+    int* h_cstat = nullptr;          // pinned landing place of the status
+    hipEvent_t ev_coarse = nullptr;  // ... and the event that says it has landed
     bool coarse_pending = false;
     double* h_rr = nullptr;          // pinned landing place of the CG residual norms
     hipEvent_t ev_rr = nullptr;      // ... and the event that says they have landed
@@ -257,41 +261,6 @@ void permute_planes(const Level& L, const T* in, T* out, int planes, bool to_sto
             }
 }
 
-// Host FP64 Cholesky (lower) and explicit inverse of the pinned coarsest system.
-bool cholesky_inverse(int m, std::vector<double>& A) {
-    for (int j = 0; j < m; j++) {
-        double s = A[(size_t)j * m + j];
-        for (int k = 0; k < j; k++) s -= A[(size_t)j * m + k] * A[(size_t)j * m + k];
-        if (!(s > 0)) return false;
-        const double l = std::sqrt(s);
-        A[(size_t)j * m + j] = l;
-        for (int i = j + 1; i < m; i++) {
-            double t = A[(size_t)i * m + j];
-            for (int k = 0; k < j; k++) t -= A[(size_t)i * m + k] * A[(size_t)j * m + k];
-            A[(size_t)i * m + j] = t / l;
-        }
-    }
-    // Linv (lower) in place of L
-    std::vector<double> Li((size_t)m * m, 0.0);
-    for (int j = 0; j < m; j++) {
-        Li[(size_t)j * m + j] = 1.0 / A[(size_t)j * m + j];
-        for (int i = j + 1; i < m; i++) {
-            double s = 0;
-            for (int k = j; k < i; k++) s -= A[(size_t)i * m + k] * Li[(size_t)k * m + j];
-            Li[(size_t)i * m + j] = s / A[(size_t)i * m + i];
-        }
-    }
-    // A^{-1} = Linv^T Linv
-    for (int i = 0; i < m; i++)
-        for (int j = 0; j <= i; j++) {
-            double s = 0;
-            for (int k = i; k < m; k++) s += Li[(size_t)k * m + i] * Li[(size_t)k * m + j];
-            A[(size_t)i * m + j] = s;
-            A[(size_t)j * m + i] = s;
-        }
-    return true;
-}
-
 // Symmetry groups (TensorExpression.cuh:72-262) in the canonical order shared with the checker:
 // element = (perm, flips), out[k] = flip_k ? n-1-p[perm[k]] : p[perm[k]].
 const int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -437,37 +406,21 @@ void op_residual(homo3d_handle* h, int l, const void* u, const float* f, float* 
     }
 }
 
-// FP64 Cholesky inverse of the coarsest Galerkin operator (assembled densely from its stencil), once
-// per density update, uploaded ahead of the coarse solve that needs it.  A system that is not positive
-// definite leaves the handle without a density.
+// The coarsest level's FP64 inverse is formed on the GPU at the density update (launch_coarse_factor,
+// on cstream); the first coarse solve after it reads the factorisation's status and makes the main
+// stream wait for it (by then the solve's finest-level work, queued behind the Galerkin kernels, has
+// kept the GPU busy meanwhile).  A system that is not positive definite
+// leaves the handle without a density.
 int coarse_factor(homo3d_handle* h) {
     if (!h->coarse_pending) return HOMO3D_OK;
     h->coarse_pending = false;
     HIPCHK(h, hipEventSynchronize(h->ev_coarse));
-    const Level& Lc = h->lv[h->nl - 1];
-    const float* st = h->h_cst;
-    const int m = h->mc, nc = Lc.n;
-    std::vector<double> A((size_t)m * m, 0.0);
-    for (size_t v = 0; v < Lc.N; v++) {
-        const int x0 = (int)(v % nc), y0 = (int)((v / nc) % nc), z0 = (int)(v / ((size_t)nc * nc));
-        for (int d = 0; d < 27; d++) {
-            const int xq = (x0 + d % 3 - 1 + nc) % nc, yq = (y0 + d / 3 % 3 - 1 + nc) % nc, zq = (z0 + d / 9 - 1 + nc) % nc;
-            const size_t q = xq + (size_t)nc * (yq + (size_t)nc * zq);
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) A[(3 * v + i) * m + 3 * q + j] += st[(size_t)(d * 9 + i * 3 + j) * Lc.N + v];
-        }
-    }
-    for (int i = 0; i < m; i++)
-        for (int j = i + 1; j < m; j++) {
-            const double s = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
-            A[(size_t)i * m + j] = A[(size_t)j * m + i] = s;
-        }
-    if (!cholesky_inverse(m, A)) {
+    HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_coarse, 0));
+    if (*h->h_cstat != 0) {
         h->density_set = false;
-        return set_err(h, HOMO3D_DIVERGED, "coarsest system is not positive definite");
+        return set_err(h, HOMO3D_DIVERGED, "coarsest system is not positive definite (pivot %d of %d)", *h->h_cstat - 1,
+                       h->mc);
     }
-    std::copy(A.begin(), A.end(), h->h_ainv);
-    HIPCHK(h, hipMemcpyAsync(h->d_Ainv, h->h_ainv, sizeof(double) * A.size(), hipMemcpyHostToDevice, h->stream));
     return HOMO3D_OK;
 }
 
@@ -867,7 +820,8 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
         return fail(HOMO3D_HIP_ERROR);
     const int nc = sizes[nl - 1];
     h->mc = 3 * nc * nc * nc;
-    if (hipMalloc(&h->d_Ainv, sizeof(double) * h->mc * h->mc)) return fail(HOMO3D_HIP_ERROR);
+    if (hipMalloc(&h->d_Ainv, sizeof(double) * h->mc * h->mc) || hipMalloc(&h->d_cstat, sizeof(int)))
+        return fail(HOMO3D_HIP_ERROR);
     if (hipDeviceSynchronize() != hipSuccess) return fail(HOMO3D_HIP_ERROR);
     *out = h;
     return HOMO3D_OK;
@@ -877,6 +831,7 @@ int homo3d_destroy(homo3d_handle* h) {
     if (!h) return HOMO3D_OK;
     // teardown: errors are not actionable here, so they are ignored
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->cstream) (void)hipStreamSynchronize(h->cstream);
     for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
     for (int l = 0; l < kMaxLevels; l++) {
         (void)hipFree(h->lv[l].u);
@@ -884,12 +839,13 @@ int homo3d_destroy(homo3d_handle* h) {
         (void)hipFree(h->lv[l].r);
         (void)hipFree(h->lv[l].st);
     }
-    void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_ke16, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->X, h->F, h->P, h->Q, h->Z,
+    void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_ke16, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->d_cstat, h->X, h->F, h->P, h->Q, h->Z,
                     h->Rr, h->d_partial, h->d_sc, h->d_tmp, h->d_ftmp, h->d_gperm, h->d_gflip};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    if (h->h_cst) (void)hipHostFree(h->h_cst);
-    if (h->h_ainv) (void)hipHostFree(h->h_ainv);
+    if (h->cstream) (void)hipStreamDestroy(h->cstream);
+    if (h->ev_gal) (void)hipEventDestroy(h->ev_gal);
+    if (h->h_cstat) (void)hipHostFree(h->h_cstat);
     if (h->ev_coarse) (void)hipEventDestroy(h->ev_coarse);
     if (h->h_rr) (void)hipHostFree(h->h_rr);
     if (h->ev_rr) (void)hipEventDestroy(h->ev_rr);
@@ -920,21 +876,29 @@ int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_de
     int rc = stage_in(h, x, pitch, on_device, &dx);
     if (rc) return rc;
     launch_set_weights(h->stream, n, pitch, h->cfg.emin, dx, h->d_w, h->d_x);
+    // a factorisation still running from the previous update reads the coarsest stencil rewritten below
+    if (h->ev_coarse) HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_coarse, 0));
     const float clamp = (float)h->cfg.diag_clamp;
     launch_galerkin_fine(h->stream, n, h->lv[1].lex, h->d_G, h->d_ke, h->d_w, h->lv[1].st, h->lv[1].st_half, clamp);
     for (int l = 2; l < h->nl; l++)
         launch_galerkin_coarse(h->stream, h->lv[l - 1].n, h->lv[l].lex, h->lv[l - 1].st, h->lv[l - 1].st_half,
                                h->lv[l].st, h->lv[l].st_half, clamp);
-    // coarsest dense system: its stencil goes to pinned host memory now; the factorisation runs at the
-    // first coarse solve (coarse_factor), when the host has already queued the finest-level work
+    // coarsest dense system: assembled and inverted on the GPU; its status is read at the
+    // first coarse solve (coarse_factor)
     const Level& Lc = h->lv[h->nl - 1];
-    if (!h->h_cst) {
-        HIPCHK(h, hipHostMalloc((void**)&h->h_cst, sizeof(float) * 243 * Lc.N, hipHostMallocDefault));
-        HIPCHK(h, hipHostMalloc((void**)&h->h_ainv, sizeof(double) * h->mc * h->mc, hipHostMallocDefault));
+    if (!h->h_cstat) {
+        HIPCHK(h, hipHostMalloc((void**)&h->h_cstat, sizeof(int), hipHostMallocDefault));
         HIPCHK(h, hipEventCreateWithFlags(&h->ev_coarse, hipEventDisableTiming));
+        HIPCHK(h, hipEventCreateWithFlags(&h->ev_gal, hipEventDisableTiming));
+        HIPCHK(h, hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
     }
-    HIPCHK(h, hipMemcpyAsync(h->h_cst, Lc.st, sizeof(float) * 243 * Lc.N, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipEventRecord(h->ev_coarse, h->stream));
+    // behind everything queued so far on the main stream (the Galerkin kernels, and earlier coarse
+    // solves that read the inverse)
+    HIPCHK(h, hipEventRecord(h->ev_gal, h->stream));
+    HIPCHK(h, hipStreamWaitEvent(h->cstream, h->ev_gal, 0));
+    launch_coarse_factor(h->cstream, Lc.n, static_cast<const float*>(Lc.st), h->d_Ainv, h->d_cstat);
+    HIPCHK(h, hipMemcpyAsync(h->h_cstat, h->d_cstat, sizeof(int), hipMemcpyDeviceToHost, h->cstream));
+    HIPCHK(h, hipEventRecord(h->ev_coarse, h->cstream));
     h->coarse_pending = true;
     h->density_set = true;
     return HOMO3D_OK;

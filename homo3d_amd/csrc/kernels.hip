@@ -2235,6 +2235,93 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int m, size_t Nc, const do
 }
 
 // ---------------------------------------------------------------------------------------------
+// Coarsest level: its dense FP64 operator factorised and inverted on the GPU (the reference does it
+// on the host, grid.cpp:349-424).  m = 3 n_c^3 <= 1029 (n_c = 4..7); A and T are m x m, row-major.
+// ---------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_fill64(size_t n, double v, double* __restrict__ a) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+// A += the coarsest stencil (lexicographic level; entry k = 9d + 3i + j of vertex v couples (v, i) with
+// (v + d, j)), one thread per row, so neighbours that coincide on the periodic grid add up.
+__global__ __launch_bounds__(256) void k_coarse_assemble(int nc, const float* __restrict__ st, double* __restrict__ A) {
+    const int Nc = nc * nc * nc, m = 3 * Nc;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m) return;
+    const int v = row / 3, i = row % 3;
+    const int x0 = v % nc, y0 = (v / nc) % nc, z0 = v / (nc * nc);
+    double* a = A + (size_t)row * m;
+    for (int d = 0; d < 27; d++) {
+        const int xq = (x0 + d % 3 - 1 + nc) % nc, yq = (y0 + d / 3 % 3 - 1 + nc) % nc, zq = (z0 + d / 9 - 1 + nc) % nc;
+        const int q = xq + nc * (yq + nc * zq);
+        for (int j = 0; j < 3; j++) a[3 * q + j] += (double)st[(size_t)(d * 9 + i * 3 + j) * Nc + v];
+    }
+}
+
+// A = (A + A^T) / 2, one thread per pair i < j
+__global__ __launch_bounds__(256) void k_symmetrize64(int m, double* __restrict__ A) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)m * m) return;
+    const int i = (int)(t / m), j = (int)(t % m);
+    if (j <= i) return;
+    const double s = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
+    A[(size_t)i * m + j] = s;
+    A[(size_t)j * m + i] = s;
+}
+
+// In-place inverse of the symmetric positive definite A by Gauss-Jordan elimination on the diagonal,
+// one workgroup: for each k, with p = A[k][k] (the k-th Schur complement's leading entry, positive
+// exactly when A is positive definite, the square of the Cholesky pivot),
+//   A[k][j] /= p (j != k), A[k][k] = 1 / p, A[i][k] = -A[i][k] / p (i != k),
+//   A[i][j] -= A[i][k] A[k][j] / p (i, j != k).
+// The pivot row and column go to LDS; thread (g, c) then updates the elements (i, j) with
+// i = g mod G and j = c mod C (C = min(m, threads) consecutive columns, G = threads / C row groups),
+// all independent.  status = 0, or 1 + the first k whose pivot is not positive; every thread reads the
+// pivot after the barrier that ends step k - 1, so all take the same exit.
+__global__ __launch_bounds__(1024) void k_spd_inverse64(int m, double* __restrict__ A, int* __restrict__ status) {
+    extern __shared__ double sh[];
+    double* rowk = sh;      // new row k
+    double* colk = sh + m;  // old column k
+    const int C = m < (int)blockDim.x ? m : (int)blockDim.x, G = (int)blockDim.x / C;
+    const int g = (int)threadIdx.x / C, c = (int)threadIdx.x - g * C;
+    for (int k = 0; k < m; k++) {
+        const double piv = A[(size_t)k * m + k];
+        if (!(piv > 0.0)) {
+            if (threadIdx.x == 0) *status = 1 + k;
+            return;
+        }
+        const double inv = 1.0 / piv;
+        for (int j = threadIdx.x; j < m; j += blockDim.x) {
+            rowk[j] = (j == k ? 1.0 : A[(size_t)k * m + j]) * inv;
+            colk[j] = A[(size_t)j * m + k];
+        }
+        __syncthreads();
+        if (g < G)
+            for (int j = c; j < m; j += C) {
+                const double rj = rowk[j];
+                // eight rows per batch: their loads are issued together (the L2 latency, not the
+                // arithmetic, bounds a step)
+                for (int i0 = g; i0 < m; i0 += 8 * G) {
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int i = i0 + u * G;
+                        v[u] = i < m ? A[(size_t)i * m + j] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int i = i0 + u * G;
+                        if (i < m) A[(size_t)i * m + j] = i == k ? rj : (j == k ? -colk[i] * inv : fma(-colk[i], rj, v[u]));
+                    }
+                }
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *status = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
 // FP64 reductions and vector updates (per load case)
 // ---------------------------------------------------------------------------------------------
 // partial[r * gridDim.x + b] = block b's grid-stride share of a_r . b_r, r = blockIdx.y (3N values
@@ -3400,6 +3487,15 @@ void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float*
     const size_t Nc = (size_t)nc * nc * nc;
     const int m = (int)(3 * Nc);
     hipLaunchKernelGGL(k_coarse_solve, dim3(6), dim3(256), sizeof(double) * m, s, m, Nc, Ainv, f, u);
+}
+
+void launch_coarse_factor(hipStream_t s, int nc, const float* st, double* ainv, int* status) {
+    const int m = 3 * nc * nc * nc;
+    const size_t mm = (size_t)m * m;
+    hipLaunchKernelGGL(k_fill64, dim3(nblk(mm) < 1024 ? nblk(mm) : 1024), dim3(256), 0, s, mm, 0.0, ainv);
+    hipLaunchKernelGGL(k_coarse_assemble, dim3(nblk((size_t)m)), dim3(256), 0, s, nc, st, ainv);
+    hipLaunchKernelGGL(k_symmetrize64, dim3(nblk(mm)), dim3(256), 0, s, m, ainv);
+    hipLaunchKernelGGL(k_spd_inverse64, dim3(1), dim3(1024), 2 * sizeof(double) * m, s, m, ainv, status);
 }
 
 void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks,

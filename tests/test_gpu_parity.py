@@ -291,6 +291,29 @@ def test_transfers_and_coarse_solve(case16, orc):
         assert rel(uc[s], ref) < 1e-4
 
 
+@pytest.mark.parametrize("n", [20, 24, 28])
+def test_coarse_solve_every_coarsest_size(h3d, orc, n):
+    """The coarsest level's inverse is assembled, Cholesky-factorised and inverted on the GPU at the
+    density update.  Coarsest grids are 4^3 to 7^3 (m = 192 .. 1029 unknowns); 4^3 is covered above.
+    Against a dense FP64 solve of the oracle's Galerkin operator of the same design."""
+    say("coarse solve: n=%d" % n)
+    hom, x = make(h3d, n, seed=3)
+    mg = orc.MG(n, E, NU)
+    mg.set_density(x.astype(np.float64))
+    lc = len(hom.sizes) - 1
+    nc = hom.sizes[lc]
+    assert nc == n // 4
+    rng = np.random.default_rng(n)
+    fcs = rng.standard_normal(hom.level_shape(lc)).astype(np.float32)
+    hom.set_field(lc, 1, fcs)
+    hom.coarse_solve()
+    uc = hom.get_field(lc, 0)
+    A = orc.dense_from_stencil(nc, mg.stencil(lc))
+    for s in range(6):
+        ref = np.linalg.solve(A, fcs[s].astype(np.float64).T.reshape(-1)).reshape(-1, 3).T
+        assert rel(uc[s], ref) < 1e-4, (n, s)
+
+
 @pytest.mark.parametrize("symmetric", [False, True])
 def test_vcycle(h3d, orc, symmetric):
     say("vcycle: n=32 symmetric=%s" % symmetric)
