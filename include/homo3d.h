@@ -113,9 +113,10 @@ int homo3d_levels(const homo3d_handle* h, int* sizes, int max_levels);
 /* Opaque hipStream_t the handle launches on (for event timing by the caller). */
 void* homo3d_stream(homo3d_handle* h);
 
-/* Weights and Galerkin operators of a new density.  The coarsest level's FP64 factorisation runs on the
- * host at the first coarse solve after the update (overlapping the GPU's finest-level work), so a
- * coarsest system that is not positive definite is reported (HOMO3D_DIVERGED) by that solve. */
+/* Weights and Galerkin operators of a new density.  The coarsest level's FP64 inverse is formed on the
+ * GPU (on a second stream of the handle, overlapping the finest-level work of the next solve); a
+ * coarsest system that is not positive definite is reported (HOMO3D_DIVERGED) by the first coarse
+ * solve after the update. */
 int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_device);
 int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* stats);
 int homo3d_elastic_matrix(homo3d_handle* h, double C[36]);
