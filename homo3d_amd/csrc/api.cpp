@@ -80,6 +80,7 @@ struct homo3d_handle {
     size_t stage_elems = 0;
     double* d_Ainv = nullptr;       // the coarsest level's dense FP64 inverse (mc x mc)
     int* d_cstat = nullptr;         // factorisation status on the device
+    double* d_cpiv = nullptr;       // its pivot rows and columns (4 mc)
     int mc = 0;
     // the coarsest factorisation runs on the GPU at the density update, on a stream of its own behind
     // the Galerkin kernels (ev_gal), so it overlaps the solve's finest-level work; its status is read,
@@ -820,7 +821,8 @@ int homo3d_create(const homo3d_config* cfg, homo3d_handle** out) {
         return fail(HOMO3D_HIP_ERROR);
     const int nc = sizes[nl - 1];
     h->mc = 3 * nc * nc * nc;
-    if (hipMalloc(&h->d_Ainv, sizeof(double) * h->mc * h->mc) || hipMalloc(&h->d_cstat, sizeof(int)))
+    if (hipMalloc(&h->d_Ainv, sizeof(double) * h->mc * h->mc) || hipMalloc(&h->d_cstat, sizeof(int)) ||
+        hipMalloc(&h->d_cpiv, sizeof(double) * 4 * h->mc))
         return fail(HOMO3D_HIP_ERROR);
     if (hipDeviceSynchronize() != hipSuccess) return fail(HOMO3D_HIP_ERROR);
     *out = h;
@@ -839,7 +841,7 @@ int homo3d_destroy(homo3d_handle* h) {
         (void)hipFree(h->lv[l].r);
         (void)hipFree(h->lv[l].st);
     }
-    void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_ke16, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->d_cstat, h->X, h->F, h->P, h->Q, h->Z,
+    void* ptrs[] = {h->d_x, h->U64, h->d_ke64, h->d_kchi64, h->d_ke, h->d_ke16, h->d_kchi, h->d_G, h->d_w, h->d_stage, h->d_Ainv, h->d_cstat, h->d_cpiv, h->X, h->F, h->P, h->Q, h->Z,
                     h->Rr, h->d_partial, h->d_sc, h->d_tmp, h->d_ftmp, h->d_gperm, h->d_gflip};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -896,7 +898,7 @@ int homo3d_update_density(homo3d_handle* h, const float* x, int pitch, int on_de
     // solves that read the inverse)
     HIPCHK(h, hipEventRecord(h->ev_gal, h->stream));
     HIPCHK(h, hipStreamWaitEvent(h->cstream, h->ev_gal, 0));
-    launch_coarse_factor(h->cstream, Lc.n, static_cast<const float*>(Lc.st), h->d_Ainv, h->d_cstat);
+    launch_coarse_factor(h->cstream, Lc.n, static_cast<const float*>(Lc.st), h->d_Ainv, h->d_cpiv, h->d_cstat);
     HIPCHK(h, hipMemcpyAsync(h->h_cstat, h->d_cstat, sizeof(int), hipMemcpyDeviceToHost, h->cstream));
     HIPCHK(h, hipEventRecord(h->ev_coarse, h->cstream));
     h->coarse_pending = true;

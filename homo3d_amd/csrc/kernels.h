@@ -79,9 +79,10 @@ void launch_unpack_half(hipStream_t s, size_t N, const void* z, float* u);
 void launch_zero_pin64(hipStream_t s, size_t N, double* u);
 // The coarsest level's dense FP64 inverse on the GPU: assembled from its FP32 stencil (lexicographic
 // n_c^3 level, m = 3 n_c^3), symmetrised and inverted in place in ainv (m x m) by Gauss-Jordan
-// elimination on the diagonal.  status (device) = 0, or 1 + the first step whose pivot is not positive
-// (the system is not positive definite; ainv is then not an inverse).
-void launch_coarse_factor(hipStream_t s, int nc, const float* st, double* ainv, int* status);
+// elimination on the diagonal, one launch per step; piv is 4 m doubles of pivot rows and columns.
+// status (device) = 0, or 1 + the first step whose pivot is not positive (the system is not positive
+// definite; ainv is then not an inverse).
+void launch_coarse_factor(hipStream_t s, int nc, const float* st, double* ainv, double* piv, int* status);
 // Coarse stencils: FP32 [243][n^3] or, with *_half, packed FP16 [122][n^3] words (entries 2w, 2w + 1).
 // u FP32 planes, or (u_half, only where stencil_u_half_ok: six load cases per launch) packed FP16 pairs.
 int stencil_u_half_ok(int n, int lex);

@@ -2271,54 +2271,44 @@ __global__ __launch_bounds__(256) void k_symmetrize64(int m, double* __restrict_
 }
 
 // In-place inverse of the symmetric positive definite A by Gauss-Jordan elimination on the diagonal,
-// one workgroup: for each k, with p = A[k][k] (the k-th Schur complement's leading entry, positive
-// exactly when A is positive definite, the square of the Cholesky pivot),
+// one launch per step k over the whole matrix (one thread per element).  With p = A[k][k] (the k-th
+// Schur complement's leading entry: positive exactly when A is positive definite, the square of the
+// Cholesky pivot):
 //   A[k][j] /= p (j != k), A[k][k] = 1 / p, A[i][k] = -A[i][k] / p (i != k),
 //   A[i][j] -= A[i][k] A[k][j] / p (i, j != k).
-// The pivot row and column go to LDS; thread (g, c) then updates the elements (i, j) with
-// i = g mod G and j = c mod C (C = min(m, threads) consecutive columns, G = threads / C row groups),
-// all independent.  status = 0, or 1 + the first k whose pivot is not positive; every thread reads the
-// pivot after the barrier that ends step k - 1, so all take the same exit.
-__global__ __launch_bounds__(1024) void k_spd_inverse64(int m, double* __restrict__ A, int* __restrict__ status) {
-    extern __shared__ double sh[];
-    double* rowk = sh;      // new row k
-    double* colk = sh + m;  // old column k
-    const int C = m < (int)blockDim.x ? m : (int)blockDim.x, G = (int)blockDim.x / C;
-    const int g = (int)threadIdx.x / C, c = (int)threadIdx.x - g * C;
-    for (int k = 0; k < m; k++) {
-        const double piv = A[(size_t)k * m + k];
-        if (!(piv > 0.0)) {
-            if (threadIdx.x == 0) *status = 1 + k;
-            return;
-        }
-        const double inv = 1.0 / piv;
-        for (int j = threadIdx.x; j < m; j += blockDim.x) {
-            rowk[j] = (j == k ? 1.0 : A[(size_t)k * m + j]) * inv;
-            colk[j] = A[(size_t)j * m + k];
-        }
-        __syncthreads();
-        if (g < G)
-            for (int j = c; j < m; j += C) {
-                const double rj = rowk[j];
-                // eight rows per batch: their loads are issued together (the L2 latency, not the
-                // arithmetic, bounds a step)
-                for (int i0 = g; i0 < m; i0 += 8 * G) {
-                    double v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const int i = i0 + u * G;
-                        v[u] = i < m ? A[(size_t)i * m + j] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const int i = i0 + u * G;
-                        if (i < m) A[(size_t)i * m + j] = i == k ? rj : (j == k ? -colk[i] * inv : fma(-colk[i], rj, v[u]));
-                    }
-                }
-            }
-        __syncthreads();
+// Step k reads its pivot row and column from prow / pcol (their values before step k) and writes the
+// new row and column k + 1 to nrow / ncol for the next step, so no launch reads what it writes.
+// status = 0, or 1 + the first k whose pivot is not positive (every block reads the same pivot and
+// stops; later steps see the status and do nothing).
+__global__ __launch_bounds__(256) void k_gj_pivot0(int m, const double* __restrict__ A, double* __restrict__ prow,
+                                                   double* __restrict__ pcol, int* __restrict__ status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *status = 0;
+    if (t < m) {
+        prow[t] = A[t];
+        pcol[t] = A[(size_t)t * m];
     }
-    if (threadIdx.x == 0) *status = 0;
+}
+
+__global__ __launch_bounds__(256) void k_gj_step64(int m, int k, double* __restrict__ A,
+                                                   const double* __restrict__ prow, const double* __restrict__ pcol,
+                                                   double* __restrict__ nrow, double* __restrict__ ncol,
+                                                   int* __restrict__ status) {
+    if (*status != 0) return;
+    const double piv = prow[k];
+    if (!(piv > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *status = 1 + k;
+        return;
+    }
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)m * m) return;
+    const int i = (int)(t / m), j = (int)(t - (size_t)i * m);
+    const double inv = 1.0 / piv;
+    const double rj = (j == k ? 1.0 : prow[j]) * inv;
+    const double v = i == k ? rj : (j == k ? -pcol[i] * inv : fma(-pcol[i], rj, A[t]));
+    A[t] = v;
+    if (i == k + 1) nrow[j] = v;
+    if (j == k + 1) ncol[i] = v;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3489,13 +3479,19 @@ void launch_coarse_solve(hipStream_t s, int nc, const double* Ainv, const float*
     hipLaunchKernelGGL(k_coarse_solve, dim3(6), dim3(256), sizeof(double) * m, s, m, Nc, Ainv, f, u);
 }
 
-void launch_coarse_factor(hipStream_t s, int nc, const float* st, double* ainv, int* status) {
+void launch_coarse_factor(hipStream_t s, int nc, const float* st, double* ainv, double* piv, int* status) {
     const int m = 3 * nc * nc * nc;
     const size_t mm = (size_t)m * m;
     hipLaunchKernelGGL(k_fill64, dim3(nblk(mm) < 1024 ? nblk(mm) : 1024), dim3(256), 0, s, mm, 0.0, ainv);
     hipLaunchKernelGGL(k_coarse_assemble, dim3(nblk((size_t)m)), dim3(256), 0, s, nc, st, ainv);
     hipLaunchKernelGGL(k_symmetrize64, dim3(nblk(mm)), dim3(256), 0, s, m, ainv);
-    hipLaunchKernelGGL(k_spd_inverse64, dim3(1), dim3(1024), 2 * sizeof(double) * m, s, m, ainv, status);
+    // pivot rows and columns, double-buffered: piv = [row 0 | col 0 | row 1 | col 1], m values each
+    double* row[2] = {piv, piv + 2 * (size_t)m};
+    double* col[2] = {piv + m, piv + 3 * (size_t)m};
+    hipLaunchKernelGGL(k_gj_pivot0, dim3(nblk((size_t)m)), dim3(256), 0, s, m, ainv, row[0], col[0], status);
+    for (int k = 0; k < m; k++)
+        hipLaunchKernelGGL(k_gj_step64, dim3(nblk(mm)), dim3(256), 0, s, m, k, ainv, row[k & 1], col[k & 1],
+                           row[(k + 1) & 1], col[(k + 1) & 1], status);
 }
 
 void launch_dot6(hipStream_t s, size_t M, const float* a, const float* b, double* partial, int nblocks,
