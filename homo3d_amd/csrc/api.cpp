@@ -988,13 +988,22 @@ int homo3d_solve(homo3d_handle* h, double tol, int warm, homo3d_solve_stats* sta
             // restarts from zero (the residual is formed once more with its slice of U64 cleared).
             if (outer == 0 && warm && !warm_checked) {
                 warm_checked = true;
-                bool reset = false;
+                int reset = 0, nonzero = 0;
                 for (int r = 0; r < kRhs; r++)
-                    if (fn2[r] > 0 && !(rn2[r] < fn2[r])) {
-                        launch_zero_case64(h->stream, L0.N, r, h->U64);
-                        reset = true;
+                    if (fn2[r] > 0) {
+                        nonzero++;
+                        if (!(rn2[r] < fn2[r])) reset++;
                     }
-                if (reset) {
+                if (reset > 0 && reset == nonzero) {
+                    // every load case falls back (those with f = 0 have u = 0): the cold start, whose
+                    // first residual is f itself and whose first correction is copied into U64
+                    u_zero = true;
+                    outer = -1;
+                    continue;
+                }
+                if (reset > 0) {
+                    for (int r = 0; r < kRhs; r++)
+                        if (fn2[r] > 0 && !(rn2[r] < fn2[r])) launch_zero_case64(h->stream, L0.N, r, h->U64);
                     outer = -1;
                     continue;
                 }

@@ -144,6 +144,31 @@ def test_warm_start_after_level0_prolongation(h3d):
     assert rel(hom.elasticMatrix(solve=False), C0) < 1e-4
 
 
+def test_warm_start_worse_than_zero_falls_back(h3d):
+    """A warm start whose FP64 residual is not below |f| is dropped for that load case: with every load
+    case falling back the solve is the cold one (same CG steps, same C^H); with some of them, those
+    restart from zero while the others keep their start."""
+    say("warm-start fallback: n=32")
+    hom, _ = make(h3d, 32, solver="pcg", seed=5)
+    st0 = hom.solve(1e-5, warm=False)
+    C0 = hom.elasticMatrix(solve=False)
+    u = hom.get_solution64()
+    rng = np.random.default_rng(1)
+    bad = rng.standard_normal(u.shape) * 100.0 * np.abs(u).max()
+    hom.set_solution64(bad)
+    st1 = hom.solve(1e-5, warm=True)
+    assert st1["status"] == 0 and list(st1["iters"]) == list(st0["iters"]), (st0["iters"], st1["iters"])
+    assert rel(hom.elasticMatrix(solve=False), C0) < 1e-6
+    mixed = u.copy()
+    mixed[3:] = bad[3:]  # load cases 3..5 bad, 0..2 at the converged solution
+    hom.set_solution64(mixed)
+    st2 = hom.solve(1e-5, warm=True)
+    assert st2["status"] == 0
+    assert list(st2["iters"])[3:] == list(st0["iters"])[3:]
+    assert max(st2["iters"][:3]) == 0
+    assert rel(hom.elasticMatrix(solve=False), C0) < 1e-4
+
+
 def test_pitched_padding_is_left_alone(h3d):
     """Every entry point that takes a row-pitched array writes only the n live columns: the caller's
     padding (pitch > n) keeps its values, on host and on device arrays, after earlier calls have left
