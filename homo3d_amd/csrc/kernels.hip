@@ -1325,8 +1325,8 @@ __global__ __launch_bounds__(kFullThreads) __attribute__((amdgpu_waves_per_eu(4)
     double emin, const double* __restrict__ u, float* __restrict__ r32, double* __restrict__ partial) {
     constexpr int R = 6;
     __shared__ double sh[kFullThreads / 64];
-    __shared__ double ke_s[576];
-    stage_ke<double, false>(ke_s, ke);  // the neighbour loop below reads KE itself
+    __shared__ double ke_s[kKeStaged];
+    stage_ke(ke_s, ke);
     uint32_t v;
     const bool live = full_vertex(L, xcd_chunk(blockIdx.x, gridDim.x), v);
     double nrm[R];
@@ -1364,22 +1364,27 @@ __global__ __launch_bounds__(kFullThreads) __attribute__((amdgpu_waves_per_eu(4)
                     un[(2 * q) / 3][(2 * q) % 3] = w.x;
                     un[(2 * q + 1) / 3][(2 * q + 1) % 3] = w.y;
                 }
-                double S[9];
-#pragma unroll
-                for (int k = 0; k < 9; k++) S[k] = 0.0;
+                // the block from signed sums of the weights of the cells holding v and v + d
+                // (fine_block's form, with d known only at run time): cell e at offset
+                // (e&1, e>>1&1, e>>2) - 1 holds v + d iff e_m = (d_m + 1) / 2 wherever d_m != 0; the sign
+                // of axis m is -1 for e_m = 1 where d_m = 0
+                double w0 = 0.0, fxy = 0.0, fxz = 0.0, fyz = 0.0;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    // cell e at offset (e&1, e>>1&1, e>>2) - 1 holds v as node 7 - e and v + d as node b
-                    const int bx = dx + 1 - (e & 1), by = dy + 1 - ((e >> 1) & 1), bz = dz + 1 - (e >> 2);
-                    if (bx >= 0 && bx <= 1 && by >= 0 && by <= 1 && bz >= 0 && bz <= 1) {
-                        const int a = 7 - e, b = bx + 2 * by + 4 * bz;
-#pragma unroll
-                        for (int i = 0; i < 3; i++)
-#pragma unroll
-                            for (int j = 0; j < 3; j++)
-                                S[i * 3 + j] = fma(wd[e], ke_s[(3 * a + i) * 24 + 3 * b + j], S[i * 3 + j]);
+                    const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+                    if ((dx == 0 || ex == (dx + 1) / 2) && (dy == 0 || ey == (dy + 1) / 2) &&
+                        (dz == 0 || ez == (dz + 1) / 2)) {
+                        const double sx = (dx == 0 && ex) ? -1.0 : 1.0, sy = (dy == 0 && ey) ? -1.0 : 1.0,
+                                     sz = (dz == 0 && ez) ? -1.0 : 1.0;
+                        w0 += wd[e];
+                        fxy = fma(sx * sy, wd[e], fxy);
+                        fxz = fma(sx * sz, wd[e], fxz);
+                        fyz = fma(sy * sz, wd[e], fyz);
                     }
                 }
+                const double* cb = ke_s + kBlockConsts + 9 * d;
+                const double S[9] = {cb[0] * w0,  cb[1] * fxy, cb[2] * fxz, cb[3] * fxy, cb[4] * w0,
+                                     cb[5] * fyz, cb[6] * fxz, cb[7] * fyz, cb[8] * w0};
 #pragma unroll
                 for (int r = 0; r < R; r++)
 #pragma unroll
