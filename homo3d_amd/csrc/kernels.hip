@@ -2136,6 +2136,10 @@ __global__ void k_galerkin_fine_pin(Lay Lf, Lay Lc, const float* __restrict__ ke
     float S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 64; c++) {
         const int ox = (c & 3) - 2, oy = ((c >> 2) & 3) - 2, oz = (c >> 4) - 2;  // cell offset from 2V
+        // only the cells in the supports of both V and V + D contribute (as in k_galerkin_fine)
+        if ((Dx > 0 && ox < 0) || (Dx < 0 && ox >= 0) || (Dy > 0 && oy < 0) || (Dy < 0 && oy >= 0) ||
+            (Dz > 0 && oz < 0) || (Dz < 0 && oz >= 0))
+            continue;
         int cx = 2 * X + ox, cy = 2 * Y + oy, cz = 2 * Z + oz;
         const int cxw = (cx % nf + nf) % nf, cyw = (cy % nf + nf) % nf, czw = (cz % nf + nf) % nf;
         const float we = w[Lf.idx(cxw, cyw, czw)];
