@@ -2211,7 +2211,10 @@ __global__ __launch_bounds__(256) void k_galerkin_coarse(Lay Lf, Lay Lc, const S
             for (int k = 0; k < 9; k++) e[k] = StencilIO<SI>::get(rf, vf * 4u, (uint32_t)(Nf * 4), d * 9 + k);
 #pragma unroll
             for (int j = 0; j < 9; j++) {
+                // o and d are the same for the whole wave, so is wj: about two of the nine coarse
+                // directions (Dx, Dy) get a non-zero weight from a fine pair
                 const float wj = wz * hat(ky - 2 * (j / 3 - 1)) * hat(kx - 2 * (j % 3 - 1));
+                if (wj == 0.f) continue;
 #pragma unroll
                 for (int k = 0; k < 9; k++) S[j][k] = fmaf(wj, e[k], S[j][k]);
             }
