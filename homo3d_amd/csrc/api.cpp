@@ -572,7 +572,6 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     *x_pending_out = false;
     const Level& L0 = h->lv[0];
     const size_t M = 3 * L0.N;
-    const size_t vb = sizeof(float) * kRhs * M;
     double sc[6 * kRhs];
     bool act[kRhs];
     bool any = false;
@@ -648,8 +647,6 @@ int pcg_inner(homo3d_handle* h, const double* r0n2, const double* tol, const boo
     *x_pending_out = x_pending;
     return HOMO3D_OK;
 }
-
-bool check_handle(homo3d_handle* h) { return h != nullptr; }
 
 // on_device < 0: classify the pointer (device or managed memory -> 1, anything else -> 0)
 int resolve_on_device(const void* p, int on_device) {

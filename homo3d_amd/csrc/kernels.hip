@@ -95,10 +95,6 @@ template <>
 __device__ __forceinline__ float bload<float>(Rsrc r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
-template <>
-__device__ __forceinline__ double bload<double>(Rsrc r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
-}
 
 // Packed FP16 storage of a 6-load-case field: word plane (p * 3 + i) holds components i of load cases
 // 2p (.x) and 2p + 1 (.y), so a neighbour of all six load cases is 9 32-bit words.
@@ -2881,7 +2877,7 @@ __global__ __launch_bounds__(256) void k_absmax(long N, int n, int pitch, const 
     sh[threadIdx.x] = m;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
@@ -3683,7 +3679,7 @@ __global__ __launch_bounds__(256) void k_net_target_mean(Lay L, int g, const dou
     sh[1][threadIdx.x] = ws;
     __syncthreads();
     for (int t = 128; t > 0; t >>= 1) {
-        if (threadIdx.x < t) {
+        if ((int)threadIdx.x < t) {
             sh[0][threadIdx.x] += sh[0][threadIdx.x + t];
             sh[1][threadIdx.x] += sh[1][threadIdx.x + t];
         }
